@@ -1,0 +1,73 @@
+// RGF.hpp -- the reference's class interface over the C ABI of librgf_hip.so.
+//
+// rjanalik/RGF exposes `template<class T> class RGF` (RGF.H:11-80).  This class
+// keeps those method names and signatures for T = double so an existing caller
+// (e.g. main.C:86-113) compiles unchanged; each method forwards to the C entry
+// point that replaces it and throws rgf_error on a non-OK status (the reference
+// ignores LAPACK info and exits on allocation failure).
+#pragma once
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+
+#include "rgf.h"
+
+class rgf_error : public std::runtime_error {
+public:
+    rgf_error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+private:
+    int code_;
+};
+
+template <class T>
+class RGF;
+
+template <>
+class RGF<double> {
+public:
+    // RGF.H:16 / RGF.H:85-146
+    RGF(size_t* ia, size_t* ja, double* a, size_t ns, size_t nt, size_t nd,
+        const rgf_options* opts = nullptr)
+    {
+        check(rgf_create_csc(ia, ja, a, ns, nt, nd, opts, &h_), nullptr);
+        n_ = ns * nt + nd;
+    }
+    ~RGF() { if (h_) rgf_destroy(h_); }                                     // RGF.H:151-170
+    RGF(const RGF&) = delete;
+    RGF& operator=(const RGF&) = delete;
+
+    void factorize() { check(rgf_factorize(h_), h_); }                     // RGF.H:510-546
+    void solve(double* b, size_t nrhs) { check(rgf_solve(h_, b, b, nrhs), h_); }   // RGF.H:550-554
+    void solve(double* x, double* b, size_t nrhs) { check(rgf_solve(h_, x, b, nrhs), h_); }  // RGF.H:558-578
+    void RGFdiag(double* diag) { check(rgf_selinv_diag(h_, diag), h_); }   // RGF.H:582-612
+    double logDet()                                                         // RGF.H:616-627
+    {
+        double v = 0.0;
+        check(rgf_logdet(h_, &v), h_);
+        return v;
+    }
+    double residualNorm(double* x, double* b)                               // RGF.H:631-655
+    {
+        double v = 0.0;
+        check(rgf_residual_norm(h_, x, b, &v), h_);
+        return v;
+    }
+    double residualNormNormalized(double* x, double* b)                     // RGF.H:659-663
+    {
+        double v = 0.0;
+        check(rgf_residual_norm_normalized(h_, x, b, &v), h_);
+        return v;
+    }
+
+    rgf_handle_t handle() const { return h_; }
+
+private:
+    static void check(int rc, rgf_handle_t h)
+    {
+        if (rc != RGF_OK) throw rgf_error(rc, rgf_last_error(h));
+    }
+    rgf_handle_t h_ = nullptr;
+    size_t n_ = 0;
+};

@@ -1,0 +1,135 @@
+/*
+ * rgf.h -- C ABI of librgf_hip.so, an MI355X (gfx950) solver for symmetric
+ * positive-definite block-tridiagonal-with-arrowhead matrices: nt diagonal
+ * blocks of size ns x ns, nt-1 sub-diagonal coupling blocks and nd dense arrow
+ * rows/columns (n = ns*nt + nd).
+ *
+ * Each entry point replaces a member of rjanalik/RGF's `template<class T> class
+ * RGF` (RGF.H:11-80) for T = double; the C++ wrapper with the reference's own
+ * method names is RGF.hpp next to this file.  All status codes are rgf_status.
+ * Handles are independent; one handle must not be used by two threads at once.
+ */
+#ifndef RGF_H
+#define RGF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rgf_handle_s *rgf_handle_t;
+
+typedef enum {
+    RGF_OK = 0,
+    RGF_EINVAL = 1,   /* bad shape, bad CSC pattern, bad argument */
+    RGF_ENOTSPD = 2,  /* a pivot of the block Cholesky was not positive */
+    RGF_ENOMEM = 3,   /* device allocation failed */
+    RGF_EDEVICE = 4,  /* HIP runtime error */
+    RGF_ESTATE = 5    /* call not valid in this state (e.g. no resident matrix) */
+} rgf_status;
+
+typedef enum {
+    RGF_FP64 = 0,          /* every product in FP64 on MFMA */
+    RGF_MIXED_FP32 = 1,    /* products in FP32 MFMA, FP64 potrf / accumulation / refinement */
+    RGF_MIXED_BF16X3 = 2   /* products as split-BF16 (3 terms) on MFMA, FP64 elsewhere */
+} rgf_precision;
+
+typedef enum {
+    RGF_SELINV_EXACT = 0,     /* full Takahashi recursion (exact diag(A^-1) for every nd) */
+    RGF_SELINV_REFERENCE = 1  /* RGF.H:476-503 as written: drops the arrow cross terms */
+} rgf_selinv_mode;
+
+typedef struct {
+    int precision;      /* rgf_precision */
+    int selinv_mode;    /* rgf_selinv_mode */
+    int keep_matrix;    /* 1: keep A resident beside the factor (residual, refinement, re-factorize) */
+    int device;         /* HIP device ordinal */
+    int refine_iters;   /* max FP64 iterative-refinement steps of a mixed-precision solve */
+    double refine_tol;  /* stop refining when ||b - A x|| / ||b|| <= refine_tol */
+} rgf_options;
+
+typedef struct {
+    uint64_t seed;
+    uint32_t matrix_id;
+    double sigma_c;     /* std-dev of the coupling blocks (0.1) */
+    double sigma_a;     /* std-dev of the arrow rows (0.05) */
+    double theta_d;     /* scale of the diagonal blocks (1) */
+    double theta_c;     /* scale of the coupling blocks (1) */
+} rgf_gen_params;
+
+typedef struct {
+    size_t ns, nt, nd, n;
+    size_t device_bytes;        /* bytes this handle holds on the device */
+    double flops_factor;        /* canonical algorithmic flops: factorize */
+    double flops_selinv;        /* canonical algorithmic flops: selected inversion */
+    double bytes_solve_per_rhs; /* canonical bytes streamed by one solve (forward + backward) */
+} rgf_info_t;
+
+void rgf_default_options(rgf_options *opts);
+void rgf_default_gen_params(rgf_gen_params *p);
+
+/* RGF<T>::RGF(size_t* ia, size_t* ja, T* a, size_t ns, size_t nt, size_t nd), RGF.H:85-146.
+ * Lower-triangle CSC, 0-based, entries confined to the block-arrow pattern (checked).
+ * Unlike the reference the arrays are copied to the device at construction and
+ * need not outlive the handle.  Requires nd <= ns and nd <= 64. */
+int rgf_create_csc(const size_t *ia, const size_t *ja, const double *a, size_t ns, size_t nt, size_t nd,
+                   const rgf_options *opts, rgf_handle_t *out);
+
+/* Extension: the matrix is generated on the device from (seed, matrix id); the
+ * definition is in rgf_gen.hip (benchmark workloads). */
+int rgf_create_generated(size_t ns, size_t nt, size_t nd, const rgf_gen_params *gen,
+                         const rgf_options *opts, rgf_handle_t *out);
+
+/* RGF<T>::~RGF(), RGF.H:151-170 */
+int rgf_destroy(rgf_handle_t h);
+
+/* RGF<T>::factorize(), RGF.H:510-546.  RGF_ENOTSPD carries the 1-based column in
+ * rgf_last_error(). */
+int rgf_factorize(rgf_handle_t h);
+
+/* RGF<T>::solve(T* x, T* b, size_t nrhs), RGF.H:558-578 (and solve(b, nrhs),
+ * RGF.H:550-554, with x == b).  Host arrays, column-major n x nrhs, ld = n.
+ * Factorizes first if needed. */
+int rgf_solve(rgf_handle_t h, double *x, const double *b, size_t nrhs);
+
+/* RGF<T>::logDet(), RGF.H:616-627. */
+int rgf_logdet(rgf_handle_t h, double *out);
+
+/* RGF<T>::RGFdiag(T* diag), RGF.H:582-612: diag(A^-1), length n, host array.
+ * The factor stays valid afterwards (the reference re-factorizes, RGF.H:588). */
+int rgf_selinv_diag(rgf_handle_t h, double *diag);
+
+/* RGF<T>::residualNorm(T* x, T* b), RGF.H:631-655: ||b - A x||_2 (needs keep_matrix). */
+int rgf_residual_norm(rgf_handle_t h, const double *x, const double *b, double *out);
+
+/* RGF<T>::residualNormNormalized(T* x, T* b), RGF.H:659-663. */
+int rgf_residual_norm_normalized(rgf_handle_t h, const double *x, const double *b, double *out);
+
+/* Message of the last failed call on h (or of the last failed create when h is NULL). */
+const char *rgf_last_error(rgf_handle_t h);
+
+/* ---- device-resident extensions (pointers are device memory of the handle's device) ---- */
+int rgf_solve_device(rgf_handle_t h, double *x_dev, const double *b_dev, size_t nrhs);
+int rgf_selinv_diag_device(rgf_handle_t h, double *diag_dev);
+int rgf_synchronize(rgf_handle_t h);
+int rgf_get_info(rgf_handle_t h, rgf_info_t *info);
+
+/* A in the reference's MF layout (RGF.H:95 doubles; lower triangles of the diagonal
+ * blocks and of the corner), for parity of the logical blocks.  Needs keep_matrix. */
+int rgf_export_matrix_mf(rgf_handle_t h, double *mf);
+/* The factor in MF layout; the diagonal blocks and the corner hold L_ii^-1 and L_dd^-1
+ * (the representation kept on the device), the couplings B_i and arrows D_i as in L. */
+int rgf_export_factor_mf(rgf_handle_t h, double *mf);
+
+/* Per-process GEMM kernel timing with HIP events on the launching stream:
+ * op_class 0 = NN, 1 = NT, 2 = TN, 3 = TT, -1 = off.  read() synchronizes and
+ * returns the summed kernel time (ms), the summed canonical flops and the launch count. */
+int rgf_kernel_timing_enable(int op_class);
+int rgf_kernel_timing_read(double *ms, double *flops, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RGF_H */

@@ -1,0 +1,28 @@
+/*
+ * rgf_testing.h -- diagnostic entry points of librgf_hip.so used by the test
+ * suite.  They have no counterpart in rjanalik/RGF: they expose single kernels
+ * of the sweep so a failure can be localised to one kernel.
+ */
+#ifndef RGF_TESTING_H
+#define RGF_TESTING_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C = alpha*op(A)*op(B) + beta*C on the device FP64 MFMA GEMM kernel.  Host
+ * arrays, column-major; ta/tb: 0 = N, 1 = T; flags as the kernel's GemmFlags
+ * (1 lower-C stores, 2/4 op(A) lower/upper, 8/16 op(B) lower/upper).
+ * A is (ta ? k x m : m x k) with ld lda, B is (tb ? n x k : k x n) with ld ldb. */
+int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+                   const double *B, int ldb, double beta, double *C, int ldc, int flags);
+
+/* In-place Cholesky (lower) of an n x n host matrix on the device, followed by
+ * L^-1 in place when invert != 0; returns the log-determinant of the input in
+ * *logdet and RGF_ENOTSPD for a non-positive pivot. */
+int rgf_debug_potrf(int n, double *A, int lda, int invert, double *logdet);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""rgf_amd -- MI355X-native RGF solver (block-tridiagonal-with-arrowhead SPD matrices).
+
+The compute path is librgf_hip.so (hand-written gfx950 HIP kernels behind the C
+ABI of include/rgf.h).  This module is the Python face of the reference's
+``template<class T> class RGF`` (rjanalik/RGF, RGF.H:11-80): the class below
+keeps the reference's method names and argument meaning so that callers and
+tests read like the reference's own driver (main.C:86-121).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (RGF_EDEVICE, RGF_EINVAL, RGF_ENOMEM, RGF_ENOTSPD, RGF_ESTATE, RGF_FP64,
+                   RGF_MIXED_BF16X3, RGF_MIXED_FP32, RGF_OK, RGF_SELINV_EXACT, RGF_SELINV_REFERENCE,
+                   DeviceBuffer, GenParams, Info, Options, gpu_count)
+
+__all__ = ["RGF", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
+           "default_gen_params", "gpu_count", "RGF_FP64", "RGF_MIXED_FP32", "RGF_MIXED_BF16X3",
+           "RGF_SELINV_EXACT", "RGF_SELINV_REFERENCE", "RGF_OK", "RGF_EINVAL", "RGF_ENOTSPD",
+           "RGF_ENOMEM", "RGF_EDEVICE", "RGF_ESTATE", "mf_size", "canonical_flops"]
+
+
+class RGFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_lib.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _check(rc: int, h) -> None:
+    if rc != RGF_OK:
+        msg = _lib.load().rgf_last_error(h)
+        raise RGFError(rc, msg.decode() if msg else "")
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    _lib.load().rgf_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def default_gen_params(**kw) -> GenParams:
+    g = GenParams()
+    _lib.load().rgf_default_gen_params(C.byref(g))
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def mf_size(ns: int, nt: int, nd: int) -> int:
+    """Length of the reference's packed MF array (RGF.H:95)."""
+    return 2 * (nt - 1) * ns * ns + ns * ns + (ns * nt + nd) * nd
+
+
+def canonical_flops(ns: int, nt: int, nd: int) -> tuple[float, float]:
+    """Canonical algorithmic flops (F_alg, S_alg) of factorize and selected inversion:
+    per block potrf ns^3/3, trsm ns^3, syrk ns^3 (factor); trtri + lauum 2ns^3/3,
+    trmm ns^3, gemm 2ns^3, gemmt ns^3 (selected inversion); plus the arrow terms."""
+    P = ns ** 3 / 3.0
+    Pd = nd ** 3 / 3.0
+    F = nt * P + (nt - 1) * 2 * ns ** 3 + nd * (nt * ns ** 2 + 2 * (nt - 1) * ns ** 2 + nt * nd * ns) + Pd
+    S = nt * 2 * P + (nt - 1) * 4 * ns ** 3 + nd * (nt * ns ** 2 + 4 * (nt - 1) * ns ** 2) + 2 * Pd
+    return float(F), float(S)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class RGF:
+    """RGF<double> of the reference (RGF.H:11-80) on one MI355X.
+
+    ``RGF(ia, ja, a, ns, nt, nd)`` takes the lower-triangle CSC of main.C:50-70
+    (0-based, size_t indices); ``RGF.generated(ns, nt, nd, seed=...)`` builds the
+    benchmark matrix on the device instead.
+    """
+
+    def __init__(self, ia, ja, a, ns: int, nt: int, nd: int, options: Options | None = None, *,
+                 _handle=None):
+        lib = _lib.load()
+        self.ns, self.nt, self.nd = int(ns), int(nt), int(nd)
+        self.n = self.ns * self.nt + self.nd
+        self._h = C.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+            return
+        ia = np.ascontiguousarray(ia, dtype=np.uint64)
+        ja = np.ascontiguousarray(ja, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if ia.size != self.n + 1:
+            raise RGFError(RGF_EINVAL, f"ia has {ia.size} entries, expected n+1 = {self.n + 1}")
+        szp = C.POINTER(C.c_size_t)
+        rc = lib.rgf_create_csc(ia.ctypes.data_as(szp), ja.ctypes.data_as(szp), _dptr(a), self.ns, self.nt,
+                                self.nd, C.byref(options) if options is not None else None, C.byref(self._h))
+        _check(rc, None)
+
+    @classmethod
+    def generated(cls, ns: int, nt: int, nd: int, seed: int = 0, matrix_id: int = 0,
+                  options: Options | None = None, **gen) -> "RGF":
+        lib = _lib.load()
+        g = default_gen_params(seed=seed, matrix_id=matrix_id, **gen)
+        h = C.c_void_p()
+        rc = lib.rgf_create_generated(ns, nt, nd, C.byref(g), C.byref(options) if options is not None else None,
+                                      C.byref(h))
+        _check(rc, None)
+        return cls(None, None, None, ns, nt, nd, _handle=h)
+
+    # ------------------------------------------------------------ reference API
+    def factorize(self) -> None:                      # RGF.H:510-546
+        _check(_lib.load().rgf_factorize(self._h), self._h)
+
+    def solve(self, x, b=None, nrhs: int | None = None):
+        """solve(b, nrhs) in place (RGF.H:550-554) or solve(x, b, nrhs) (RGF.H:558-578).
+        Arrays are column-major n x nrhs (a 2-D numpy array is taken as n x nrhs)."""
+        if b is None or isinstance(b, (int, np.integer)):
+            if isinstance(b, (int, np.integer)):
+                nrhs = int(b)
+            b = x
+        bb = np.asfortranarray(b, dtype=np.float64)
+        if nrhs is None:
+            nrhs = 1 if bb.ndim == 1 else bb.shape[1]
+        if bb.size != self.n * nrhs:
+            raise RGFError(RGF_EINVAL, f"rhs has {bb.size} values, expected n*nrhs = {self.n * nrhs}")
+        out = np.empty_like(bb)
+        _check(_lib.load().rgf_solve(self._h, _dptr(out.reshape(-1, order="F")), _dptr(bb.reshape(-1, order="F")),
+                                     nrhs), self._h)
+        if x is not None and isinstance(x, np.ndarray) and x.flags.writeable and x.size == out.size:
+            x.reshape(-1, order="F")[:] = out.reshape(-1, order="F")
+        return out
+
+    def RGFdiag(self, diag: np.ndarray | None = None) -> np.ndarray:   # RGF.H:582-612
+        out = np.empty(self.n) if diag is None else diag
+        assert out.dtype == np.float64 and out.flags.c_contiguous and out.size == self.n
+        _check(_lib.load().rgf_selinv_diag(self._h, _dptr(out)), self._h)
+        return out
+
+    def logDet(self) -> float:                        # RGF.H:616-627
+        v = C.c_double()
+        _check(_lib.load().rgf_logdet(self._h, C.byref(v)), self._h)
+        return v.value
+
+    def residualNorm(self, x, b) -> float:            # RGF.H:631-655
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        v = C.c_double()
+        _check(_lib.load().rgf_residual_norm(self._h, _dptr(xx), _dptr(bb), C.byref(v)), self._h)
+        return v.value
+
+    def residualNormNormalized(self, x, b) -> float:  # RGF.H:659-663
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        v = C.c_double()
+        _check(_lib.load().rgf_residual_norm_normalized(self._h, _dptr(xx), _dptr(bb), C.byref(v)), self._h)
+        return v.value
+
+    # ------------------------------------------------------------ extensions
+    def solve_device(self, x: DeviceBuffer, b: DeviceBuffer, nrhs: int) -> None:
+        _check(_lib.load().rgf_solve_device(self._h, x.ptr, b.ptr, nrhs), self._h)
+
+    def selinv_diag_device(self, diag: DeviceBuffer) -> None:
+        _check(_lib.load().rgf_selinv_diag_device(self._h, diag.ptr), self._h)
+
+    def synchronize(self) -> None:
+        _check(_lib.load().rgf_synchronize(self._h), self._h)
+
+    def info(self) -> Info:
+        i = Info()
+        _check(_lib.load().rgf_get_info(self._h, C.byref(i)), self._h)
+        return i
+
+    def export_matrix_mf(self) -> np.ndarray:
+        out = np.empty(mf_size(self.ns, self.nt, self.nd))
+        _check(_lib.load().rgf_export_matrix_mf(self._h, _dptr(out)), self._h)
+        return out
+
+    def export_factor_mf(self) -> np.ndarray:
+        out = np.empty(mf_size(self.ns, self.nt, self.nd))
+        _check(_lib.load().rgf_export_factor_mf(self._h, _dptr(out)), self._h)
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            _lib.load().rgf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def kernel_timing_enable(op_class: int) -> None:
+    _lib.load().rgf_kernel_timing_enable(op_class)
+
+
+def kernel_timing_read() -> tuple[float, float, int]:
+    ms, fl, n = C.c_double(), C.c_double(), C.c_longlong()
+    rc = _lib.load().rgf_kernel_timing_read(C.byref(ms), C.byref(fl), C.byref(n))
+    if rc != 0:
+        raise RGFError(rc, "kernel timing read failed")
+    return ms.value, fl.value, n.value

@@ -1,0 +1,849 @@
+// C ABI of librgf_hip.so: handle lifetime, the FP64 block sweep (factorize,
+// logdet, solve, selected inversion) and the residual, all device-resident.
+//
+// Device layout of one matrix (all FP64, column-major, 256-byte aligned arrays):
+//   panel t (t = 0..nt-1), ld = ldp = roundup(2ns+nd, 8), ns columns:
+//       rows [0, ns)        diagonal block  (A: full symmetric;  factor: L_tt^-1, lower)
+//       rows [ns, 2ns)      coupling block A_{t+1,t}            (factor: B_t = L_{t+1,t})
+//       rows [2ns, 2ns+nd)  arrow block A_{d,t}                 (factor: D_t = L_{d,t})
+//   corner nd x nd (ld nd)  (A: full symmetric; factor: L_dd and X_dd = L_dd^-1)
+// i.e. the reference's supernodes (RGF.H:114-138) with a uniform, aligned
+// leading dimension, kept resident for the handle's lifetime instead of being
+// streamed to host memory after every block (RGF.H:263, 290, 321, 704-728).
+#include "rgf_internal.hpp"
+#include "../../include/rgf.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace rgf {
+
+static thread_local std::string g_create_error;
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file,
+             line);
+    g_create_error = buf;
+    return (int)e;
+}
+
+static inline int r8(long long v) { return (int)((v + 7) / 8 * 8); }
+
+}  // namespace rgf
+
+using namespace rgf;
+
+struct rgf_handle_s {
+    int ns = 0, nt = 0, nd = 0;
+    long long n = 0;
+    rgf_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ldp = 0;                // panel leading dimension
+    long long pstride = 0;      // elements between panels
+    int T = 0;                  // 64-wide tiles per diagonal block
+    int ldf = 0;                // ld of Ftmp / W / Z / S buffers
+    // device buffers
+    double* Ap = nullptr;       // resident matrix panels (keep_matrix)
+    double* Add = nullptr;      // resident matrix corner
+    double* Lp = nullptr;       // factor panels
+    double* Ldd = nullptr;      // corner L_dd
+    double* Xdd = nullptr;      // corner X_dd = L_dd^-1
+    double* Ftmp = nullptr;     // (ns+nd) x ns staging of the rows below a diagonal block
+    double* ttmp = nullptr;     // trtri merge scratch
+    double* parts = nullptr;    // per-tile log-diagonal sums: nt*T + 1
+    double* cpart = nullptr;    // corner partials nt * nd * nd
+    int* dinfo = nullptr;       // first non-positive pivot (INT_MAX = none)
+    // selected inversion
+    double* Sb[2] = {nullptr, nullptr};
+    double* W = nullptr;
+    double* Z = nullptr;
+    double* diag = nullptr;
+    // solve
+    double* Y = nullptr;        // n x 16
+    double* X = nullptr;        // n x 16
+    double* Bv = nullptr;       // n x 16
+    double* work = nullptr;     // skinny split-k partials
+    double* scal = nullptr;     // small device scalars
+    bool factorized = false;
+    bool broken = false;        // in-place factorization failed: A is gone
+    size_t bytes = 0;
+    std::string err;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int set_err(rgf_handle_s* h, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf; else g_create_error = buf;
+    return code;
+}
+
+int dev_status(rgf_handle_s* h, int hip_rc, const char* where)
+{
+    if (hip_rc == 0) return RGF_OK;
+    return set_err(h, hip_rc == (int)hipErrorOutOfMemory ? RGF_ENOMEM : RGF_EDEVICE, "%s: HIP error %d (%s)",
+                   where, hip_rc, hipGetErrorString((hipError_t)hip_rc));
+}
+
+template <class T>
+int dalloc(rgf_handle_s* h, T** p, size_t count)
+{
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(h, RGF_ENOMEM, "hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+    }
+    h->allocs.push_back(q);
+    h->bytes += bytes;
+    *p = (T*)q;
+    return RGF_OK;
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+int validate_shape(size_t ns, size_t nt, size_t nd)
+{
+    if (ns == 0 || nt == 0) return set_err(nullptr, RGF_EINVAL, "ns and nt must be positive (ns=%zu nt=%zu)", ns, nt);
+    if (nd > ns) return set_err(nullptr, RGF_EINVAL, "nd=%zu > ns=%zu is not supported (the reference's block stride needs nd <= ns)", nd, ns);
+    if (nd > 64) return set_err(nullptr, RGF_EINVAL, "nd=%zu > 64 is not supported", nd);
+    if (2 * ns + nd > 65536 || ns * nt + nd > (size_t)INT_MAX / 16)
+        return set_err(nullptr, RGF_EINVAL, "matrix too large (ns=%zu nt=%zu nd=%zu)", ns, nt, nd);
+    return RGF_OK;
+}
+
+int alloc_core(rgf_handle_s* h)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd;
+    h->ldp = r8(2LL * ns + nd);
+    h->pstride = (long long)h->ldp * ns;
+    h->T = (ns + 63) / 64;
+    h->ldf = r8((long long)ns + nd);
+    int rc;
+    if (h->opt.keep_matrix) {
+        if ((rc = dalloc(h, &h->Ap, (size_t)h->pstride * nt))) return rc;
+        if ((rc = dalloc(h, &h->Add, (size_t)std::max(nd * nd, 1)))) return rc;
+    }
+    if ((rc = dalloc(h, &h->Lp, (size_t)h->pstride * nt))) return rc;
+    if ((rc = dalloc(h, &h->Ldd, (size_t)std::max(nd * nd, 1)))) return rc;
+    if ((rc = dalloc(h, &h->Xdd, (size_t)std::max(nd * nd, 1)))) return rc;
+    if ((rc = dalloc(h, &h->Ftmp, (size_t)h->ldf * ns))) return rc;
+    // trtri merge scratch: max over levels of (groups * (s*64)^2)
+    size_t tt = 1;
+    for (int sz = 1; sz < h->T; sz *= 2) {
+        size_t G = (size_t)((h->T - sz + 2 * sz - 1) / (2 * sz));
+        tt = std::max(tt, G * (size_t)(sz * 64) * (size_t)(sz * 64));
+    }
+    if ((rc = dalloc(h, &h->ttmp, tt))) return rc;
+    if ((rc = dalloc(h, &h->parts, (size_t)nt * h->T + 1))) return rc;
+    if ((rc = dalloc(h, &h->cpart, (size_t)std::max(nt * nd * nd, 1)))) return rc;
+    if ((rc = dalloc(h, &h->dinfo, 1))) return rc;
+    if ((rc = dalloc(h, &h->scal, 64))) return rc;
+    return RGF_OK;
+}
+
+int ensure_solve_buffers(rgf_handle_s* h)
+{
+    if (h->Y) return RGF_OK;
+    int rc;
+    const size_t n16 = (size_t)h->n * 16;
+    if ((rc = dalloc(h, &h->Y, n16))) return rc;
+    if ((rc = dalloc(h, &h->X, n16))) return rc;
+    if ((rc = dalloc(h, &h->Bv, n16))) return rc;
+    const int mx = std::max<int>((int)h->n, 2 * h->ns + h->nd);
+    if ((rc = dalloc(h, &h->work, skinny_work_elems(mx, mx, 16)))) return rc;
+    return RGF_OK;
+}
+
+int ensure_selinv_buffers(rgf_handle_s* h)
+{
+    if (h->W) return RGF_OK;
+    int rc;
+    const int m = h->ns + h->nd;
+    if ((rc = dalloc(h, &h->Sb[0], (size_t)h->ldf * m))) return rc;
+    if ((rc = dalloc(h, &h->Sb[1], (size_t)h->ldf * m))) return rc;
+    if ((rc = dalloc(h, &h->W, (size_t)h->ldf * h->ns))) return rc;
+    if ((rc = dalloc(h, &h->Z, (size_t)h->ldf * h->ns))) return rc;
+    if ((rc = dalloc(h, &h->diag, (size_t)h->n))) return rc;
+    return RGF_OK;
+}
+
+rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, const rgf_options* opts, int* rc)
+{
+    *rc = validate_shape(ns, nt, nd);
+    if (*rc) return nullptr;
+    auto* h = new rgf_handle_s();
+    rgf_default_options(&h->opt);
+    if (opts) h->opt = *opts;
+    h->ns = (int)ns; h->nt = (int)nt; h->nd = (int)nd;
+    h->n = (long long)ns * nt + nd;
+    h->device = h->opt.device;
+    if (h->opt.precision != RGF_FP64) {
+        *rc = set_err(nullptr, RGF_EINVAL, "precision mode %d is not available in this build", h->opt.precision);
+        delete h;
+        return nullptr;
+    }
+    hipError_t e = hipSetDevice(h->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        *rc = set_err(nullptr, RGF_EDEVICE, "no usable HIP device %d (%s)", h->device, hipGetErrorString(e));
+        delete h;
+        return nullptr;
+    }
+    *rc = alloc_core(h);
+    if (*rc) {
+        g_create_error = h->err;
+        rgf_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+// pointers into the resident matrix (A) storage: Ap if kept, else the factor panels
+double* a_panel(rgf_handle_s* h, int t) { return (h->Ap ? h->Ap : h->Lp) + (long long)t * h->pstride; }
+double* a_corner(rgf_handle_s* h) { return h->Add ? h->Add : h->Ldd; }
+double* l_panel(rgf_handle_s* h, int t) { return h->Lp + (long long)t * h->pstride; }
+
+// ------------------------------------------------------------------ factorize
+// FirstStageFactor (RGF.H:233-340) with every block resident:
+//   S_i = A_ii - B_{i-1} B_{i-1}^T ; A_di -= D_{i-1} B_{i-1}^T      (one GEMM launch)
+//   L_ii = chol(S_i)   (log-diagonal summed in the panel kernel)
+//   X_i  = L_ii^-1     (in place)
+//   [B_i; D_i] = [A_{i+1,i}; A_di] X_i^T                             (one GEMM launch)
+//   S_dd = A_dd - sum_i D_i D_i^T ; L_dd = chol(S_dd) ; X_dd = L_dd^-1
+int factor_sweep(rgf_handle_s* h)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, ldf = h->ldf;
+    hipStream_t s = h->stream;
+    int rc;
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, h->dinfo, INT_MAX);
+    if ((rc = (int)hipGetLastError())) return rc;
+    for (int i = 0; i < nt; i++) {
+        double* P = l_panel(h, i);
+        double* A = a_panel(h, i);
+        if (i == 0) {
+            if (h->Ap && (rc = copy2d(A, ldp, P, ldp, ns, ns, 1.0, s))) return rc;
+            if (nd > 0 && (rc = copy2d(A + 2 * ns, ldp, h->Ftmp + ns, ldf, nd, ns, 1.0, s))) return rc;
+        } else {
+            const double* Pm = l_panel(h, i - 1);
+            GemmDesc d[2];
+            d[0] = gdesc(Pm + ns, ldp, Pm + ns, ldp, P, ldp, ns, ns, ns, -1.0, 1.0, G_CLOWER);
+            d[0].Cin = A; d[0].ldcin = ldp;
+            int ndesc = 1;
+            if (nd > 0) {
+                d[1] = gdesc(Pm + 2 * ns, ldp, Pm + ns, ldp, h->Ftmp + ns, ldf, nd, ns, ns, -1.0, 1.0, 0);
+                d[1].Cin = A + 2 * ns; d[1].ldcin = ldp;
+                ndesc = 2;
+            }
+            if ((rc = gemm_f64(Op::N, Op::T, d, ndesc, s))) return rc;
+        }
+        if ((rc = potrf_f64(P, ldp, ns, h->parts + (long long)i * h->T, h->dinfo, i * ns, s))) return rc;
+        if ((rc = trtri_f64_inplace(P, ldp, ns, h->ttmp, s))) return rc;
+        const bool has_next = i < nt - 1;
+        if (has_next && (rc = copy2d(A + ns, ldp, h->Ftmp, ldf, ns, ns, 1.0, s))) return rc;
+        const int mF = (has_next ? ns : 0) + nd;
+        if (mF > 0) {
+            const double* src = h->Ftmp + (has_next ? 0 : ns);
+            double* dst = P + (has_next ? ns : 2 * ns);
+            GemmDesc d = gdesc(src, ldf, P, ldp, dst, ldp, mF, ns, ns, 1.0, 0.0, G_BUPPER);
+            if ((rc = gemm_f64(Op::N, Op::T, &d, 1, s))) return rc;
+        }
+    }
+    if (nd > 0) {
+        rc = corner_factor(h->Lp + 2 * ns, h->pstride, ldp, ns, nt, nd, a_corner(h), nd, h->Ldd, h->Xdd, h->cpart,
+                           h->parts + (long long)nt * h->T, h->dinfo, nt * ns, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int do_factorize(rgf_handle_s* h)
+{
+    if (!h->Ap && h->factorized) return RGF_OK;   // in place: A is gone, the factor is current
+    hipSetDevice(h->device);
+    int rc = factor_sweep(h);
+    if (rc) return dev_status(h, rc, "factorize");
+    int info = 0;
+    hipError_t e = hipMemcpyAsync(&info, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return dev_status(h, (int)e, "factorize");
+    if (info != INT_MAX) {
+        h->factorized = false;
+        if (!h->Ap) h->broken = true;
+        return set_err(h, RGF_ENOTSPD, "matrix is not positive definite: non-positive pivot at column %d", info);
+    }
+    h->factorized = true;
+    return RGF_OK;
+}
+
+int ensure_factorized(rgf_handle_s* h)
+{
+    if (h->factorized) return RGF_OK;
+    if (h->broken)
+        return set_err(h, RGF_ESTATE, "factorization failed earlier and the matrix was not kept");
+    return do_factorize(h);
+}
+
+// ------------------------------------------------------------------ solve
+// SecondStageSolve (RGF.H:344-418) as products with the stored inverses:
+//   forward   y_i = X_i (b_i - B_{i-1} y_{i-1}),  y_d = X_dd (b_d - sum D_i y_i)
+//   backward  x_d = X_dd^T y_d,  x_i = X_i^T (y_i - B_i^T x_{i+1} - D_i^T x_d)
+// x, b: device, n x nr (ld n), nr <= 16; Y is scratch.
+int solve_block(rgf_handle_s* h, double* x, const double* b, int nr)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp;
+    const int n = (int)h->n;
+    hipStream_t s = h->stream;
+    double* Y = h->Y;
+    double* w = h->work;
+    int rc;
+    if (x != b && (rc = copy2d(b, n, x, n, n, nr, 1.0, s))) return rc;   // x holds r during the forward pass
+    double* xd = x + (long long)nt * ns;
+    double* yd = Y + (long long)nt * ns;
+    for (int i = 0; i < nt; i++) {
+        double* P = l_panel(h, i);
+        double* ri = x + (long long)i * ns;
+        double* yi = Y + (long long)i * ns;
+        if (i > 0) {
+            const double* Pm = l_panel(h, i - 1);
+            if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, n, 1.0, ri, n,
+                                 ri, n, w, s)))
+                return rc;
+        }
+        if ((rc = skinny_f64(Op::N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s))) return rc;
+        if (nd > 0 &&
+            (rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, yi, n, 1.0, xd, n, xd, n, w, s)))
+            return rc;
+    }
+    if (nd > 0) {
+        if ((rc = skinny_f64(Op::N, 1, nd, nd, nr, 1.0, h->Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::T, 2, nd, nd, nr, 1.0, h->Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s))) return rc;
+    }
+    for (int i = nt - 1; i >= 0; i--) {
+        double* P = l_panel(h, i);
+        double* yi = Y + (long long)i * ns;
+        double* xi = x + (long long)i * ns;
+        if (nd > 0 &&
+            (rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s)))
+            return rc;
+        if (i < nt - 1 &&
+            (rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, P + ns, ldp, x + (long long)(i + 1) * ns, n, 1.0, yi, n,
+                             yi, n, w, s)))
+            return rc;
+        if ((rc = skinny_f64(Op::T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s))) return rc;
+    }
+    return 0;
+}
+
+int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
+{
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    if ((rc = ensure_solve_buffers(h))) return rc;
+    const long long n = h->n;
+    for (size_t c0 = 0; c0 < nrhs; c0 += 16) {
+        const int nr = (int)std::min<size_t>(16, nrhs - c0);
+        rc = solve_block(h, x + c0 * n, b + c0 * n, nr);
+        if (rc) return dev_status(h, rc, "solve");
+    }
+    return RGF_OK;
+}
+
+// ------------------------------------------------------------------ selected inversion
+// ThirdStageRGF (RGF.H:422-506) in the G-formulation with X_i = L_ii^-1 resident:
+//   W = F_i X_i, F_i = [B_i; D_i]            Z = Sb W   (Sb = [[S_{i+1}, S_{d,i+1}^T], [S_{d,i+1}, S_dd]])
+//   S_ii = X_i^T X_i + W^T Z                 S_{d,i} = -Z[D rows]
+// exact = 0 keeps Sb block-diagonal, which is the reference's formula.
+int selinv_sweep(rgf_handle_s* h, double* diag_out)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, lds = h->ldf;
+    const int m = ns + nd;
+    const int exact = h->opt.selinv_mode == RGF_SELINV_EXACT;
+    hipStream_t s = h->stream;
+    int rc;
+    if ((rc = fill2d(h->Sb[0], lds, m, m, 0.0, s))) return rc;
+    if ((rc = fill2d(h->Sb[1], lds, m, m, 0.0, s))) return rc;
+    if (nd > 0) {
+        for (int q = 0; q < 2; q++) {
+            double* Sdd = h->Sb[q] + (long long)ns * lds + ns;
+            GemmDesc d = gdesc(h->Xdd, nd, h->Xdd, nd, Sdd, lds, nd, nd, nd, 1.0, 0.0, 0);
+            if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
+        }
+        if ((rc = extract_diag(h->Sb[0] + (long long)ns * lds + ns, lds, nd, diag_out + (long long)nt * ns, s))) return rc;
+    }
+    int cur = 0;
+    for (int i = nt - 1; i >= 0; i--) {
+        const double* P = l_panel(h, i);
+        const bool last = i == nt - 1;
+        const int mF = last ? nd : m;
+        const double* F = last ? P + 2 * ns : P + ns;
+        const int r0 = last ? ns : 0;
+        double* Sc = h->Sb[cur];
+        double* Sn = h->Sb[cur ^ 1];
+        if (mF > 0) {
+            GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
+            if ((rc = gemm_f64(Op::N, Op::N, &d, 1, s))) return rc;
+            d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
+            if ((rc = gemm_f64(Op::N, Op::N, &d, 1, s))) return rc;
+        }
+        {
+            GemmDesc d = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
+            if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
+        }
+        if (mF > 0) {
+            GemmDesc d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 1.0, G_CLOWER);
+            if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
+        }
+        if ((rc = selinv_finish(Sn, lds, ns, nd, h->Z, lds, mF - nd, exact && mF > 0, diag_out + (long long)i * ns, s)))
+            return rc;
+        cur ^= 1;
+    }
+    return 0;
+}
+
+int do_selinv_device(rgf_handle_s* h, double* diag_dev)
+{
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    if ((rc = ensure_selinv_buffers(h))) return rc;
+    rc = selinv_sweep(h, diag_dev);
+    if (rc) return dev_status(h, rc, "selinv");
+    return RGF_OK;
+}
+
+// ------------------------------------------------------------------ residual
+// r = b - A x with the resident A (full symmetric diagonal blocks), then ||r||_2.
+__global__ __launch_bounds__(1024) void norm2_kernel(const double* r, long long n, double* out)
+{
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 1024) s += r[i] * r[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sqrt(red[0]);
+}
+
+int block_matvec_residual(rgf_handle_s* h, const double* x, const double* b, double* r, int nr)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, n = (int)h->n;
+    hipStream_t s = h->stream;
+    double* w = h->work;
+    int rc;
+    if ((rc = copy2d(b, n, r, n, n, nr, 1.0, s))) return rc;
+    const double* xd = x + (long long)nt * ns;
+    double* rd = r + (long long)nt * ns;
+    for (int t = 0; t < nt; t++) {
+        const double* A = h->Ap + (long long)t * h->pstride;
+        const double* xt = x + (long long)t * ns;
+        double* rt = r + (long long)t * ns;
+        if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s))) return rc;
+        if (t < nt - 1) {
+            if ((rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, A + ns, ldp, xt + ns, n, 1.0, rt, n, rt, n, w, s))) return rc;
+            if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A + ns, ldp, xt, n, 1.0, rt + ns, n, rt + ns, n, w, s)))
+                return rc;
+        }
+        if (nd > 0) {
+            if ((rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, A + 2 * ns, ldp, xd, n, 1.0, rt, n, rt, n, w, s))) return rc;
+            if ((rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s))) return rc;
+        }
+    }
+    if (nd > 0 && (rc = skinny_f64(Op::N, 0, nd, nd, nr, -1.0, h->Add, nd, xd, n, 1.0, rd, n, rd, n, w, s))) return rc;
+    return 0;
+}
+
+int residual_norm_host(rgf_handle_s* h, const double* x, const double* b, double* out, bool normalized)
+{
+    if (!h->Ap) return set_err(h, RGF_ESTATE, "residual needs the matrix: create the handle with keep_matrix=1");
+    int rc = ensure_solve_buffers(h);
+    if (rc) return rc;
+    hipSetDevice(h->device);
+    const size_t bytes = (size_t)h->n * sizeof(double);
+    hipStream_t s = h->stream;
+    hipError_t e = hipMemcpyAsync(h->X, x, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->Bv, b, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return dev_status(h, (int)e, "residual upload");
+    rc = block_matvec_residual(h, h->X, h->Bv, h->Y, 1);
+    if (rc) return dev_status(h, rc, "residual");
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, h->Y, h->n, h->scal);
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, h->Bv, h->n, h->scal + 1);
+    double v[2];
+    e = hipMemcpyAsync(v, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return dev_status(h, (int)e, "residual");
+    *out = normalized ? v[0] / v[1] : v[0];
+    return RGF_OK;
+}
+
+// ------------------------------------------------------------------ MF export
+// Packs panels into the reference's MF layout (RGF.H:95, 114-138).
+int export_mf(rgf_handle_s* h, const double* panels, const double* corner, double* mf, bool lower_diag)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp;
+    std::vector<double> buf((size_t)h->pstride);
+    size_t off = 0;
+    for (int t = 0; t < nt; t++) {
+        hipError_t e = hipMemcpy(buf.data(), panels + (long long)t * h->pstride, buf.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return dev_status(h, (int)e, "export");
+        const bool last = t == nt - 1;
+        const int lda = last ? ns + nd : 2 * ns + nd;
+        for (int c = 0; c < ns; c++) {
+            double* col = mf + off + (size_t)c * lda;
+            const double* src = buf.data() + (size_t)c * ldp;
+            for (int r = 0; r < ns; r++) col[r] = (lower_diag && r < c) ? 0.0 : src[r];
+            if (!last)
+                for (int r = 0; r < ns; r++) col[ns + r] = src[ns + r];
+            for (int d = 0; d < nd; d++) col[(last ? ns : 2 * ns) + d] = src[2 * ns + d];
+        }
+        off += (size_t)lda * ns;
+    }
+    if (nd > 0) {
+        std::vector<double> cb((size_t)nd * nd);
+        hipError_t e = hipMemcpy(cb.data(), corner, cb.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return dev_status(h, (int)e, "export");
+        for (int c = 0; c < nd; c++)
+            for (int r = 0; r < nd; r++) mf[off + (size_t)c * nd + r] = (r < c) ? 0.0 : cb[(size_t)c * nd + r];
+    }
+    return RGF_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+void rgf_default_options(rgf_options* o)
+{
+    o->precision = RGF_FP64;
+    o->selinv_mode = RGF_SELINV_EXACT;
+    o->keep_matrix = 1;
+    o->device = 0;
+    o->refine_iters = 10;
+    o->refine_tol = 1e-14;
+}
+
+void rgf_default_gen_params(rgf_gen_params* p)
+{
+    p->seed = 0;
+    p->matrix_id = 0;
+    p->sigma_c = 0.1;
+    p->sigma_a = 0.05;
+    p->theta_d = 1.0;
+    p->theta_c = 1.0;
+}
+
+int rgf_create_csc(const size_t* ia, const size_t* ja, const double* a, size_t ns, size_t nt, size_t nd,
+                   const rgf_options* opts, rgf_handle_t* out)
+{
+    if (!ia || !ja || !a || !out) return set_err(nullptr, RGF_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = validate_shape(ns, nt, nd);
+    if (rc) return rc;
+    const size_t n = ns * nt + nd, sn = ns * nt;
+    if (ia[0] != 0) return set_err(nullptr, RGF_EINVAL, "ia[0] must be 0");
+    for (size_t c = 0; c < n; c++) {
+        if (ia[c + 1] < ia[c]) return set_err(nullptr, RGF_EINVAL, "ia is not non-decreasing at column %zu", c);
+        for (size_t k = ia[c]; k < ia[c + 1]; k++) {
+            const size_t r = ja[k];
+            bool ok;
+            if (r < c || r >= n) ok = false;
+            else if (c >= sn || r >= sn) ok = true;
+            else ok = (r / ns == c / ns) || (r / ns == c / ns + 1);
+            if (!ok)
+                return set_err(nullptr, RGF_EINVAL,
+                               "CSC entry %zu (row %zu, column %zu) is outside the lower block-arrow pattern", k, r, c);
+        }
+    }
+    rgf_handle_s* h = new_handle(ns, nt, nd, opts, &rc);
+    if (!h) return rc;
+    const size_t nnz = ia[n];
+    uint64_t *dia = nullptr, *dja = nullptr;
+    double* da = nullptr;
+    hipError_t e = hipMalloc(&dia, (n + 1) * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&dja, std::max<size_t>(nnz, 1) * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&da, std::max<size_t>(nnz, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(dia, ia, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz) e = hipMemcpy(dja, ja, nnz * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz) e = hipMemcpy(da, a, nnz * sizeof(double), hipMemcpyHostToDevice);
+    int krc = 0;
+    if (e == hipSuccess) {
+        double* P = a_panel(h, 0);
+        krc = fill2d(P, h->ldp, h->ldp, (int)(nt * ns), 0.0, h->stream);   // panels are contiguous
+        if (!krc) krc = fill2d(a_corner(h), (int)std::max<size_t>(nd, 1), (int)nd, (int)nd, 0.0, h->stream);
+        if (!krc)
+            krc = scatter_csc(dia, dja, da, (long long)nnz, (int)ns, (int)nt, (int)nd, P, h->pstride, h->ldp,
+                              a_corner(h), h->stream);
+        if (!krc) e = hipStreamSynchronize(h->stream);
+    }
+    hipFree(dia); hipFree(dja); hipFree(da);
+    if (e != hipSuccess || krc) {
+        rc = dev_status(h, e != hipSuccess ? (int)e : krc, "CSC upload");
+        g_create_error = h->err;
+        rgf_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return RGF_OK;
+}
+
+int rgf_create_generated(size_t ns, size_t nt, size_t nd, const rgf_gen_params* gen, const rgf_options* opts,
+                         rgf_handle_t* out)
+{
+    if (!out) return set_err(nullptr, RGF_EINVAL, "null argument");
+    *out = nullptr;
+    int rc;
+    rgf_handle_s* h = new_handle(ns, nt, nd, opts, &rc);
+    if (!h) return rc;
+    rgf_gen_params gp;
+    rgf_default_gen_params(&gp);
+    if (gen) gp = *gen;
+    GenParams g{gp.seed, gp.matrix_id, (int)ns, (int)nt, (int)nd, gp.sigma_c, gp.sigma_a, gp.theta_d, gp.theta_c};
+    double *tmp = nullptr, *rows = nullptr;
+    hipError_t e = hipMalloc(&tmp, ns * ns * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&rows, (ns * nt + nd) * sizeof(double));
+    int krc = 0;
+    if (e == hipSuccess) {
+        krc = generate_matrix(g, a_panel(h, 0), h->pstride, h->ldp, a_corner(h), tmp, rows, h->stream);
+        if (!krc) e = hipStreamSynchronize(h->stream);
+    }
+    hipFree(tmp); hipFree(rows);
+    if (e != hipSuccess || krc) {
+        rc = dev_status(h, e != hipSuccess ? (int)e : krc, "generate");
+        g_create_error = h->err;
+        rgf_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return RGF_OK;
+}
+
+int rgf_destroy(rgf_handle_t h)
+{
+    if (!h) return RGF_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) hipFree(p);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return RGF_OK;
+}
+
+const char* rgf_last_error(rgf_handle_t h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int rgf_factorize(rgf_handle_t h)
+{
+    if (!h) return RGF_EINVAL;
+    if (!h->Ap && h->factorized) return RGF_OK;
+    h->factorized = false;
+    return do_factorize(h);
+}
+
+int rgf_logdet(rgf_handle_t h, double* out)
+{
+    if (!h || !out) return RGF_EINVAL;
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    const size_t np = (size_t)h->nt * h->T + (h->nd > 0 ? 1 : 0);
+    std::vector<double> parts(np);
+    hipError_t e = hipMemcpyAsync(parts.data(), h->parts, np * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return dev_status(h, (int)e, "logdet");
+    double s = 0.0;
+    for (double v : parts) s += v;   // fixed order: block by block, tile by tile, corner last
+    *out = 2.0 * s;
+    return RGF_OK;
+}
+
+int rgf_solve_device(rgf_handle_t h, double* x_dev, const double* b_dev, size_t nrhs)
+{
+    if (!h || !x_dev || !b_dev) return RGF_EINVAL;
+    hipSetDevice(h->device);
+    return do_solve_device(h, x_dev, b_dev, nrhs);
+}
+
+int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
+{
+    if (!h || !x || !b) return RGF_EINVAL;
+    if (nrhs == 0) return RGF_OK;
+    hipSetDevice(h->device);
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    if ((rc = ensure_solve_buffers(h))) return rc;
+    const long long n = h->n;
+    for (size_t c0 = 0; c0 < nrhs; c0 += 16) {
+        const int nr = (int)std::min<size_t>(16, nrhs - c0);
+        const size_t bytes = (size_t)n * nr * sizeof(double);
+        hipError_t e = hipMemcpyAsync(h->Bv, b + c0 * n, bytes, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) return dev_status(h, (int)e, "solve upload");
+        rc = solve_block(h, h->X, h->Bv, nr);
+        if (rc) return dev_status(h, rc, "solve");
+        e = hipMemcpyAsync(x + c0 * n, h->X, bytes, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return dev_status(h, (int)e, "solve download");
+    }
+    return RGF_OK;
+}
+
+int rgf_selinv_diag_device(rgf_handle_t h, double* diag_dev)
+{
+    if (!h || !diag_dev) return RGF_EINVAL;
+    hipSetDevice(h->device);
+    return do_selinv_device(h, diag_dev);
+}
+
+int rgf_selinv_diag(rgf_handle_t h, double* diag)
+{
+    if (!h || !diag) return RGF_EINVAL;
+    hipSetDevice(h->device);
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    if ((rc = ensure_selinv_buffers(h))) return rc;
+    rc = do_selinv_device(h, h->diag);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(diag, h->diag, (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return dev_status(h, (int)e, "selinv download");
+    return RGF_OK;
+}
+
+int rgf_residual_norm(rgf_handle_t h, const double* x, const double* b, double* out)
+{
+    if (!h || !x || !b || !out) return RGF_EINVAL;
+    return residual_norm_host(h, x, b, out, false);
+}
+
+int rgf_residual_norm_normalized(rgf_handle_t h, const double* x, const double* b, double* out)
+{
+    if (!h || !x || !b || !out) return RGF_EINVAL;
+    return residual_norm_host(h, x, b, out, true);
+}
+
+int rgf_synchronize(rgf_handle_t h)
+{
+    if (!h) return RGF_EINVAL;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? RGF_OK : dev_status(h, (int)e, "synchronize");
+}
+
+int rgf_get_info(rgf_handle_t h, rgf_info_t* info)
+{
+    if (!h || !info) return RGF_EINVAL;
+    const double ns = h->ns, nt = h->nt, nd = h->nd;
+    const double P = ns * ns * ns / 3.0, Pd = nd * nd * nd / 3.0;
+    info->ns = h->ns; info->nt = h->nt; info->nd = h->nd; info->n = (size_t)h->n;
+    info->device_bytes = h->bytes;
+    info->flops_factor = nt * P + (nt - 1) * 2 * ns * ns * ns + nd * (nt * ns * ns + 2 * (nt - 1) * ns * ns + nt * nd * ns) + Pd;
+    info->flops_selinv = nt * 2 * P + (nt - 1) * 4 * ns * ns * ns + nd * (nt * ns * ns + 4 * (nt - 1) * ns * ns) + 2 * Pd;
+    info->bytes_solve_per_rhs = 2.0 * 8.0 * (nt * ns * (ns + 1) / 2 + (nt - 1) * ns * ns + nt * ns * nd + nd * (nd + 1) / 2);
+    return RGF_OK;
+}
+
+int rgf_export_matrix_mf(rgf_handle_t h, double* mf)
+{
+    if (!h || !mf) return RGF_EINVAL;
+    if (!h->Ap) return set_err(h, RGF_ESTATE, "matrix not kept (keep_matrix=0)");
+    hipSetDevice(h->device);
+    hipStreamSynchronize(h->stream);
+    return export_mf(h, h->Ap, h->Add, mf, true);
+}
+
+int rgf_export_factor_mf(rgf_handle_t h, double* mf)
+{
+    if (!h || !mf) return RGF_EINVAL;
+    hipSetDevice(h->device);
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    hipStreamSynchronize(h->stream);
+    rc = export_mf(h, h->Lp, h->Xdd, mf, true);
+    return rc;
+}
+
+}  // extern "C"
+
+// ====================================================================== diagnostics
+#include "../../include/rgf_testing.h"
+
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+}  // namespace
+
+extern "C" int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const double* A, int lda,
+                              const double* B, int ldb, double beta, double* C, int ldc, int flags)
+{
+    if (m <= 0 || n <= 0 || k < 0) return RGF_EINVAL;
+    const size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
+    DevBuf a, b, c;
+    hipStream_t s = nullptr;
+    hipError_t e = hipMalloc(&a.p, std::max<size_t>(na, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b.p, std::max<size_t>(nb, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&c.p, nc * 8);
+    if (e == hipSuccess && na) e = hipMemcpy(a.p, A, na * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nb) e = hipMemcpy(b.p, B, nb * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c.p, C, nc * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_gemm: %s", hipGetErrorString(e));
+    GemmDesc d = gdesc((const double*)a.p, lda, (const double*)b.p, ldb, (double*)c.p, ldc, m, n, k, alpha, beta, flags);
+    int rc = gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s);
+    if (rc) return dev_status(nullptr, rc, "debug_gemm");
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(C, c.p, nc * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_gemm: %s", hipGetErrorString(e));
+    return RGF_OK;
+}
+
+extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* logdet)
+{
+    if (n <= 0 || lda < n) return RGF_EINVAL;
+    const int T = (n + 63) / 64;
+    size_t tt = 1;
+    for (int sz = 1; sz < T; sz *= 2) {
+        size_t G = (size_t)((T - sz + 2 * sz - 1) / (2 * sz));
+        tt = std::max(tt, G * (size_t)(sz * 64) * (size_t)(sz * 64));
+    }
+    DevBuf a, parts, info, tmp;
+    hipError_t e = hipMalloc(&a.p, (size_t)lda * n * 8);
+    if (e == hipSuccess) e = hipMalloc(&parts.p, (size_t)T * 8);
+    if (e == hipSuccess) e = hipMalloc(&info.p, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&tmp.p, tt * 8);
+    if (e == hipSuccess) e = hipMemcpy(a.p, A, (size_t)lda * n * 8, hipMemcpyHostToDevice);
+    int init = INT_MAX;
+    if (e == hipSuccess) e = hipMemcpy(info.p, &init, sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: %s", hipGetErrorString(e));
+    int rc = potrf_f64((double*)a.p, lda, n, (double*)parts.p, (int*)info.p, 0, nullptr);
+    if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, n, (double*)tmp.p, nullptr);
+    if (rc) return dev_status(nullptr, rc, "debug_potrf");
+    std::vector<double> hp(T);
+    int hinfo = 0;
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(A, a.p, (size_t)lda * n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hp.data(), parts.p, (size_t)T * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: %s", hipGetErrorString(e));
+    double sum = 0.0;
+    for (double v : hp) sum += v;
+    if (logdet) *logdet = 2.0 * sum;
+    if (hinfo != INT_MAX) return set_err(nullptr, RGF_ENOTSPD, "non-positive pivot at column %d", hinfo);
+    return RGF_OK;
+}

@@ -1,0 +1,411 @@
+// Dense FP64 block kernels of the factor / selected-inversion sweep:
+//   potrf   blocked right-looking Cholesky of one ns x ns diagonal block
+//           (replaces magma_dpotrf_gpu at RGF.H:254,284,312,335); the 64-wide
+//           panel is factored in LDS and its log-diagonal summed in the same
+//           kernel (logDet, RGF.H:616-627), the trailing update is an MFMA GEMM.
+//   trtri   in-place L^-1 by diagonal-tile inversion in LDS + recursive
+//           GEMM merges X21 = -X22 L21 X11 (replaces magma_dtrtri_gpu,
+//           RGF.H:441,450,471,487, and the tril/identity kernels d_tril,
+//           d_init_eye at CWC_utility.cu:231-251, 792-815).
+//   corner  the nd x nd arrowhead corner (RGF.H:280,308,332,335,439-442).
+//   small   copy / fill / mirror / diagonal gather (d_fill, d_indexed_copy_offset,
+//           CWC_utility.cu:883-901, 964-979).
+#include "rgf_internal.hpp"
+#include <climits>
+
+namespace rgf {
+
+constexpr int NB = 64;
+
+// ------------------------------------------------------------------ potrf panel
+// grid.x = 1 + ceil((n-k0-kb)/64).  Workgroup 0 factors the kb x kb diagonal tile
+// and writes it back (zeros above the diagonal); workgroup w>0 redundantly factors
+// the tile in LDS and solves its 64 rows of the panel below: P <- P L_kk^-T.
+__global__ __launch_bounds__(256) void potrf_panel_kernel(double* A, int lda, int n, int k0, int kb,
+                                                          double* logdet_slot, int* info,
+                                                          int info_base)
+{
+    __shared__ double Ld[NB][NB + 1];
+    __shared__ double P[NB][NB + 1];
+    __shared__ int bad_col;
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    const int r0 = k0 + kb + (wg - 1) * NB;
+    const int nr = wg > 0 ? min(NB, n - r0) : 0;
+
+    if (tid == 0) bad_col = INT_MAX;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e & (NB - 1), c = e >> 6;
+        Ld[r][c] = (r < kb && c < kb && r >= c) ? A[(long long)(k0 + c) * lda + (k0 + r)] : 0.0;
+        if (wg > 0) P[r][c] = (r < nr && c < kb) ? A[(long long)(k0 + c) * lda + (r0 + r)] : 0.0;
+    }
+    __syncthreads();
+
+    // unblocked right-looking Cholesky of the tile
+    for (int j = 0; j < kb; j++) {
+        double d = Ld[j][j];
+        __syncthreads();
+        bool bad = !(d > 0.0);
+        double sd = bad ? 1.0 : sqrt(d);
+        if (tid < NB) {
+            if (tid == j) {
+                Ld[j][j] = sd;
+                if (bad) bad_col = min(bad_col, j);
+            } else if (tid > j && tid < kb) {
+                Ld[tid][j] /= sd;
+            }
+        }
+        __syncthreads();
+        const int w = kb - j - 1;
+        for (int e = tid; e < w * w; e += 256) {
+            int r = j + 1 + e % w, c = j + 1 + e / w;
+            if (r >= c) Ld[r][c] -= Ld[r][j] * Ld[c][j];
+        }
+        __syncthreads();
+    }
+
+    if (wg == 0) {
+        if (tid == 0) {
+            double s = 0.0;
+            for (int j = 0; j < kb; j++) s += log(Ld[j][j]);
+            *logdet_slot = s;
+            if (bad_col != INT_MAX) atomicMin(info, info_base + k0 + bad_col + 1);
+        }
+        for (int e = tid; e < kb * kb; e += 256) {
+            int r = e % kb, c = e / kb;
+            A[(long long)(k0 + c) * lda + (k0 + r)] = (r >= c) ? Ld[r][c] : 0.0;
+        }
+        return;
+    }
+
+    // rows below: P <- P * L^-T  (forward substitution over the tile's columns)
+    for (int c = 0; c < kb; c++) {
+        if (tid < NB) P[tid][c] /= Ld[c][c];
+        __syncthreads();
+        const int w = kb - c - 1;
+        for (int e = tid; e < NB * w; e += 256) {
+            int r = e & (NB - 1), q = c + 1 + (e >> 6);
+            P[r][q] -= P[r][c] * Ld[q][c];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < NB * kb; e += 256) {
+        int r = e & (NB - 1), c = e >> 6;
+        if (r < nr) A[(long long)(k0 + c) * lda + (r0 + r)] = P[r][c];
+    }
+}
+
+int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base,
+              hipStream_t s)
+{
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0);
+        const int below = n - k0 - kb;
+        const int grid = 1 + (below + NB - 1) / NB;
+        hipLaunchKernelGGL(potrf_panel_kernel, dim3(grid), dim3(256), 0, s, A, lda, n, k0, kb,
+                           logdet_parts + k0 / NB, info, info_base);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+        if (below > 0) {
+            const double* Pn = A + (long long)k0 * lda + (k0 + kb);
+            double* Cn = A + (long long)(k0 + kb) * lda + (k0 + kb);
+            GemmDesc d = gdesc(Pn, lda, Pn, lda, Cn, lda, below, below, kb, -1.0, 1.0, G_CLOWER);
+            int rc = gemm_f64(Op::N, Op::T, &d, 1, s);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ trtri
+// grid.x = T*T tiles.  (ti < tj): zero the tile above the diagonal.  (ti == tj):
+// invert the diagonal tile in LDS.  (ti > tj): nothing (written by the merges).
+__global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int n, int T)
+{
+    __shared__ double Ls[NB][NB + 1];
+    __shared__ double Xs[NB][NB + 1];
+    const int ti = blockIdx.x % T, tj = blockIdx.x / T;
+    const int tid = threadIdx.x;
+    const int r0 = ti * NB, c0 = tj * NB;
+    const int rb = min(NB, n - r0), cb = min(NB, n - c0);
+    if (ti > tj) return;
+    if (ti < tj) {
+        for (int e = tid; e < NB * NB; e += 256) {
+            int r = e & (NB - 1), c = e >> 6;
+            if (r < rb && c < cb) A[(long long)(c0 + c) * lda + (r0 + r)] = 0.0;
+        }
+        return;
+    }
+    const int kb = rb;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e & (NB - 1), c = e >> 6;
+        Ls[r][c] = (r < kb && c < kb && r >= c) ? A[(long long)(r0 + c) * lda + (r0 + r)] : 0.0;
+        Xs[r][c] = (r == c && r < kb) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // forward substitution L X = I, row by row
+    for (int r = 0; r < kb; r++) {
+        if (tid <= r) Xs[r][tid] /= Ls[r][r];
+        __syncthreads();
+        const int w = kb - r - 1, cols = r + 1;
+        for (int e = tid; e < w * cols; e += 256) {
+            int p = r + 1 + e % w, c = e / w;
+            Xs[p][c] -= Ls[p][r] * Xs[r][c];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < kb * kb; e += 256) {
+        int r = e % kb, c = e / kb;
+        A[(long long)(r0 + c) * lda + (r0 + r)] = (r >= c) ? Xs[r][c] : 0.0;
+    }
+}
+
+int trtri_f64_inplace(double* A, int lda, int n, double* tmp, hipStream_t s)
+{
+    const int T = (n + NB - 1) / NB;
+    hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T), dim3(256), 0, s, A, lda, n, T);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    for (int sz = 1; sz < T; sz *= 2) {
+        const int G = (T - sz + 2 * sz - 1) / (2 * sz);   // groups with a non-empty right half
+        const int n1 = sz * NB;
+        const long long gstride = (long long)(2 * sz * NB) * (1 + (long long)lda);
+        const long long tstride = (long long)n1 * n1;
+        // groups whose right half is complete share one descriptor
+        int gfull = 0;
+        for (int g = 0; g < G; g++)
+            if ((g * 2 * sz + 2 * sz) * NB <= n) gfull = g + 1;
+        GemmDesc d1[2], d2[2];
+        int nd1 = 0;
+        for (int part = 0; part < 2; part++) {
+            int g0 = part == 0 ? 0 : gfull;
+            int cnt = part == 0 ? gfull : G - gfull;
+            if (cnt <= 0) continue;
+            const int c0 = g0 * 2 * sz * NB, r0 = c0 + n1;
+            const int n2 = min(n1, n - r0);
+            double* L21 = A + (long long)c0 * lda + r0;
+            double* X11 = A + (long long)c0 * lda + c0;
+            double* X22 = A + (long long)r0 * lda + r0;
+            double* Tg = tmp + (long long)g0 * tstride;
+            // T = L21 X11   (X11 lower)
+            GemmDesc a = gdesc(L21, lda, X11, lda, Tg, n1, n2, n1, n1, 1.0, 0.0, G_BLOWER);
+            a.count = cnt; a.sA = gstride; a.sB = gstride; a.sC = tstride; a.sCin = tstride;
+            // X21 = -X22 T  (X22 lower)
+            GemmDesc b = gdesc(X22, lda, Tg, n1, L21, lda, n2, n1, n2, -1.0, 0.0, G_ALOWER);
+            b.count = cnt; b.sA = gstride; b.sB = tstride; b.sC = gstride; b.sCin = gstride;
+            d1[nd1] = a; d2[nd1] = b; nd1++;
+        }
+        int rc = gemm_f64(Op::N, Op::N, d1, nd1, s);
+        if (rc) return rc;
+        rc = gemm_f64(Op::N, Op::N, d2, nd1, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ small kernels
+__global__ void copy2d_kernel(const double* src, int lds, double* dst, int ldd, int rows, int cols,
+                              double alpha)
+{
+    const long long total = (long long)rows * cols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        int r = (int)(e % rows), c = (int)(e / rows);
+        dst[(long long)c * ldd + r] = alpha * src[(long long)c * lds + r];
+    }
+}
+
+int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
+           hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    long long total = (long long)rows * cols;
+    int grid = (int)min((total + 255) / 256, (long long)8192);
+    hipLaunchKernelGGL(copy2d_kernel, dim3(grid), dim3(256), 0, s, src, lds, dst, ldd, rows, cols, alpha);
+    return (int)hipGetLastError();
+}
+
+__global__ void fill2d_kernel(double* dst, int ldd, int rows, int cols, double v)
+{
+    const long long total = (long long)rows * cols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        int r = (int)(e % rows), c = (int)(e / rows);
+        dst[(long long)c * ldd + r] = v;
+    }
+}
+
+int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    long long total = (long long)rows * cols;
+    int grid = (int)min((total + 255) / 256, (long long)8192);
+    hipLaunchKernelGGL(fill2d_kernel, dim3(grid), dim3(256), 0, s, dst, ldd, rows, cols, v);
+    return (int)hipGetLastError();
+}
+
+// mirror the lower triangle into the upper one through 32x32 LDS tiles; optional
+// diagonal gather.
+__global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, int n, int T, double* diag)
+{
+    __shared__ double t[32][33];
+    const int bi = blockIdx.x % T, bj = blockIdx.x / T;   // destination tile (bi row, bj col), bi <= bj
+    if (bi > bj) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    // source: tile (bj row-block, bi col-block) of the lower triangle
+    for (int q = ty; q < 32; q += 8) {
+        int r = bj * 32 + tx, c = bi * 32 + q;
+        t[q][tx] = (r < n && c < n) ? A[(long long)c * lda + r] : 0.0;   // t[c_local][r_local]
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        int r = bi * 32 + tx, c = bj * 32 + q;   // destination (r, c) = source (c, r)
+        if (r < n && c < n && r < c) A[(long long)c * lda + r] = t[tx][q];
+        if (diag && r < n && r == c) diag[r] = A[(long long)c * lda + r];
+    }
+}
+
+int mirror_lower_diag(double* A, int lda, int n, double* diag, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    const int T = (n + 31) / 32;
+    hipLaunchKernelGGL(mirror_kernel, dim3(T * T), dim3(256), 0, s, A, lda, n, T, diag);
+    return (int)hipGetLastError();
+}
+
+int mirror_lower(double* A, int lda, int n, hipStream_t s) { return mirror_lower_diag(A, lda, n, nullptr, s); }
+
+__global__ void extract_diag_kernel(const double* A, int lda, int n, double* out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = A[(long long)i * lda + i];
+}
+
+int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(extract_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, lda, n, out);
+    return (int)hipGetLastError();
+}
+
+// S_{d,i} = -Z[zrow0 + d, c] into S[ns + d, c] and S[c, ns + d]
+__global__ void selinv_offdiag_kernel(double* S, int lds, int ns, int nd, const double* Z, int ldz,
+                                      int zrow0)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ns) return;
+    for (int d = 0; d < nd; d++) {
+        double v = -Z[(long long)c * ldz + zrow0 + d];
+        S[(long long)c * lds + ns + d] = v;
+        S[(long long)(ns + d) * lds + c] = v;
+    }
+}
+
+int selinv_finish(double* S, int lds, int ns, int nd, const double* Z, int ldz, int zrow0, int exact,
+                  double* diag_out, hipStream_t s)
+{
+    int rc = mirror_lower_diag(S, lds, ns, diag_out, s);
+    if (rc) return rc;
+    if (exact && nd > 0) {
+        hipLaunchKernelGGL(selinv_offdiag_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, S, lds, ns,
+                           nd, Z, ldz, zrow0);
+        return (int)hipGetLastError();
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ corner
+// partial[i] = D_i D_i^T for panel i (nd x nd, full), one workgroup per panel.
+__global__ __launch_bounds__(256) void corner_partial_kernel(const double* Dbase, long long dstride,
+                                                             int ldd, int ns, int nd, double* partial)
+{
+    const int i = blockIdx.x;
+    const double* D = Dbase + (long long)i * dstride;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int npairs = nd * nd;
+    for (int pq = wave; pq < npairs; pq += 4) {
+        const int p = pq % nd, q = pq / nd;
+        if (p < q) continue;
+        double acc = 0.0;
+        for (int c = lane; c < ns; c += 64) acc += D[(long long)c * ldd + p] * D[(long long)c * ldd + q];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) {
+            partial[(long long)i * npairs + p + q * nd] = acc;
+            partial[(long long)i * npairs + q + p * nd] = acc;
+        }
+    }
+}
+
+// one workgroup: S = Cin - sum_i partial[i]; Cholesky; log-det; L_dd and X_dd = L_dd^-1.
+__global__ __launch_bounds__(256) void corner_finish_kernel(const double* partial, int nparts, int nd,
+                                                            const double* Cin, int ldcin, double* Ldd,
+                                                            double* Xdd, double* logdet_slot,
+                                                            int* info, int info_base)
+{
+    __shared__ double Ls[NB][NB + 1];
+    __shared__ double Xs[NB][NB + 1];
+    __shared__ int bad_col;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad_col = INT_MAX;
+    for (int e = tid; e < nd * nd; e += 256) {
+        int r = e % nd, c = e / nd;
+        double v = Cin[(long long)c * ldcin + r];
+        for (int i = 0; i < nparts; i++) v -= partial[(long long)i * nd * nd + e];
+        Ls[r][c] = (r >= c) ? v : 0.0;
+        Xs[r][c] = (r == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nd; j++) {
+        double d = Ls[j][j];
+        __syncthreads();
+        bool bad = !(d > 0.0);
+        double sd = bad ? 1.0 : sqrt(d);
+        if (tid == j) { Ls[j][j] = sd; if (bad) bad_col = min(bad_col, j); }
+        else if (tid > j && tid < nd) Ls[tid][j] /= sd;
+        __syncthreads();
+        const int w = nd - j - 1;
+        for (int e = tid; e < w * w; e += 256) {
+            int r = j + 1 + e % w, c = j + 1 + e / w;
+            if (r >= c) Ls[r][c] -= Ls[r][j] * Ls[c][j];
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < nd; r++) {
+        if (tid <= r) Xs[r][tid] /= Ls[r][r];
+        __syncthreads();
+        const int w = nd - r - 1, cols = r + 1;
+        for (int e = tid; e < w * cols; e += 256) {
+            int p = r + 1 + e % w, c = e / w;
+            Xs[p][c] -= Ls[p][r] * Xs[r][c];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nd * nd; e += 256) {
+        int r = e % nd, c = e / nd;
+        Ldd[(long long)c * nd + r] = (r >= c) ? Ls[r][c] : 0.0;
+        Xdd[(long long)c * nd + r] = (r >= c) ? Xs[r][c] : 0.0;
+    }
+    if (tid == 0) {
+        double s = 0.0;
+        for (int j = 0; j < nd; j++) s += log(Ls[j][j]);
+        *logdet_slot = s;
+        if (bad_col != INT_MAX) atomicMin(info, info_base + bad_col + 1);
+    }
+}
+
+int corner_factor(const double* Dbase, long long dstride, int ldd, int ns, int nt, int nd,
+                  const double* Cin, int ldcin, double* Ldd, double* Xdd, double* partial,
+                  double* logdet_slot, int* info, int info_base, hipStream_t s)
+{
+    if (nd <= 0) return 0;
+    if (nd > NB) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(corner_partial_kernel, dim3(nt), dim3(256), 0, s, Dbase, dstride, ldd, ns, nd,
+                       partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(corner_finish_kernel, dim3(1), dim3(256), 0, s, partial, nt, nd, Cin, ldcin, Ldd,
+                       Xdd, logdet_slot, info, info_base);
+    return (int)hipGetLastError();
+}
+
+}  // namespace rgf

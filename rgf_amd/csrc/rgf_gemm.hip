@@ -1,0 +1,248 @@
+// FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+// One workgroup (4 waves, 256 lanes) computes a 64x64 tile of C; each wave a
+// 32x32 quarter as 2x2 MFMA 16x16 blocks.  K advances in steps of 16 through
+// LDS (op(A) and op(B) stored k-major, one double of padding per row), with the
+// next step's global loads issued into registers before the current step's
+// MFMAs.  The MFMA is fed (op(B) fragment, op(A) fragment), so each lane's four
+// accumulator values are four C columns of one C row; the 16 lanes of a row
+// group then store 16 consecutive rows of a column (a 128-byte segment) --
+// coalesced for column-major C.
+//
+// Used for every dense product of the sweep: Schur updates (RGF.H:276-280,
+// 304-308, 332), the triangular solves as products with L^-1 (RGF.H:260, 287,
+// 317), the trtri merges (RGF.H:441, 450, 471, 487) and the Takahashi products
+// (RGF.H:442, 454-458, 472, 491-501).
+#include "rgf_internal.hpp"
+#include "../../include/rgf.h"
+
+#include <vector>
+
+namespace rgf {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16, PADL = 1;
+
+__device__ __forceinline__ void select_desc(const GemmArgs& args, int z, GemmDesc& D)
+{
+    D = args.d[0];
+#pragma unroll
+    for (int q = 1; q < GEMM_MAXD; q++)
+        if (q < args.nd && z >= args.d[q].zbase) D = args.d[q];
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs args)
+{
+    __shared__ double As[BK][BM + PADL];
+    __shared__ double Bs[BK][BN + PADL];
+
+    const int z = blockIdx.y;
+    GemmDesc D;
+    select_desc(args, z, D);
+    const int tile = blockIdx.x;
+    if (tile >= D.tiles_m * D.tiles_n) return;
+    const int ti = tile % D.tiles_m, tj = tile / D.tiles_m;
+    const int i0 = ti * BM, j0 = tj * BN;
+    const int flags = D.flags;
+    if ((flags & G_CLOWER) && j0 > i0 + BM - 1) return;
+
+    const long long bz = (long long)(z - D.zbase);
+    const double* __restrict__ A = D.A + bz * D.sA;
+    const double* __restrict__ B = D.B + bz * D.sB;
+    double* C = D.C + bz * D.sC;
+    const double* Cin = D.Cin + bz * D.sCin;
+    const int M = D.m, N = D.n, K = D.k;
+    const int lda = D.lda, ldb = D.ldb;
+
+    int klo = 0, khi = K;
+    if (flags & G_ALOWER) khi = min(khi, i0 + BM);
+    if (flags & G_AUPPER) klo = max(klo, i0);
+    if (flags & G_BLOWER) klo = max(klo, j0);
+    if (flags & G_BUPPER) khi = min(khi, j0 + BN);
+    klo = (klo / BK) * BK;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    v4d acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    double ra[4], rb[4];
+
+    // global -> register loads for the k-step starting at k0
+    auto load_tiles = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!TA) {   // op(A)[i][k] = A[i + k*lda]; rows contiguous
+                int r = tid & 63, kk = (tid >> 6) + 4 * q;
+                int gi = i0 + r, gk = k0 + kk;
+                ra[q] = (gi < M && gk < khi) ? A[(long long)gk * lda + gi] : 0.0;
+            } else {     // op(A)[i][k] = A[k + i*lda]; k contiguous
+                int kk = tid & 15, r = (tid >> 4) + 16 * q;
+                int gi = i0 + r, gk = k0 + kk;
+                ra[q] = (gi < M && gk < khi) ? A[(long long)gi * lda + gk] : 0.0;
+            }
+            if (!TB) {   // op(B)[k][j] = B[k + j*ldb]; k contiguous
+                int kk = tid & 15, c = (tid >> 4) + 16 * q;
+                int gj = j0 + c, gk = k0 + kk;
+                rb[q] = (gj < N && gk < khi) ? B[(long long)gj * ldb + gk] : 0.0;
+            } else {     // op(B)[k][j] = B[j + k*ldb]; cols contiguous
+                int c = tid & 63, kk = (tid >> 6) + 4 * q;
+                int gj = j0 + c, gk = k0 + kk;
+                rb[q] = (gj < N && gk < khi) ? B[(long long)gk * ldb + gj] : 0.0;
+            }
+        }
+    };
+    auto store_tiles = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!TA) As[(tid >> 6) + 4 * q][tid & 63] = ra[q];
+            else     As[tid & 15][(tid >> 4) + 16 * q] = ra[q];
+            if (!TB) Bs[tid & 15][(tid >> 4) + 16 * q] = rb[q];
+            else     Bs[(tid >> 6) + 4 * q][tid & 63] = rb[q];
+        }
+    };
+
+    if (klo < khi) load_tiles(klo);
+    for (int k0 = klo; k0 < khi; k0 += BK) {
+        store_tiles();
+        __syncthreads();
+        if (k0 + BK < khi) load_tiles(k0 + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ks++) {
+            const int kr = ks * 4 + lq;
+            double af[2], bf[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) af[a] = As[kr][wm * 32 + a * 16 + l16];
+#pragma unroll
+            for (int b = 0; b < 2; b++) bf[b] = Bs[kr][wn * 32 + b * 16 + l16];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // epilogue: lane holds C[i = l16][j = 4*lq + r] of each 16x16 block
+    const double alpha = D.alpha, beta = D.beta;
+    const int ldc = D.ldc, ldcin = D.ldcin;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        const int gi = i0 + wm * 32 + a * 16 + l16;
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gj = j0 + wn * 32 + b * 16 + 4 * lq + r;
+                if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
+                    double v = alpha * acc[a][b][r];
+                    if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
+                    C[(long long)gj * ldc + gi] = v;
+                }
+            }
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------- kernel timing
+// Per-process instrumentation used by the benchmark: HIP events recorded on the
+// launching stream around every launch of one transpose class, so the measured
+// time is the kernel's own duration on that stream.
+namespace {
+struct TimedLaunch {
+    hipEvent_t a, b;
+    double flops;
+};
+int g_timing_class = -1;
+std::vector<TimedLaunch> g_timed;
+
+double canonical_flops(const GemmDesc& d)
+{
+    const int atri = d.flags & (G_ALOWER | G_AUPPER), btri = d.flags & (G_BLOWER | G_BUPPER);
+    const int cl = d.flags & G_CLOWER;
+    double f = 2.0 * d.m * (double)d.n * d.k * d.count;
+    if (atri && btri && cl) return f / 6.0;
+    if (atri) f *= 0.5;
+    if (btri) f *= 0.5;
+    if (cl) f *= 0.5;
+    return f;
+}
+}  // namespace
+
+int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
+{
+    if (ndesc <= 0) return 0;
+    if (ndesc > GEMM_MAXD) return (int)hipErrorInvalidValue;
+    GemmArgs args{};
+    int z = 0, maxtiles = 0, nd = 0;
+    for (int q = 0; q < ndesc; q++) {
+        GemmDesc d = descs[q];
+        if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
+        d.tiles_m = (d.m + BM - 1) / BM;
+        d.tiles_n = (d.n + BN - 1) / BN;
+        d.zbase = z;
+        z += d.count;
+        if (d.tiles_m * d.tiles_n > maxtiles) maxtiles = d.tiles_m * d.tiles_n;
+        args.d[nd++] = d;
+    }
+    if (nd == 0) return 0;
+    args.nd = nd;
+    dim3 grid(maxtiles, z), block(256);
+    const int cls = (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0);
+    TimedLaunch tl{};
+    const bool timed = cls == g_timing_class;
+    if (timed) {
+        hipEventCreate(&tl.a);
+        hipEventCreate(&tl.b);
+        tl.flops = 0.0;
+        for (int q = 0; q < nd; q++) tl.flops += canonical_flops(args.d[q]);
+        hipEventRecord(tl.a, s);
+    }
+    if (ta == Op::N && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, args);
+    else if (ta == Op::N && tb == Op::T) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, args);
+    else if (ta == Op::T && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, args);
+    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, args);
+    hipError_t err = hipGetLastError();
+    if (timed) {
+        hipEventRecord(tl.b, s);
+        g_timed.push_back(tl);
+    }
+    return (int)err;
+}
+
+}  // namespace rgf
+
+extern "C" int rgf_kernel_timing_enable(int op_class)
+{
+    for (auto& t : rgf::g_timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    rgf::g_timed.clear();
+    rgf::g_timing_class = op_class;
+    return 0;
+}
+
+extern "C" int rgf_kernel_timing_read(double* ms, double* flops, long long* launches)
+{
+    double tms = 0.0, tf = 0.0;
+    for (auto& t : rgf::g_timed) {
+        if (hipEventSynchronize(t.b) != hipSuccess) return RGF_EDEVICE;
+        float e = 0.f;
+        if (hipEventElapsedTime(&e, t.a, t.b) != hipSuccess) return RGF_EDEVICE;
+        tms += e;
+        tf += t.flops;
+    }
+    if (ms) *ms = tms;
+    if (flops) *flops = tf;
+    if (launches) *launches = (long long)rgf::g_timed.size();
+    return 0;
+}
+

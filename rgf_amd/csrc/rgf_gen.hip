@@ -1,0 +1,237 @@
+// Matrix ingestion on the device.
+//
+// 1. Synthetic generator (the benchmark workloads): Philox4x32-10 counter RNG
+//    (Salmon, Moraes, Dror, Shaw, SC'11) with a Box-Muller normal; every value
+//    is a pure function of (seed, matrix id, kind, block, element), so the
+//    matrix is regenerated identically in any launch order.  Definition:
+//      diag block   B_t = theta_d * (G_t G_t^T / ns + I),   G_t ~ N(0,1) ns x ns
+//      coupling     A_{t+1,t} = theta_c * sigma_c * N(0,1)
+//      arrow        A_{d,c}   = sigma_a * N(0,1), nd x (ns*nt)
+//      corner       H H^T + nd*I, H ~ N(0,1) nd x nd
+//      dominance    a_kk += sum_{j != k} |a_kj| + 1 over the full symmetric row.
+//    Stream of element e: key = (seed lo, seed hi), counter = (e/2 lo, e/2 hi,
+//    kind << 24 | block, matrix id), normal e % 2; kinds 1 = G_t, 2 = coupling
+//    t, 3 = arrow (e = global column * nd + row), 4 = H.
+// 2. CSC scatter for the drop-in constructor (init_supernode / d_init_supernode,
+//    RGF.H:684-700, CWC_utility.cu:1085-1116): one lane per nonzero with a
+//    binary search for its column (the reference searches linearly), written
+//    straight into the resident panel layout; diagonal blocks are stored full
+//    (mirrored) so the block matvec needs no triangle logic.
+#include "rgf_internal.hpp"
+#include <algorithm>
+
+namespace rgf {
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+}
+
+__device__ double gen_normal(uint64_t seed, uint32_t mid, uint32_t kind, uint32_t block, uint64_t e)
+{
+    const uint64_t p = e >> 1;
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (kind << 24) | block, mid};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * (1.0 / 9007199254740992.0);
+    const double r = sqrt(-2.0 * log(1.0 - u1));
+    const double t = 6.283185307179586476925286766559 * u2;
+    return (e & 1) ? r * sin(t) : r * cos(t);
+}
+
+// dst[i + j*ld] = scale * N(e), e = (col0 + j) * rows + i
+__global__ void gen_fill_kernel(double* dst, int ld, int rows, int cols, long long col0, uint64_t seed,
+                                uint32_t mid, uint32_t kind, uint32_t block, double scale)
+{
+    const long long total = (long long)rows * cols;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % rows), j = (int)(e / rows);
+        const uint64_t idx = (uint64_t)((col0 + j) * rows + i);
+        dst[(long long)j * ld + i] = scale * gen_normal(seed, mid, kind, block, idx);
+    }
+}
+
+__global__ void diag_add_kernel(double* A, int lda, int n, double v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(long long)i * lda + i] += v;
+}
+
+// row abs-sums of the full symmetric matrix, excluding the diagonal.
+// One lane per row a of block t: diag block row (coalesced), coupling column a of
+// A_{t+1,t} (block (t,t+1) is its transpose), coupling row a of A_{t,t-1}, arrow column.
+__global__ void rowsum_block_kernel(const double* P, long long pstride, int ldp, int ns, int nt, int nd,
+                                    double* rowsum)
+{
+    const int t = blockIdx.y;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= ns) return;
+    const double* Pt = P + (long long)t * pstride;
+    double s = 0.0;
+    for (int j = 0; j < ns; j++)
+        if (j != a) s += fabs(Pt[(long long)j * ldp + a]);
+    if (t < nt - 1) {
+        const double* col = Pt + (long long)a * ldp + ns;
+        for (int j = 0; j < ns; j++) s += fabs(col[j]);
+    }
+    if (t > 0) {
+        const double* Pm = P + (long long)(t - 1) * pstride + ns;
+        for (int j = 0; j < ns; j++) s += fabs(Pm[(long long)j * ldp + a]);
+    }
+    const double* ar = Pt + (long long)a * ldp + 2 * ns;
+    for (int d = 0; d < nd; d++) s += fabs(ar[d]);
+    rowsum[(long long)t * ns + a] = s;
+}
+
+// arrow rows: one workgroup per arrow row d, fixed-order tree reduction.
+__global__ __launch_bounds__(256) void rowsum_arrow_kernel(const double* P, long long pstride, int ldp, int ns,
+                                                           int nt, int nd, const double* Cdd, double* rowsum)
+{
+    __shared__ double red[256];
+    const int d = blockIdx.x;
+    const long long total = (long long)ns * nt;
+    double s = 0.0;
+    for (long long e = threadIdx.x; e < total; e += 256) {
+        const int t = (int)(e / ns), j = (int)(e % ns);
+        s += fabs(P[(long long)t * pstride + (long long)j * ldp + 2 * ns + d]);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int e2 = 0; e2 < nd; e2++)
+            if (e2 != d) v += fabs(Cdd[(long long)e2 * nd + d]);
+        rowsum[(long long)nt * ns + d] = v;
+    }
+}
+
+__global__ void dominance_kernel(double* P, long long pstride, int ldp, int ns, int nt, int nd, double* Cdd,
+                                 const double* rowsum)
+{
+    const long long n = (long long)ns * nt + nd;
+    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double v = rowsum[k] + 1.0;
+    if (k < (long long)ns * nt) {
+        const int t = (int)(k / ns), a = (int)(k % ns);
+        P[(long long)t * pstride + (long long)a * ldp + a] += v;
+    } else {
+        const int d = (int)(k - (long long)ns * nt);
+        Cdd[(long long)d * nd + d] += v;
+    }
+}
+
+static int grid_for(long long total) { return (int)std::min<long long>((total + 255) / 256, 16384); }
+
+int generate_matrix(const GenParams& g, double* P, long long pstride, int ldp, double* Cdd, double* tmp,
+                    double* rowsum, hipStream_t s)
+{
+    const int ns = g.ns, nt = g.nt, nd = g.nd;
+    int rc;
+    for (int t = 0; t < nt; t++) {
+        double* Pt = P + (long long)t * pstride;
+        // G_t -> tmp, B_t = theta_d/ns * G G^T (lower), mirrored, + theta_d on the diagonal
+        hipLaunchKernelGGL(gen_fill_kernel, dim3(grid_for((long long)ns * ns)), dim3(256), 0, s, tmp, ns, ns,
+                           ns, 0LL, g.seed, g.matrix_id, 1u, (uint32_t)t, 1.0);
+        if ((rc = (int)hipGetLastError())) return rc;
+        GemmDesc d = gdesc(tmp, ns, tmp, ns, Pt, ldp, ns, ns, ns, g.theta_d / (double)ns, 0.0, G_CLOWER);
+        if ((rc = gemm_f64(Op::N, Op::T, &d, 1, s))) return rc;
+        if ((rc = mirror_lower(Pt, ldp, ns, s))) return rc;
+        hipLaunchKernelGGL(diag_add_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, Pt, ldp, ns, g.theta_d);
+        if ((rc = (int)hipGetLastError())) return rc;
+        if (t < nt - 1) {
+            hipLaunchKernelGGL(gen_fill_kernel, dim3(grid_for((long long)ns * ns)), dim3(256), 0, s, Pt + ns, ldp,
+                               ns, ns, 0LL, g.seed, g.matrix_id, 2u, (uint32_t)t, g.theta_c * g.sigma_c);
+            if ((rc = (int)hipGetLastError())) return rc;
+        } else {
+            if ((rc = fill2d(Pt + ns, ldp, ns, ns, 0.0, s))) return rc;
+        }
+        if (nd > 0) {
+            hipLaunchKernelGGL(gen_fill_kernel, dim3(grid_for((long long)nd * ns)), dim3(256), 0, s, Pt + 2 * ns,
+                               ldp, nd, ns, (long long)t * ns, g.seed, g.matrix_id, 3u, 0u, g.sigma_a);
+            if ((rc = (int)hipGetLastError())) return rc;
+        }
+    }
+    if (nd > 0) {
+        hipLaunchKernelGGL(gen_fill_kernel, dim3(1), dim3(256), 0, s, tmp, nd, nd, nd, 0LL, g.seed, g.matrix_id,
+                           4u, 0u, 1.0);
+        if ((rc = (int)hipGetLastError())) return rc;
+        GemmDesc d = gdesc(tmp, nd, tmp, nd, Cdd, nd, nd, nd, nd, 1.0, 0.0, G_CLOWER);
+        if ((rc = gemm_f64(Op::N, Op::T, &d, 1, s))) return rc;
+        if ((rc = mirror_lower(Cdd, nd, nd, s))) return rc;
+        hipLaunchKernelGGL(diag_add_kernel, dim3(1), dim3(256), 0, s, Cdd, nd, nd, (double)nd);
+        if ((rc = (int)hipGetLastError())) return rc;
+    }
+    hipLaunchKernelGGL(rowsum_block_kernel, dim3((ns + 255) / 256, nt), dim3(256), 0, s, P, pstride, ldp, ns, nt,
+                       nd, rowsum);
+    if ((rc = (int)hipGetLastError())) return rc;
+    if (nd > 0) {
+        hipLaunchKernelGGL(rowsum_arrow_kernel, dim3(nd), dim3(256), 0, s, P, pstride, ldp, ns, nt, nd, Cdd,
+                           rowsum);
+        if ((rc = (int)hipGetLastError())) return rc;
+    }
+    const long long n = (long long)ns * nt + nd;
+    hipLaunchKernelGGL(dominance_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, pstride, ldp, ns,
+                       nt, nd, Cdd, rowsum);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------ CSC scatter
+__global__ void scatter_csc_kernel(const uint64_t* __restrict__ ia, const uint64_t* __restrict__ ja,
+                                   const double* __restrict__ a, long long nnz, int ns, int nt, int nd,
+                                   double* P, long long pstride, int ldp, double* Cdd)
+{
+    const long long n = (long long)ns * nt + nd;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nnz;
+         k += (long long)gridDim.x * blockDim.x) {
+        // column c: largest c with ia[c] <= k
+        long long lo = 0, hi = n;   // invariant ia[lo] <= k < ia[hi]
+        while (hi - lo > 1) {
+            long long mid = (lo + hi) >> 1;
+            if (ia[mid] <= (uint64_t)k) lo = mid; else hi = mid;
+        }
+        const long long c = lo, r = (long long)ja[k];
+        const double v = a[k];
+        const long long sn = (long long)ns * nt;
+        if (c < sn) {
+            const int t = (int)(c / ns), cc = (int)(c % ns);
+            double* Pt = P + (long long)t * pstride;
+            if (r >= sn) {
+                Pt[(long long)cc * ldp + 2 * ns + (r - sn)] = v;
+            } else if (r / ns == t) {
+                const int rr = (int)(r - (long long)t * ns);
+                Pt[(long long)cc * ldp + rr] = v;
+                Pt[(long long)rr * ldp + cc] = v;
+            } else {
+                Pt[(long long)cc * ldp + ns + (r - (long long)(t + 1) * ns)] = v;
+            }
+        } else {
+            const int dc = (int)(c - sn), dr = (int)(r - sn);
+            Cdd[(long long)dc * nd + dr] = v;
+            Cdd[(long long)dr * nd + dc] = v;
+        }
+    }
+}
+
+int scatter_csc(const uint64_t* ia, const uint64_t* ja, const double* a, long long nnz, int ns, int nt, int nd,
+                double* P, long long pstride, int ldp, double* Cdd, hipStream_t s)
+{
+    if (nnz <= 0) return 0;
+    hipLaunchKernelGGL(scatter_csc_kernel, dim3(grid_for(nnz)), dim3(256), 0, s, ia, ja, a, nnz, ns, nt, nd, P,
+                       pstride, ldp, Cdd);
+    return (int)hipGetLastError();
+}
+
+}  // namespace rgf

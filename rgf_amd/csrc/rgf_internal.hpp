@@ -1,0 +1,117 @@
+// Internal declarations shared by the HIP translation units of librgf_hip.so.
+// Everything here is gfx950 (CDNA4) code; no other target is built.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define RGF_HIP_CHECK(expr)                                                        \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) return rgf::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+namespace rgf {
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+// ---------------------------------------------------------------- GEMM
+// C = alpha * op(A) * op(B) + beta * Cin, column-major, FP64 on MFMA.
+// Flags restrict the k-range per output tile for triangular operands (the
+// operand must hold explicit zeros in its other triangle; skipping is an
+// optimisation, not a mask) and restrict stores to the lower triangle of C.
+enum GemmFlags : int {
+    G_CLOWER = 1,   // store only C[i][j] with i >= j
+    G_ALOWER = 2,   // op(A)[i][k] == 0 for k > i
+    G_AUPPER = 4,   // op(A)[i][k] == 0 for k < i
+    G_BLOWER = 8,   // op(B)[k][j] == 0 for k < j
+    G_BUPPER = 16,  // op(B)[k][j] == 0 for k > j
+};
+
+struct GemmDesc {
+    const double* A;
+    const double* B;
+    const double* Cin;   // may alias C; ignored when beta == 0
+    double* C;
+    long long sA, sB, sC, sCin;   // strides between batch entries (elements)
+    int m, n, k;
+    int lda, ldb, ldc, ldcin;
+    int flags;
+    int count;                    // batch entries described by this desc
+    double alpha, beta;
+    // filled by the launcher
+    int tiles_m, tiles_n, zbase, pad_;
+};
+
+constexpr int GEMM_MAXD = 4;
+struct GemmArgs {
+    int nd;
+    int pad_[3];
+    GemmDesc d[GEMM_MAXD];
+};
+
+enum class Op { N, T };
+
+// Launch one GEMM "call" made of up to GEMM_MAXD descriptors that share the
+// transposes (all entries are independent).  Returns hipError_t as int.
+int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
+
+// Convenience for a single, non-batched product.
+inline GemmDesc gdesc(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int m,
+                      int n, int k, double alpha, double beta, int flags = 0)
+{
+    GemmDesc d{};
+    d.A = A; d.B = B; d.C = C; d.Cin = C;
+    d.lda = lda; d.ldb = ldb; d.ldc = ldc; d.ldcin = ldc;
+    d.m = m; d.n = n; d.k = k; d.alpha = alpha; d.beta = beta; d.flags = flags; d.count = 1;
+    return d;
+}
+
+// ---------------------------------------------------------------- dense block kernels
+int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base,
+              hipStream_t s);
+int trtri_f64_inplace(double* A, int lda, int n, double* tmp, hipStream_t s);
+int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
+           hipStream_t s);
+int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s);
+// Corner: Cout = Cin - sum_i D_i D_i^T (D_i: nd x ns at Dbase + i*dstride, ld ldd),
+// then in-place Cholesky of the nd x nd corner, its inverse X_dd (written to Xdd),
+// log-det part and info.  nd <= 64.
+int corner_factor(const double* Dbase, long long dstride, int ldd, int ns, int nt, int nd,
+                  const double* Cin, int ldcin, double* Ldd, double* Xdd, double* partial,
+                  double* logdet_slot, int* info, int info_base, hipStream_t s);
+
+// ---------------------------------------------------------------- skinny products (solves)
+// Y = alpha * op(A) * R + beta * Y0, A is m x k (N) or k x m (T) col-major,
+// R is k x nr (ld ldr), Y is m x nr (ld ldy), nr <= 16.  tri: 0 none, 1 = op(A) lower,
+// 2 = op(A) upper.  Uses a split-k partial buffer `work` of size >= skinny_work(m, k, nr).
+size_t skinny_work_elems(int m, int k, int nr);
+int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
+               const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
+               double* work, hipStream_t s);
+
+// ---------------------------------------------------------------- generator / ingestion
+struct GenParams {
+    uint64_t seed;
+    uint32_t matrix_id;
+    int ns, nt, nd;
+    double sigma_c, sigma_a, theta_d, theta_c;
+};
+// Writes A (lower diag blocks mirrored to full, couplings, arrow, lower corner mirrored)
+// into panels P (panel stride pstride, ld ldp) and corner C (ld nd).  tmp: ns*ns doubles,
+// rowsum: n doubles.
+int generate_matrix(const GenParams& g, double* P, long long pstride, int ldp, double* Cdd,
+                    double* tmp, double* rowsum, hipStream_t s);
+int scatter_csc(const uint64_t* ia, const uint64_t* ja, const double* a, long long nnz, int ns,
+                int nt, int nd, double* P, long long pstride, int ldp, double* Cdd, hipStream_t s);
+int mirror_lower(double* A, int lda, int n, hipStream_t s);
+
+// ---------------------------------------------------------------- selinv helpers
+// Finish one Takahashi step: mirror the lower triangle of S (ns x ns, ld lds) into
+// its upper triangle, write diag(S) to diag_out, and (exact mode) store
+// S_{d,i} = -Z[zrow0:zrow0+nd, :] into S[ns:ns+nd, 0:ns] and its transpose.
+int selinv_finish(double* S, int lds, int ns, int nd, const double* Z, int ldz, int zrow0,
+                  int exact, double* diag_out, hipStream_t s);
+int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s);
+
+}  // namespace rgf

@@ -1,0 +1,67 @@
+"""Test inputs shared by the CPU and GPU suites (formulas only; no reference data)."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+from oracle import oracle
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def goldens() -> dict:
+    with open(os.path.join(GOLDEN_DIR, "goldens.json")) as f:
+        return json.load(f)
+
+
+def rhs_formula(n: int, nrhs: int = 1) -> np.ndarray:
+    """Deterministic right-hand side used by the golden vectors: b[k, j] = cos(k + 0.5 j) + 0.25."""
+    k = np.arange(n)[:, None]
+    j = np.arange(nrhs)[None, :]
+    return np.cos(k + 0.5 * j) + 0.25
+
+
+def spde_like(nx: int = 6, ny: int = 7, nt: int = 3, nd: int = 0, kappa2: float = 0.3, rho: float = 0.9):
+    """INLA-style spatio-temporal precision (the class of data/A_126_126_ns42_nt3.dat):
+    Q_s = K K with K = kappa^2 I + (grid Laplacian of nx x ny), temporal AR(1) precision
+    Q_t (tridiagonal), Q = Q_t (x) Q_s: block tridiagonal with sparse blocks, SPD, not
+    diagonally dominant.  With nd > 0, nd dense fixed-effect rows are appended (small
+    covariates, corner 1 + sum of their squares), which keeps it SPD.
+    Returns (A dense, ns, nt, nd)."""
+    ns = nx * ny
+    Lg = np.zeros((ns, ns))
+    for x in range(nx):
+        for y in range(ny):
+            i = x * ny + y
+            for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                xx, yy = x + dx, y + dy
+                if 0 <= xx < nx and 0 <= yy < ny:
+                    Lg[i, i] += 1.0
+                    Lg[i, xx * ny + yy] -= 1.0
+    K = kappa2 * np.eye(ns) + Lg
+    Qs = K @ K
+    Qt = np.zeros((nt, nt))
+    for t in range(nt):
+        Qt[t, t] = 1.0 + (rho * rho if 0 < t < nt - 1 else 0.0)
+        if t + 1 < nt:
+            Qt[t, t + 1] = Qt[t + 1, t] = -rho
+    Qt += 0.01 * np.eye(nt)
+    Q = np.kron(Qt, Qs)
+    n = ns * nt + nd
+    A = np.zeros((n, n))
+    A[:ns * nt, :ns * nt] = Q
+    if nd > 0:
+        rng = np.random.default_rng(1234)
+        Xc = 0.05 * rng.standard_normal((nd, ns * nt))
+        A[ns * nt:, :ns * nt] = Xc
+        A[:ns * nt, ns * nt:] = Xc.T
+        # Schur complement of the arrow stays SPD: corner >> X Q^-1 X^T
+        A[ns * nt:, ns * nt:] = np.eye(nd) * (1.0 + 10.0 * np.abs(Xc).sum())
+    return A, ns, nt, nd
+
+
+def closed_form(ns: int, nt: int, nd: int):
+    ia, ja, a = oracle.closed_form_csc(ns, nt, nd)
+    return ia, ja, a

@@ -1,0 +1,63 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol, and
+rejects bad input before touching a device."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import rgf_amd
+from rgf_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rgf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for header in ("rgf.h", "rgf_testing.h"):
+        names = _declared(header)
+        assert names
+        for name in names:
+            assert hasattr(lib, name), f"{name} declared in {header} but not exported"
+    assert set(_declared("rgf.h")) == set(_lib.EXPORTS)
+    assert set(_declared("rgf_testing.h")) == set(_lib.TEST_EXPORTS)
+
+
+def test_defaults():
+    o = rgf_amd.default_options()
+    assert (o.precision, o.selinv_mode, o.keep_matrix) == (rgf_amd.RGF_FP64, rgf_amd.RGF_SELINV_EXACT, 1)
+    g = rgf_amd.default_gen_params()
+    assert (g.sigma_c, g.sigma_a, g.theta_d, g.theta_c) == (0.1, 0.05, 1.0, 1.0)
+
+
+def test_bad_pattern_rejected_without_device():
+    # (7, 0) lies outside the block-arrow pattern of ns=3, nt=3, nd=0
+    ia = np.array([0, 2, 3, 4, 5, 6, 7, 8, 9, 10], dtype=np.uint64)
+    ja = np.array([0, 7, 1, 2, 3, 4, 5, 6, 7, 8], dtype=np.uint64)
+    with pytest.raises(rgf_amd.RGFError) as e:
+        rgf_amd.RGF(ia, ja, np.ones(10), 3, 3, 0)
+    assert e.value.code == rgf_amd.RGF_EINVAL
+    assert "outside" in str(e.value)
+
+
+def test_bad_shapes_rejected():
+    ia = np.zeros(3 * 3 + 4 + 1, dtype=np.uint64)
+    with pytest.raises(rgf_amd.RGFError) as e:
+        rgf_amd.RGF(ia, np.zeros(0, np.uint64), np.zeros(0), 3, 3, 4)   # nd > ns
+    assert e.value.code == rgf_amd.RGF_EINVAL
+    with pytest.raises(rgf_amd.RGFError):
+        rgf_amd.RGF(np.zeros(5, np.uint64), np.zeros(0, np.uint64), np.zeros(0), 3, 3, 0)   # wrong ia length
+
+
+def test_canonical_flops_closed_form():
+    F, S = rgf_amd.canonical_flops(2865, 100, 6)
+    assert F + S == pytest.approx(1.636e13, rel=2e-3)
+    F1, S1 = rgf_amd.canonical_flops(1000, 50, 4)
+    assert F1 + S1 == pytest.approx(3.456e11, rel=2e-3)
+    assert rgf_amd.mf_size(42, 3, 0) == 42 * 42 * 5

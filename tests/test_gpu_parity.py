@@ -1,0 +1,309 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the goldens.
+
+Bar: FP64 logdet to rel 1e-10 (tests use 1e-11 at small sizes), solves to 1e-9,
+diag(A^-1) to 1e-9 (see each assert)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import rgf_amd
+from rgf_amd import _lib
+from oracle import oracle
+import fixtures
+
+pytestmark = pytest.mark.gpu
+G = fixtures.goldens()
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dptr(a):
+    import ctypes as C
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ------------------------------------------------------------------ kernels
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(16, 16, 4), (64, 64, 16), (130, 70, 33), (5, 200, 129)])
+def test_gemm_kernel(gpu, ta, tb, m, n, k):
+    say(f"gemm ta={ta} tb={tb} m={m} n={n} k={k}")
+    rng = np.random.default_rng(m * 7 + n + k)
+    lda, ldb, ldc = (k if ta else m) + 3, (n if tb else k) + 1, m + 2
+    A = np.asfortranarray(rng.standard_normal((lda, m if ta else k)))
+    B = np.asfortranarray(rng.standard_normal((ldb, k if tb else n)))
+    C = np.asfortranarray(rng.standard_normal((ldc, n)))
+    opA = (A[:k, :m].T if ta else A[:m, :k])
+    opB = (B[:n, :k].T if tb else B[:k, :n])
+    ref = C.copy()
+    ref[:m, :n] = 1.5 * opA @ opB - 0.5 * C[:m, :n]
+    Cf = C.reshape(-1, order="F").copy()
+    rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.5, dptr(A.reshape(-1, order="F").copy()), lda,
+                                    dptr(B.reshape(-1, order="F").copy()), ldb, -0.5, dptr(Cf), ldc, 0)
+    assert rc == 0
+    got = Cf.reshape(ldc, n, order="F")
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * k
+
+
+@pytest.mark.parametrize("flags", [1, 2, 8, 16, 1 | 4 | 8])
+def test_gemm_kernel_triangular_flags(gpu, flags):
+    say(f"gemm flags={flags}")
+    rng = np.random.default_rng(flags)
+    m = n = k = 150
+    L = np.tril(rng.standard_normal((k, k)))
+    G0 = rng.standard_normal((k, k))
+    # choose operands so that the flagged operand really is triangular
+    if flags == 2:     # op(A) lower (N): A = L
+        ta, tb, A, B = 0, 0, L, G0
+    elif flags == 8:   # op(B) lower (N): B = L
+        ta, tb, A, B = 0, 0, G0, L
+    elif flags == 16:  # op(B) upper: op(B) = L^T with tb = 1
+        ta, tb, A, B = 0, 1, G0, L
+    elif flags == 1 | 4 | 8:   # L^T L, lower stores
+        ta, tb, A, B = 1, 0, L, L
+    else:              # lower stores only
+        ta, tb, A, B = 0, 1, G0, G0
+    opA = A.T if ta else A
+    opB = B.T if tb else B
+    C0 = np.full((m, n), 7.0)
+    ref = opA @ opB
+    if flags & 1:
+        ref = np.where(np.tril(np.ones((m, n))) > 0, ref, 7.0)
+    Cf = C0.reshape(-1, order="F").copy()
+    rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.0, dptr(A.reshape(-1, order="F").copy()), k,
+                                    dptr(B.reshape(-1, order="F").copy()), k, 0.0, dptr(Cf), m, flags)
+    assert rc == 0
+    assert np.abs(Cf.reshape(m, n, order="F") - ref).max() < 1e-11
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 333])
+def test_potrf_trtri_kernels(gpu, n):
+    say(f"potrf/trtri n={n}")
+    rng = np.random.default_rng(n)
+    Gm = rng.standard_normal((n, n))
+    A = Gm @ Gm.T / n + np.eye(n)
+    lda = n + 5
+    buf = np.zeros((lda, n), order="F")
+    buf[:n, :n] = A
+    flat = buf.reshape(-1, order="F").copy()
+    import ctypes as C
+    ld = C.c_double()
+    assert _lib.load().rgf_debug_potrf(n, dptr(flat), lda, 0, C.byref(ld)) == 0
+    L = np.tril(flat.reshape(lda, n, order="F")[:n])
+    Lc = np.linalg.cholesky(A)
+    assert np.abs(L - Lc).max() < 1e-12 * np.abs(Lc).max() * n
+    assert ld.value == pytest.approx(np.linalg.slogdet(A)[1], rel=1e-12, abs=1e-12)
+    flat = buf.reshape(-1, order="F").copy()
+    assert _lib.load().rgf_debug_potrf(n, dptr(flat), lda, 1, C.byref(ld)) == 0
+    X = flat.reshape(lda, n, order="F")[:n]
+    assert np.abs(np.triu(X, 1)).max() == 0.0   # explicit zeros above the diagonal
+    Xr = np.linalg.inv(Lc)
+    assert np.abs(X - Xr).max() < 1e-11 * np.abs(Xr).max()
+
+
+def test_potrf_not_spd(gpu):
+    say("potrf not SPD")
+    n = 100
+    A = np.eye(n)
+    A[70, 70] = -1.0
+    flat = A.reshape(-1, order="F").copy()
+    import ctypes as C
+    ld = C.c_double()
+    assert _lib.load().rgf_debug_potrf(n, dptr(flat), n, 0, C.byref(ld)) == rgf_amd.RGF_ENOTSPD
+    assert b"column 71" in _lib.load().rgf_last_error(None)
+
+
+# ------------------------------------------------------------------ full path vs oracle
+SHAPES = [(42, 3, 0), (8, 5, 3), (65, 4, 5), (130, 3, 8), (64, 1, 4), (100, 1, 0), (1, 5, 1), (3, 3, 2),
+          (200, 6, 6), (129, 2, 0)]
+
+
+@pytest.mark.parametrize("ns,nt,nd", SHAPES)
+def test_generated_vs_oracle(gpu, ns, nt, nd):
+    say(f"generated ns={ns} nt={nt} nd={nd}")
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=1234 + ns) as s:
+        A_mf = s.export_matrix_mf()
+        # device generator vs host restatement of the same definition
+        H_mf = oracle.generate(1234 + ns, ns, nt, nd)
+        assert np.abs(A_mf - H_mf).max() <= 1e-13 * np.abs(H_mf).max()
+        L = oracle.factor(A_mf, ns, nt, nd)
+        ld = s.logDet()
+        ld0 = oracle.logdet(L, ns, nt, nd)
+        assert abs(ld - ld0) <= 1e-11 * abs(ld0)
+        # factor blocks: diagonal blocks hold L_ii^-1
+        F = s.export_factor_mf()
+        Ld = oracle.mf_to_dense(L, ns, nt, nd, symmetric=False)
+        Fd = oracle.mf_to_dense(F, ns, nt, nd, symmetric=False)
+        n = ns * nt + nd
+        for t in range(nt):
+            sl = slice(t * ns, (t + 1) * ns)
+            Xr = np.linalg.inv(Ld[sl, sl])
+            assert np.abs(Fd[sl, sl] - Xr).max() <= 1e-11 * np.abs(Xr).max()
+        off = Ld.copy()
+        offF = Fd.copy()
+        for t in range(nt):
+            sl = slice(t * ns, (t + 1) * ns)
+            off[sl, sl] = 0
+            offF[sl, sl] = 0
+        off[nt * ns:, nt * ns:] = 0
+        offF[nt * ns:, nt * ns:] = 0
+        assert np.abs(offF - off).max() <= 1e-11 * max(1.0, np.abs(off).max())
+        for nrhs in (1, 3, 16, 17):
+            b = np.random.default_rng(nrhs).standard_normal((n, nrhs))
+            x = s.solve(None, b, nrhs)
+            x0 = oracle.solve(L, ns, nt, nd, b)
+            assert np.abs(x - x0).max() <= 1e-10 * np.abs(x0).max(), nrhs
+        d = s.RGFdiag()
+        d0 = oracle.selinv(L, ns, nt, nd, exact=True)
+        assert np.abs(d / d0 - 1).max() <= 1e-10
+        # factor still valid after RGFdiag (solve again)
+        b = np.ones((n, 1))
+        assert np.abs(s.solve(None, b, 1) - oracle.solve(L, ns, nt, nd, b)).max() <= 1e-10 * np.abs(
+            oracle.solve(L, ns, nt, nd, b)).max()
+
+
+@pytest.mark.parametrize("ns,nt,nd", [(8, 5, 3), (65, 4, 5), (3, 3, 2)])
+def test_reference_compat_selinv(gpu, ns, nt, nd):
+    say(f"compat selinv ns={ns} nt={nt} nd={nd}")
+    opts = rgf_amd.default_options(selinv_mode=rgf_amd.RGF_SELINV_REFERENCE)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=7, options=opts) as s:
+        A_mf = s.export_matrix_mf()
+        d = s.RGFdiag()
+    L = oracle.factor(A_mf, ns, nt, nd)
+    d0 = oracle.selinv(L, ns, nt, nd, exact=False)
+    assert np.abs(d / d0 - 1).max() <= 1e-10
+
+
+def _csc_case(name):
+    if name.startswith("closed"):
+        ns, nt, nd = (3, 3, 0) if name == "closed_9x9" else (3, 3, 2)
+        ia, ja, a = oracle.closed_form_csc(ns, nt, nd)
+    else:
+        A, ns, nt, nd = fixtures.spde_like(nd=G[name]["nd"])
+        ia, ja, a = oracle.dense_to_csc_lower(A, ns, nt, nd)
+    return ia, ja, a, ns, nt, nd
+
+
+@pytest.mark.parametrize("name", ["closed_9x9", "closed_11x11", "spde_126", "spde_126_nd2"])
+def test_csc_fixture_goldens(gpu, name):
+    say(f"csc fixture {name}")
+    ia, ja, a, ns, nt, nd = _csc_case(name)
+    g = G[name]
+    n = ns * nt + nd
+    with rgf_amd.RGF(ia, ja, a, ns, nt, nd) as s:
+        s.factorize()
+        assert s.logDet() == pytest.approx(g["logdet"], rel=1e-11)
+        b = fixtures.rhs_formula(n, 2)
+        x = s.solve(None, b, 2)
+        xg = np.array(g["x_rhs_formula_2"]).reshape(n, 2, order="F")
+        assert np.abs(x - xg).max() <= 1e-9 * np.abs(xg).max()
+        d = s.RGFdiag()
+        assert np.abs(d / np.array(g["diag_inv"]) - 1).max() <= 1e-9
+        r = s.residualNormNormalized(x[:, 0].copy(), b[:, 0].copy())
+        r0 = oracle.residual_norm(ia, ja, a, n, x[:, 0].copy(), b[:, 0].copy(), normalized=True)
+        assert r < 1e-12 and r == pytest.approx(r0, rel=0.5, abs=1e-15)
+    opts = rgf_amd.default_options(selinv_mode=rgf_amd.RGF_SELINV_REFERENCE)
+    with rgf_amd.RGF(ia, ja, a, ns, nt, nd, opts) as s:
+        d = s.RGFdiag()
+        assert np.abs(d / np.array(g["diag_reference"]) - 1).max() <= 1e-9
+
+
+def test_in_place_mode(gpu):
+    say("keep_matrix=0")
+    opts = rgf_amd.default_options(keep_matrix=0)
+    ns, nt, nd = 70, 4, 3
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=3, options=opts) as s, \
+            rgf_amd.RGF.generated(ns, nt, nd, seed=3) as k:
+        assert s.logDet() == pytest.approx(k.logDet(), rel=1e-14)
+        n = ns * nt + nd
+        b = np.ones((n, 1))
+        assert np.abs(s.solve(None, b, 1) - k.solve(None, b, 1)).max() < 1e-13
+        assert np.abs(s.RGFdiag() - k.RGFdiag()).max() < 1e-13
+        with pytest.raises(rgf_amd.RGFError) as e:
+            s.residualNorm(np.ones(n), np.ones(n))
+        assert e.value.code == rgf_amd.RGF_ESTATE
+
+
+def test_not_spd_reports_column(gpu):
+    say("not SPD through the API")
+    ns, nt, nd = 3, 3, 0
+    ia, ja, a = oracle.closed_form_csc(ns, nt, nd)
+    a = a.copy()
+    # make A[4,4] tiny so the Schur complement of block 1 goes negative at its second pivot
+    idx = [k for c in range(9) for k in range(int(ia[c]), int(ia[c + 1])) if c == 4 and ja[k] == 4][0]
+    a[idx] = -50.0
+    with rgf_amd.RGF(ia, ja, a, ns, nt, nd) as s:
+        with pytest.raises(rgf_amd.RGFError) as e:
+            s.factorize()
+        assert e.value.code == rgf_amd.RGF_ENOTSPD
+        assert "column 5" in str(e.value)
+
+
+def test_device_entry_points_and_determinism(gpu):
+    say("device entry points")
+    ns, nt, nd = 150, 5, 4
+    n = ns * nt + nd
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=77) as s:
+        b = np.random.default_rng(1).standard_normal((n, 5))
+        bd = rgf_amd.DeviceBuffer(n * 5)
+        xd = rgf_amd.DeviceBuffer(n * 5)
+        bd.upload(b.reshape(-1, order="F"))
+        s.solve_device(xd, bd, 5)
+        s.synchronize()
+        x1 = xd.download().reshape(n, 5, order="F")
+        assert np.array_equal(x1, s.solve(None, b, 5))
+        dd = rgf_amd.DeviceBuffer(n)
+        s.selinv_diag_device(dd)
+        s.synchronize()
+        d1 = dd.download()
+        s.factorize()
+        assert np.array_equal(d1, s.RGFdiag())   # bitwise reproducible
+        info = s.info()
+        F, S = rgf_amd.canonical_flops(ns, nt, nd)
+        assert info.flops_factor == pytest.approx(F) and info.flops_selinv == pytest.approx(S)
+
+
+def test_larger_block_properties(gpu):
+    """ns = 1000 (the INLA-like single-matrix block size) on 5 blocks: parity with the
+    oracle on the same matrix and a small residual."""
+    ns, nt, nd = 1000, 5, 4
+    say(f"larger ns={ns} nt={nt} nd={nd}")
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=2024) as s:
+        A_mf = s.export_matrix_mf()
+        n = s.n
+        b = np.random.default_rng(5).standard_normal(n)
+        x = s.solve(None, b, 1)
+        assert s.residualNormNormalized(x, b) < 1e-14
+        ld = s.logDet()
+        d = s.RGFdiag()
+    L = oracle.factor(A_mf, ns, nt, nd)
+    assert abs(ld - oracle.logdet(L, ns, nt, nd)) <= 1e-12 * abs(ld)
+    assert np.abs(d / oracle.selinv(L, ns, nt, nd, exact=True) - 1).max() <= 1e-11
+
+
+def test_cli_matches_reference_driver_format(gpu, tmp_path):
+    say("RGFSolver CLI")
+    ia, ja, a = oracle.closed_form_csc(3, 3, 2)
+    mpath = tmp_path / "A_11.dat"
+    oracle.write_dat(str(mpath), 11, ia, ja, a)
+    b = fixtures.rhs_formula(11)[:, 0]
+    bpath = tmp_path / "b.dat"
+    bpath.write_text("\n".join(repr(float(v)) for v in b) + "\n")
+    out = subprocess.run([os.path.join(ROOT, "rgf_amd", "RGFSolver"), "3", "3", "2", str(mpath), str(bpath)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    full = [l for l in lines if l.startswith("logdet (full precision):")][0]
+    assert float(full.split(":")[1]) == pytest.approx(G["closed_11x11"]["logdet"], rel=1e-12)
+    for key in ("factorize time:", "solve time:", "logdet time:", "RGF time:", "Residual norm:",
+                "Residual norm normalized:"):
+        assert any(l.startswith(key) for l in lines), key
+    x = np.array([float(l) for l in lines[-11:]])
+    xg = np.array(G["closed_11x11"]["x_rhs_formula_2"])[:11]
+    assert np.abs(x - xg).max() <= 1e-12 * np.abs(xg).max()
