@@ -1,0 +1,167 @@
+"""Benchmark: factorize + logdet + selected inversion (diag(A^-1)) per matrix on one MI355X.
+
+A step is one pass of the hot path over one synthetic matrix resident in HBM:
+factorize (block Cholesky with L_ii^-1), logdet, and the Takahashi recursion for
+diag(A^-1), output left in device memory.  Default workload: ns=2865, nt=100, nd=6
+(n = 286506), FP64, generated on the device from a seed (no dataset).
+
+Prints one JSON line with value = canonical algorithmic TFLOP/s of the step,
+the FP64-MFMA roofline of the dominant GEMM class measured with HIP events on the
+library's stream, and the CPU baseline (the oracle's LAPACK restatement of the
+reference's block algorithm on the host cores, on a bounded sample of blocks).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (ns, nt, nd)
+    "c2": (2865, 100, 6),
+    "c1": (1000, 50, 4),
+    "c0": (42, 3, 0),
+}
+PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+PEAK_HBM = 8000.0       # GB/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-blocks", type=int, default=16, help="blocks of the CPU baseline sample")
+    p.add_argument("--no-kernel-timing", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(ns, nt, nd, seed, blocks):
+    """The oracle (reference block algorithm on host MKL) on `blocks` of the workload's
+    nt blocks; throughput in the same unit as the GPU value."""
+    import numpy as np  # noqa: F401
+    from oracle import oracle
+    import rgf_amd
+    nts = max(2, min(blocks, nt))
+    mf = oracle.generate(seed, ns, nts, nd)
+    t0 = time.perf_counter()
+    L = oracle.factor(mf, ns, nts, nd)
+    oracle.logdet(L, ns, nts, nd)
+    oracle.selinv(L, ns, nts, nd, exact=True)
+    dt = time.perf_counter() - t0
+    F, S = rgf_amd.canonical_flops(ns, nts, nd)
+    threads = int(os.environ.get("MKL_NUM_THREADS", "1"))
+    return {
+        "value": round((F + S) / dt / 1e12, 4),
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"factorize+logdet+selinv(exact) of ns={ns} nt={nts} nd={nd} ({nts} of {nt} blocks) "
+                  f"on host MKL, {dt:.2f} s; s/matrix extrapolated to nt={nt}: {dt / nts * nt:.2f}",
+        "s_per_matrix_extrapolated": round(dt / nts * nt, 3),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and not launched here.", file=sys.stderr)
+        sys.exit(2)
+    # host threads for the CPU baseline: the box's share (16), set before MKL loads
+    os.environ.setdefault("MKL_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
+    import numpy as np
+    import rgf_amd
+
+    ns, nt, nd = WORKLOADS[args.workload]
+    F, S = rgf_amd.canonical_flops(ns, nt, nd)
+    t_gen = time.perf_counter()
+    s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed)
+    s.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    diag = rgf_amd.DeviceBuffer(s.n)
+
+    def step():
+        s.factorize()
+        ld = s.logDet()
+        s.selinv_diag_device(diag)
+        return ld
+
+    for _ in range(args.warmup):
+        step()
+    s.synchronize()
+    if not args.no_kernel_timing:
+        rgf_amd.kernel_timing_enable(0)   # NN class: trmm W = F X, Z = Sb W, trtri merges
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ld = step()
+    s.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    kms, kfl, kn = rgf_amd.kernel_timing_read() if not args.no_kernel_timing else (0.0, 0.0, 0)
+    rgf_amd.kernel_timing_enable(-1)
+
+    # per-phase times (one extra pass, outside the timed region)
+    ph = {}
+    t = time.perf_counter(); s.factorize(); s.synchronize(); ph["factorize_ms"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter(); s.logDet(); ph["logdet_ms"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter(); s.selinv_diag_device(diag); s.synchronize(); ph["selinv_ms"] = (time.perf_counter() - t) * 1e3
+    b = rgf_amd.DeviceBuffer(s.n)
+    x = rgf_amd.DeviceBuffer(s.n)
+    b.upload(np.random.default_rng(1).standard_normal(s.n))
+    s.solve_device(x, b, 1)
+    s.synchronize()
+    t = time.perf_counter(); s.solve_device(x, b, 1); s.synchronize(); ph["solve_1rhs_ms"] = (time.perf_counter() - t) * 1e3
+    info = s.info()
+    ph["solve_1rhs_GBps"] = info.bytes_solve_per_rhs / (ph["solve_1rhs_ms"] * 1e-3) / 1e9
+    ph["factorize_TFLOPs"] = F / (ph["factorize_ms"] * 1e-3) / 1e12
+    ph["selinv_TFLOPs"] = S / (ph["selinv_ms"] * 1e-3) / 1e12
+    ph = {k: round(v, 4) for k, v in ph.items()}
+
+    achieved = (kfl / (kms * 1e-3) / 1e12) if kms > 0 else None
+    roofline = {
+        "bound": "mfma",
+        "kernel": "gemm_f64_kernel<false,false> (NN: W = F L^-1, Z = S_below W, trtri merges)",
+        "achieved": round(achieved, 3) if achieved else None,
+        "peak": PEAK_FP64_MFMA,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64_MFMA, 4) if achieved else None,
+        "traffic": None,
+        "launches": kn,
+        "avg_launch_ms": round(kms / kn, 4) if kn else None,
+    }
+    cpu = None if args.no_cpu_baseline else cpu_baseline(ns, nt, nd, args.seed, args.cpu_blocks)
+    out = {
+        "metric": "factorize+logdet+selinv s/matrix & TFLOP/s; batched req/s + p99, mixed vs FP64",
+        "value": round((F + S) / dt / 1e12, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (device Philox generator, seed %d)" % args.seed,
+        "config": {"workload": f"{args.workload}: FP64 factorize+logdet+selinv(diag), ns={ns} nt={nt} nd={nd} "
+                               f"n={ns * nt + nd}", "s_per_matrix": round(dt, 4),
+                   "canonical_flops": F + S, "precision": "fp64", "selinv": "exact",
+                   "logdet": ld, "device_bytes": info.device_bytes, "generate_s": round(t_gen, 3)},
+        "phases": ph,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -443,6 +443,31 @@ static double gen_normal(uint64_t seed, uint32_t matrix_id, uint32_t kind, uint3
     return (e & 1) ? r * sin(t) : r * cos(t);
 }
 
+/* normals of elements e0 .. e0+count-1 of one (kind, block) stream, one Philox call per pair */
+static void gen_normals(uint64_t seed, uint32_t matrix_id, uint32_t kind, uint32_t block, uint64_t e0,
+                        long count, double scale, double *out)
+{
+    long k = 0;
+    while (k < count) {
+        uint64_t e = e0 + (uint64_t)k;
+        uint64_t p = e >> 1;
+        uint32_t ctr[4] = {(uint32_t)p, (uint32_t)(p >> 32), (kind << 24) | block, matrix_id};
+        uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+        uint32_t w[4];
+        rgf_oracle_philox4x32_10(ctr, key, w);
+        double u1 = (double)(((uint64_t)(w[0] >> 5) << 26) | (w[1] >> 6)) * (1.0 / 9007199254740992.0);
+        double u2 = (double)(((uint64_t)(w[2] >> 5) << 26) | (w[3] >> 6)) * (1.0 / 9007199254740992.0);
+        double r = sqrt(-2.0 * log(1.0 - u1));
+        double t = RGF_TWO_PI * u2;
+        if ((e & 1) == 0) {
+            out[k++] = scale * (r * cos(t));
+            if (k < count) out[k++] = scale * (r * sin(t));
+        } else {
+            out[k++] = scale * (r * sin(t));
+        }
+    }
+}
+
 double rgf_oracle_gen_uniform(uint64_t seed, uint32_t matrix_id, uint32_t idx)
 {
     uint32_t ctr[4] = {idx, 0u, (5u << 24), matrix_id};
@@ -470,7 +495,7 @@ int rgf_oracle_generate(uint64_t seed, uint32_t matrix_id, long ns, long nt, lon
     for (long t = 0; t < nt; t++) {
         double *P = mf + snode(&L, t);
         long ldp = lda_of(&L, t);
-        for (long e = 0; e < ns * ns; e++) G[e] = gen_normal(seed, matrix_id, 1, (uint32_t)t, (uint64_t)e);
+        gen_normals(seed, matrix_id, 1, (uint32_t)t, 0, ns * ns, 1.0, G);
         /* B_t = G G^T / ns + I, lower triangle only */
         gemm('N', 'T', ns, ns, ns, 1.0 / (double)ns, G, ns, G, ns, 0.0, P, ldp);
         for (long j = 0; j < ns; j++) {
@@ -478,20 +503,21 @@ int rgf_oracle_generate(uint64_t seed, uint32_t matrix_id, long ns, long nt, lon
             P[j + j * ldp] += 1.0;
             for (long i = j; i < ns; i++) P[i + j * ldp] *= theta_d;
         }
-        if (t < nt - 1)
+        if (t < nt - 1) {
+            gen_normals(seed, matrix_id, 2, (uint32_t)t, 0, ns * ns, theta_c * sigma_c, G);
             for (long j = 0; j < ns; j++)
-                for (long i = 0; i < ns; i++)
-                    P[ns + i + j * ldp] = theta_c * sigma_c *
-                        gen_normal(seed, matrix_id, 2, (uint32_t)t, (uint64_t)(i + j * ns));
-        for (long j = 0; j < ns; j++)
-            for (long d = 0; d < nd; d++)
-                P[arrow_off(&L, t) + d + j * ldp] = sigma_a *
-                    gen_normal(seed, matrix_id, 3, 0, (uint64_t)((t * ns + j) * nd + d));
+                for (long i = 0; i < ns; i++) P[ns + i + j * ldp] = G[i + j * ns];
+        }
+        if (nd > 0) {
+            gen_normals(seed, matrix_id, 3, 0, (uint64_t)(t * ns * nd), ns * nd, sigma_a, G);
+            for (long j = 0; j < ns; j++)
+                for (long d = 0; d < nd; d++) P[arrow_off(&L, t) + d + j * ldp] = G[d + j * nd];
+        }
     }
     if (nd > 0) {
         double *C = mf + L.diag_pos[nt * ns];
         double *H = G;   /* ns >= nd */
-        for (long e = 0; e < nd * nd; e++) H[e] = gen_normal(seed, matrix_id, 4, 0, (uint64_t)e);
+        gen_normals(seed, matrix_id, 4, 0, 0, nd * nd, 1.0, H);
         gemm('N', 'T', nd, nd, nd, 1.0, H, nd, H, nd, 0.0, C, nd);
         for (long j = 0; j < nd; j++) {
             for (long i = 0; i < j; i++) C[i + j * nd] = 0.0;

@@ -5,7 +5,7 @@
 // LDS (op(A) and op(B) stored k-major, one double of padding per row), with the
 // next step's global loads issued into registers before the current step's
 // MFMAs.  The MFMA is fed (op(B) fragment, op(A) fragment), so each lane's four
-// accumulator values are four C columns of one C row; the 16 lanes of a row
+// accumulator values are four C columns of one C row; the 16 lanes of a lane
 // group then store 16 consecutive rows of a column (a 128-byte segment) --
 // coalesced for column-major C.
 //
@@ -132,7 +132,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs args)
         __syncthreads();
     }
 
-    // epilogue: lane holds C[i = l16][j = 4*lq + r] of each 16x16 block
+    // epilogue: accumulator item r of lane (l16 + 16*lq) is D[4*r + lq][l16] of the
+    // 16x16 MFMA block (v_mfma_f64_16x16x4_f64 layout, checked on gfx950), i.e.
+    // C[i = l16][j = 4*r + lq]
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
 #pragma unroll
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs args)
         for (int b = 0; b < 2; b++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gj = j0 + wn * 32 + b * 16 + 4 * lq + r;
+                const int gj = j0 + wn * 32 + b * 16 + 4 * r + lq;
                 if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
                     double v = alpha * acc[a][b][r];
                     if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
