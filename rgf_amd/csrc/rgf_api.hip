@@ -59,6 +59,7 @@ struct rgf_handle_s {
     double* Xdd = nullptr;      // corner X_dd = L_dd^-1
     double* Ftmp = nullptr;     // (ns+nd) x ns staging of the rows below a diagonal block
     double* ttmp = nullptr;     // trtri merge scratch
+    double* xdiag = nullptr;    // inverted 64 x 64 diagonal tiles of the current block
     double* parts = nullptr;    // per-tile log-diagonal sums: nt*T + 1
     double* cpart = nullptr;    // corner partials nt * nd * nd
     int* dinfo = nullptr;       // first non-positive pivot (INT_MAX = none)
@@ -151,6 +152,7 @@ int alloc_core(rgf_handle_s* h)
         tt = std::max(tt, G * (size_t)(sz * 64) * (size_t)(sz * 64));
     }
     if ((rc = dalloc(h, &h->ttmp, tt))) return rc;
+    if ((rc = dalloc(h, &h->xdiag, (size_t)h->T * 64 * 64))) return rc;
     if ((rc = dalloc(h, &h->parts, (size_t)nt * h->T + 1))) return rc;
     if ((rc = dalloc(h, &h->cpart, (size_t)std::max(nt * nd * nd, 1)))) return rc;
     if ((rc = dalloc(h, &h->dinfo, 1))) return rc;
@@ -253,8 +255,8 @@ int factor_sweep(rgf_handle_s* h)
             }
             if ((rc = gemm_f64(Op::N, Op::T, d, ndesc, s))) return rc;
         }
-        if ((rc = potrf_f64(P, ldp, ns, h->parts + (long long)i * h->T, h->dinfo, i * ns, s))) return rc;
-        if ((rc = trtri_f64_inplace(P, ldp, ns, h->ttmp, s))) return rc;
+        if ((rc = potrf_f64(P, ldp, ns, h->parts + (long long)i * h->T, h->dinfo, i * ns, h->xdiag, s))) return rc;
+        if ((rc = trtri_f64_inplace(P, ldp, ns, h->ttmp, h->xdiag, s))) return rc;
         const bool has_next = i < nt - 1;
         if (has_next && (rc = copy2d(A + ns, ldp, h->Ftmp, ldf, ns, ns, 1.0, s))) return rc;
         const int mF = (has_next ? ns : 0) + nd;
@@ -822,8 +824,9 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
         size_t G = (size_t)((T - sz + 2 * sz - 1) / (2 * sz));
         tt = std::max(tt, G * (size_t)(sz * 64) * (size_t)(sz * 64));
     }
-    DevBuf a, parts, info, tmp;
+    DevBuf a, parts, info, tmp, xd;
     hipError_t e = hipMalloc(&a.p, (size_t)lda * n * 8);
+    if (e == hipSuccess) e = hipMalloc(&xd.p, (size_t)T * 64 * 64 * 8);
     if (e == hipSuccess) e = hipMalloc(&parts.p, (size_t)T * 8);
     if (e == hipSuccess) e = hipMalloc(&info.p, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&tmp.p, tt * 8);
@@ -831,8 +834,8 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
     int init = INT_MAX;
     if (e == hipSuccess) e = hipMemcpy(info.p, &init, sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: %s", hipGetErrorString(e));
-    int rc = potrf_f64((double*)a.p, lda, n, (double*)parts.p, (int*)info.p, 0, nullptr);
-    if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, n, (double*)tmp.p, nullptr);
+    int rc = potrf_f64((double*)a.p, lda, n, (double*)parts.p, (int*)info.p, 0, (double*)xd.p, nullptr);
+    if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, n, (double*)tmp.p, (const double*)xd.p, nullptr);
     if (rc) return dev_status(nullptr, rc, "debug_potrf");
     std::vector<double> hp(T);
     int hinfo = 0;

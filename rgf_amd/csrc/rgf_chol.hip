@@ -17,100 +17,219 @@ namespace rgf {
 
 constexpr int NB = 64;
 
-// ------------------------------------------------------------------ potrf panel
-// grid.x = 1 + ceil((n-k0-kb)/64).  Workgroup 0 factors the kb x kb diagonal tile
-// and writes it back (zeros above the diagonal); workgroup w>0 redundantly factors
-// the tile in LDS and solves its 64 rows of the panel below: P <- P L_kk^-T.
-__global__ __launch_bounds__(256) void potrf_panel_kernel(double* A, int lda, int n, int k0, int kb,
-                                                          double* logdet_slot, int* info,
-                                                          int info_base)
+// ------------------------------------------------------------------ potrf
+// One workgroup factors the kb x kb (kb <= 64) diagonal tile at A[k0, k0] and
+// inverts it.  Each of the 256 lanes owns a 4x4 register block (bi, bj) of the
+// 16x16 block grid (lower blocks only do work), so both sweeps are 16 blocked
+// steps of register math with one or two barriers each:
+//   factor   step K: lane (K,K) factors its 4x4 block; lanes (i,K) solve
+//            L_iK = A_iK L_KK^-T; lanes (i,j), j > K, update A_ij -= L_iK L_jK^T
+//   inverse  step K: lanes (K,J) finish X_KJ = -X_KK T_KJ (X_KK = L_KK^-1);
+//            lanes (I,J), I > K >= J, accumulate T_IJ += L_IK X_KJ
+// The tile is identity-padded beyond kb.  Writes L_kk back (zeros above the
+// diagonal), X_kk = L_kk^-1 to xout (64 x 64, ld 64), the tile's sum of log L_jj
+// (fixed-order tree) and the first non-positive pivot.
+__global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, int k0, int kb, double* xout,
+                                                         double* logdet_slot, int* info, int info_base)
 {
-    __shared__ double Ld[NB][NB + 1];
-    __shared__ double P[NB][NB + 1];
+    __shared__ double L[NB][NB + 1];
+    __shared__ double X[NB][NB + 1];
+    __shared__ double red[NB];
     __shared__ int bad_col;
     const int tid = threadIdx.x;
-    const int wg = blockIdx.x;
-    const int r0 = k0 + kb + (wg - 1) * NB;
-    const int nr = wg > 0 ? min(NB, n - r0) : 0;
-
+    const int bi = tid >> 4, bj = tid & 15;
+    const bool lower = bi >= bj;
     if (tid == 0) bad_col = INT_MAX;
-    for (int e = tid; e < NB * NB; e += 256) {
-        int r = e & (NB - 1), c = e >> 6;
-        Ld[r][c] = (r < kb && c < kb && r >= c) ? A[(long long)(k0 + c) * lda + (k0 + r)] : 0.0;
-        if (wg > 0) P[r][c] = (r < nr && c < kb) ? A[(long long)(k0 + c) * lda + (r0 + r)] : 0.0;
+    double a[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = 4 * bi + u, c = 4 * bj + v;
+            double x = 0.0;
+            if (r < kb && c < kb) { if (r >= c) x = A[(long long)(k0 + c) * lda + (k0 + r)]; }
+            else if (r == c) x = 1.0;
+            a[u][v] = x;
+        }
+    __syncthreads();
+    // ---- factor
+    for (int K = 0; K < 16; K++) {
+        if (bi == K && bj == K) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double d = a[v][v];
+                const bool bad = !(d > 0.0);
+                const double sd = bad ? 1.0 : sqrt(d);
+                if (bad && 4 * K + v < kb && bad_col == INT_MAX) bad_col = 4 * K + v;
+                a[v][v] = sd;
+                const double ri = 1.0 / sd;
+#pragma unroll
+                for (int u = v + 1; u < 4; u++) a[u][v] *= ri;
+#pragma unroll
+                for (int u = v + 1; u < 4; u++)
+#pragma unroll
+                    for (int w = v + 1; w <= u; w++) a[u][w] -= a[u][v] * a[w][v];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) L[4 * K + u][4 * K + v] = (u >= v) ? a[u][v] : 0.0;
+        }
+        __syncthreads();
+        if (bj == K && bi > K) {
+            double lk[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v <= u; v++) lk[u][v] = L[4 * K + u][4 * K + v];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double ri = 1.0 / lk[v][v];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    double x = a[u][v];
+#pragma unroll
+                    for (int w = 0; w < v; w++) x -= a[u][w] * lk[v][w];
+                    a[u][v] = x * ri;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) L[4 * bi + u][4 * K + v] = a[u][v];
+        }
+        __syncthreads();
+        if (bj > K && lower) {
+            double li[4][4], lj[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int w = 0; w < 4; w++) {
+                    li[u][w] = L[4 * bi + u][4 * K + w];
+                    lj[u][w] = L[4 * bj + u][4 * K + w];
+                }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double x = a[u][v];
+#pragma unroll
+                    for (int w = 0; w < 4; w++) x -= li[u][w] * lj[v][w];
+                    a[u][v] = x;
+                }
+        }
     }
     __syncthreads();
-
-    // unblocked right-looking Cholesky of the tile
-    for (int j = 0; j < kb; j++) {
-        double d = Ld[j][j];
-        __syncthreads();
-        bool bad = !(d > 0.0);
-        double sd = bad ? 1.0 : sqrt(d);
-        if (tid < NB) {
-            if (tid == j) {
-                Ld[j][j] = sd;
-                if (bad) bad_col = min(bad_col, j);
-            } else if (tid > j && tid < kb) {
-                Ld[tid][j] /= sd;
+    // ---- inverse: diagonal blocks first (lanes (K,K)), T accumulators in registers
+    double t[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) t[u][v] = 0.0;
+    double xd[4][4];   // X_ii of this lane's block row (used by lanes with bi == K at step K)
+    {
+        double l[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) l[u][v] = (u >= v) ? L[4 * bi + u][4 * bi + v] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                double x = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+                for (int q = 0; q < r; q++) x -= l[r][q] * xd[q][c];
+                xd[r][c] = (r >= c) ? x / l[r][r] : 0.0;
             }
         }
-        __syncthreads();
-        const int w = kb - j - 1;
-        for (int e = tid; e < w * w; e += 256) {
-            int r = j + 1 + e % w, c = j + 1 + e / w;
-            if (r >= c) Ld[r][c] -= Ld[r][j] * Ld[c][j];
+    }
+    for (int K = 0; K < 16; K++) {
+        if (bi == K && bj <= K) {   // finish X_KJ
+            double xk[4][4];
+            if (bj == K) {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) xk[u][v] = xd[u][v];
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        double x = 0.0;
+#pragma unroll
+                        for (int w = 0; w <= u; w++) x -= xd[u][w] * t[w][v];
+                        xk[u][v] = x;
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) X[4 * K + u][4 * bj + v] = xk[u][v];
         }
+        __syncthreads();
+        if (bi > K && bj <= K) {    // T_IJ += L_IK X_KJ
+            double li[4][4], xk[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int w = 0; w < 4; w++) {
+                    li[u][w] = L[4 * bi + u][4 * K + w];
+                    xk[u][w] = X[4 * K + u][4 * bj + w];
+                }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double x = t[u][v];
+#pragma unroll
+                    for (int w = 0; w < 4; w++) x += li[u][w] * xk[w][v];
+                    t[u][v] = x;
+                }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e & (NB - 1), c = e >> 6;
+        xout[c * NB + r] = (r >= c) ? X[r][c] : 0.0;
+        if (r < kb && c < kb) A[(long long)(k0 + c) * lda + (k0 + r)] = (r >= c) ? L[r][c] : 0.0;
+    }
+    if (tid < NB) red[tid] = tid < kb ? log(L[tid][tid]) : 0.0;
+    __syncthreads();
+    for (int off = NB / 2; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
         __syncthreads();
     }
-
-    if (wg == 0) {
-        if (tid == 0) {
-            double s = 0.0;
-            for (int j = 0; j < kb; j++) s += log(Ld[j][j]);
-            *logdet_slot = s;
-            if (bad_col != INT_MAX) atomicMin(info, info_base + k0 + bad_col + 1);
-        }
-        for (int e = tid; e < kb * kb; e += 256) {
-            int r = e % kb, c = e / kb;
-            A[(long long)(k0 + c) * lda + (k0 + r)] = (r >= c) ? Ld[r][c] : 0.0;
-        }
-        return;
-    }
-
-    // rows below: P <- P * L^-T  (forward substitution over the tile's columns)
-    for (int c = 0; c < kb; c++) {
-        if (tid < NB) P[tid][c] /= Ld[c][c];
-        __syncthreads();
-        const int w = kb - c - 1;
-        for (int e = tid; e < NB * w; e += 256) {
-            int r = e & (NB - 1), q = c + 1 + (e >> 6);
-            P[r][q] -= P[r][c] * Ld[q][c];
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < NB * kb; e += 256) {
-        int r = e & (NB - 1), c = e >> 6;
-        if (r < nr) A[(long long)(k0 + c) * lda + (r0 + r)] = P[r][c];
+    if (tid == 0) {
+        *logdet_slot = red[0];
+        if (bad_col != INT_MAX) atomicMin(info, info_base + k0 + bad_col + 1);
     }
 }
 
-int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base,
+// Blocked right-looking Cholesky of the n x n block at A (lower): per 64-wide panel
+// the tile kernel, the rows below as an in-place product with X_kk^T (the output has
+// one column tile, so every workgroup reads only the rows it writes), and the
+// trailing lower update on MFMA.  xdiag receives the T inverted diagonal tiles.
+int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base, double* xdiag,
               hipStream_t s)
 {
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int kb = min(NB, n - k0);
         const int below = n - k0 - kb;
-        const int grid = 1 + (below + NB - 1) / NB;
-        hipLaunchKernelGGL(potrf_panel_kernel, dim3(grid), dim3(256), 0, s, A, lda, n, k0, kb,
+        double* xt = xdiag + (long long)(k0 / NB) * NB * NB;
+        hipLaunchKernelGGL(potrf_tile_kernel, dim3(1), dim3(256), 0, s, A, lda, k0, kb, xt,
                            logdet_parts + k0 / NB, info, info_base);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
         if (below > 0) {
-            const double* Pn = A + (long long)k0 * lda + (k0 + kb);
+            double* Pn = A + (long long)k0 * lda + (k0 + kb);
+            GemmDesc p = gdesc(Pn, lda, xt, NB, Pn, lda, below, kb, kb, 1.0, 0.0, G_BUPPER);
+            int rc = gemm_f64(Op::N, Op::T, &p, 1, s);
+            if (rc) return rc;
             double* Cn = A + (long long)(k0 + kb) * lda + (k0 + kb);
             GemmDesc d = gdesc(Pn, lda, Pn, lda, Cn, lda, below, below, kb, -1.0, 1.0, G_CLOWER);
-            int rc = gemm_f64(Op::N, Op::T, &d, 1, s);
+            rc = gemm_f64(Op::N, Op::T, &d, 1, s);
             if (rc) return rc;
         }
     }
@@ -120,7 +239,7 @@ int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int in
 // ------------------------------------------------------------------ trtri
 // grid.x = T*T tiles.  (ti < tj): zero the tile above the diagonal.  (ti == tj):
 // invert the diagonal tile in LDS.  (ti > tj): nothing (written by the merges).
-__global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int n, int T)
+__global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int n, int T, const double* xdiag)
 {
     __shared__ double Ls[NB][NB + 1];
     __shared__ double Xs[NB][NB + 1];
@@ -137,6 +256,14 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int
         return;
     }
     const int kb = rb;
+    if (xdiag) {   // tiles already inverted by the Cholesky tile kernel
+        const double* xt = xdiag + (long long)ti * NB * NB;
+        for (int e = tid; e < kb * kb; e += 256) {
+            int r = e % kb, c = e / kb;
+            A[(long long)(r0 + c) * lda + (r0 + r)] = xt[c * NB + r];
+        }
+        return;
+    }
     for (int e = tid; e < NB * NB; e += 256) {
         int r = e & (NB - 1), c = e >> 6;
         Ls[r][c] = (r < kb && c < kb && r >= c) ? A[(long long)(r0 + c) * lda + (r0 + r)] : 0.0;
@@ -160,10 +287,10 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int
     }
 }
 
-int trtri_f64_inplace(double* A, int lda, int n, double* tmp, hipStream_t s)
+int trtri_f64_inplace(double* A, int lda, int n, double* tmp, const double* xdiag, hipStream_t s)
 {
     const int T = (n + NB - 1) / NB;
-    hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T), dim3(256), 0, s, A, lda, n, T);
+    hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T), dim3(256), 0, s, A, lda, n, T, xdiag);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
     for (int sz = 1; sz < T; sz *= 2) {

@@ -41,16 +41,36 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs args)
     const int z = blockIdx.y;
     GemmDesc D;
     select_desc(args, z, D);
-    const int tile = blockIdx.x;
-    if (tile >= D.tiles_m * D.tiles_n) return;
-    const int ti = tile % D.tiles_m, tj = tile / D.tiles_m;
-    const int i0 = ti * BM, j0 = tj * BN;
     const int flags = D.flags;
+    // XCD-aware tile order: workgroups are dealt round-robin over the 8 XCDs, so
+    // remap the launch index to give each XCD a contiguous range of tiles (their
+    // A row-panels and B column-panels then meet in that XCD's L2).
+    int g = blockIdx.x;
+    {
+        const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
+        g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
+    }
+    int ti, tj;
+    if (D.tri_tiles) {   // lower-triangular C: enumerate only the tiles on or below the diagonal
+        if (g >= D.tiles_m * (D.tiles_m + 1) / 2) return;
+        ti = (int)((sqrt(8.0 * g + 1.0) - 1.0) * 0.5);
+        while (ti * (ti + 1) / 2 > g) ti--;
+        while ((ti + 1) * (ti + 2) / 2 <= g) ti++;
+        tj = g - ti * (ti + 1) / 2;
+    } else {             // bands of 8 tile-rows, column-major inside a band
+        if (g >= D.tiles_m * D.tiles_n) return;
+        const int band = g / (8 * D.tiles_n);
+        const int ti0 = band * 8, gsz = min(D.tiles_m - ti0, 8);
+        const int in = g - band * 8 * D.tiles_n;
+        ti = ti0 + in % gsz;
+        tj = in / gsz;
+    }
+    const int i0 = ti * BM, j0 = tj * BN;
     if ((flags & G_CLOWER) && j0 > i0 + BM - 1) return;
 
     const long long bz = (long long)(z - D.zbase);
-    const double* __restrict__ A = D.A + bz * D.sA;
-    const double* __restrict__ B = D.B + bz * D.sB;
+    const double* A = D.A + bz * D.sA;
+    const double* B = D.B + bz * D.sB;
     double* C = D.C + bz * D.sC;
     const double* Cin = D.Cin + bz * D.sCin;
     const int M = D.m, N = D.n, K = D.k;
@@ -192,9 +212,11 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
         d.tiles_m = (d.m + BM - 1) / BM;
         d.tiles_n = (d.n + BN - 1) / BN;
+        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n;
         d.zbase = z;
         z += d.count;
-        if (d.tiles_m * d.tiles_n > maxtiles) maxtiles = d.tiles_m * d.tiles_n;
+        const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
+        if (nt > maxtiles) maxtiles = nt;
         args.d[nd++] = d;
     }
     if (nd == 0) return 0;

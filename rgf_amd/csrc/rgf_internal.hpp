@@ -40,7 +40,7 @@ struct GemmDesc {
     int count;                    // batch entries described by this desc
     double alpha, beta;
     // filled by the launcher
-    int tiles_m, tiles_n, zbase, pad_;
+    int tiles_m, tiles_n, zbase, tri_tiles;
 };
 
 constexpr int GEMM_MAXD = 4;
@@ -68,9 +68,11 @@ inline GemmDesc gdesc(const double* A, int lda, const double* B, int ldb, double
 }
 
 // ---------------------------------------------------------------- dense block kernels
-int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base,
+// xdiag: T = ceil(n/64) tiles of 64 x 64 receiving the inverted diagonal tiles.
+int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base, double* xdiag,
               hipStream_t s);
-int trtri_f64_inplace(double* A, int lda, int n, double* tmp, hipStream_t s);
+// In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.
+int trtri_f64_inplace(double* A, int lda, int n, double* tmp, const double* xdiag, hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
            hipStream_t s);
 int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s);
