@@ -57,6 +57,8 @@ typedef struct {
     double sigma_a;     /* std-dev of the arrow rows (0.05) */
     double theta_d;     /* scale of the diagonal blocks (1) */
     double theta_c;     /* scale of the coupling blocks (1) */
+    int theta_mode;     /* 0: theta_d / theta_c as given; 1: per instance theta_d ~ U(0.5,2),
+                           theta_c ~ U(0.1,1) drawn from (seed, matrix id) */
 } rgf_gen_params;
 
 typedef struct {
@@ -82,6 +84,19 @@ int rgf_create_csc(const size_t *ia, const size_t *ja, const double *a, size_t n
 int rgf_create_generated(size_t ns, size_t nt, size_t nd, const rgf_gen_params *gen,
                          const rgf_options *opts, rgf_handle_t *out);
 
+/* Extension: `count` matrices of one block structure in one handle (matrix b has id
+ * gen->matrix_id + b).  factorize / selected inversion run over all of them in the
+ * same launches; the single-matrix entry points address the matrix chosen by
+ * rgf_select (default 0). */
+int rgf_create_generated_batch(size_t ns, size_t nt, size_t nd, size_t count, const rgf_gen_params *gen,
+                               const rgf_options *opts, rgf_handle_t *out);
+int rgf_batch_count(rgf_handle_t h);
+int rgf_select(rgf_handle_t h, size_t index);
+/* Regenerate the matrices of a generated handle (e.g. after an in-place factorization). */
+int rgf_regenerate(rgf_handle_t h);
+/* Device bytes a handle of this shape holds (0 for an invalid shape). */
+size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, int keep_matrix);
+
 /* RGF<T>::~RGF(), RGF.H:151-170 */
 int rgf_destroy(rgf_handle_t h);
 
@@ -101,6 +116,11 @@ int rgf_logdet(rgf_handle_t h, double *out);
  * The factor stays valid afterwards (the reference re-factorizes, RGF.H:588). */
 int rgf_selinv_diag(rgf_handle_t h, double *diag);
 
+/* Batched forms: logdet of every matrix (count values) and diag(A^-1) of every
+ * matrix (count x n, matrix b at b*n). */
+int rgf_logdet_batch(rgf_handle_t h, double *out);
+int rgf_selinv_diag_batch(rgf_handle_t h, double *diag);
+
 /* RGF<T>::residualNorm(T* x, T* b), RGF.H:631-655: ||b - A x||_2 (needs keep_matrix). */
 int rgf_residual_norm(rgf_handle_t h, const double *x, const double *b, double *out);
 
@@ -112,6 +132,7 @@ const char *rgf_last_error(rgf_handle_t h);
 
 /* ---- device-resident extensions (pointers are device memory of the handle's device) ---- */
 int rgf_solve_device(rgf_handle_t h, double *x_dev, const double *b_dev, size_t nrhs);
+/* diag(A^-1) of every matrix of the handle into device memory (count x n). */
 int rgf_selinv_diag_device(rgf_handle_t h, double *diag_dev);
 int rgf_synchronize(rgf_handle_t h);
 int rgf_get_info(rgf_handle_t h, rgf_info_t *info);

@@ -100,14 +100,39 @@ class RGF:
 
     @classmethod
     def generated(cls, ns: int, nt: int, nd: int, seed: int = 0, matrix_id: int = 0,
-                  options: Options | None = None, **gen) -> "RGF":
+                  options: Options | None = None, count: int = 1, **gen) -> "RGF":
+        """Matrix (or `count` matrices of one structure, ids matrix_id + b) generated on
+        the device; gen keys: sigma_c, sigma_a, theta_d, theta_c, theta_mode."""
         lib = _lib.load()
         g = default_gen_params(seed=seed, matrix_id=matrix_id, **gen)
         h = C.c_void_p()
-        rc = lib.rgf_create_generated(ns, nt, nd, C.byref(g), C.byref(options) if options is not None else None,
-                                      C.byref(h))
+        opt = C.byref(options) if options is not None else None
+        if count == 1:
+            rc = lib.rgf_create_generated(ns, nt, nd, C.byref(g), opt, C.byref(h))
+        else:
+            rc = lib.rgf_create_generated_batch(ns, nt, nd, count, C.byref(g), opt, C.byref(h))
         _check(rc, None)
         return cls(None, None, None, ns, nt, nd, _handle=h)
+
+    @property
+    def count(self) -> int:
+        return int(_lib.load().rgf_batch_count(self._h))
+
+    def select(self, index: int) -> None:
+        _check(_lib.load().rgf_select(self._h, index), self._h)
+
+    def regenerate(self) -> None:
+        _check(_lib.load().rgf_regenerate(self._h), self._h)
+
+    def logdet_batch(self) -> np.ndarray:
+        out = np.empty(self.count)
+        _check(_lib.load().rgf_logdet_batch(self._h, _dptr(out)), self._h)
+        return out
+
+    def selinv_diag_batch(self) -> np.ndarray:
+        out = np.empty(self.count * self.n)
+        _check(_lib.load().rgf_selinv_diag_batch(self._h, _dptr(out)), self._h)
+        return out.reshape(self.count, self.n)
 
     # ------------------------------------------------------------ reference API
     def factorize(self) -> None:                      # RGF.H:510-546
@@ -198,6 +223,11 @@ class RGF:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def footprint_bytes(ns: int, nt: int, nd: int, count: int = 1, keep_matrix: int = 1) -> int:
+    """Device bytes a handle of this shape allocates (admission control)."""
+    return int(_lib.load().rgf_footprint_bytes(ns, nt, nd, count, keep_matrix))
 
 
 def kernel_timing_enable(op_class: int) -> None:

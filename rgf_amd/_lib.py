@@ -26,6 +26,8 @@ EXPORTS = [
     "rgf_residual_norm", "rgf_residual_norm_normalized", "rgf_last_error", "rgf_solve_device",
     "rgf_selinv_diag_device", "rgf_synchronize", "rgf_get_info", "rgf_export_matrix_mf",
     "rgf_export_factor_mf", "rgf_kernel_timing_enable", "rgf_kernel_timing_read",
+    "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
+    "rgf_logdet_batch", "rgf_selinv_diag_batch",
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf"]
@@ -38,7 +40,8 @@ class Options(C.Structure):
 
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("matrix_id", C.c_uint32), ("sigma_c", C.c_double),
-                ("sigma_a", C.c_double), ("theta_d", C.c_double), ("theta_c", C.c_double)]
+                ("sigma_a", C.c_double), ("theta_d", C.c_double), ("theta_c", C.c_double),
+                ("theta_mode", C.c_int)]
 
 
 class Info(C.Structure):
@@ -90,6 +93,14 @@ def load():
         lib.rgf_export_factor_mf.argtypes = [H, dp]
         lib.rgf_kernel_timing_enable.argtypes = [C.c_int]
         lib.rgf_kernel_timing_read.argtypes = [dp, dp, C.POINTER(C.c_longlong)]
+        lib.rgf_create_generated_batch.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                                   C.POINTER(GenParams), C.POINTER(Options), C.POINTER(H)]
+        lib.rgf_batch_count.argtypes = [H]
+        lib.rgf_select.argtypes = [H, C.c_size_t]
+        lib.rgf_regenerate.argtypes = [H]
+        lib.rgf_footprint_bytes.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]
+        lib.rgf_logdet_batch.argtypes = [H, dp]
+        lib.rgf_selinv_diag_batch.argtypes = [H, dp]
         lib.rgf_debug_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int,
                                        dp, C.c_int, C.c_double, dp, C.c_int, C.c_int]
         lib.rgf_debug_gemm.restype = C.c_int
@@ -98,6 +109,7 @@ def load():
         for name in EXPORTS:
             getattr(lib, name).restype = getattr(lib, name).restype if name in (
                 "rgf_last_error", "rgf_default_options", "rgf_default_gen_params") else C.c_int
+        lib.rgf_footprint_bytes.restype = C.c_size_t
         _lib = lib
         return lib
 

@@ -44,14 +44,23 @@ using namespace rgf;
 struct rgf_handle_s {
     int ns = 0, nt = 0, nd = 0;
     long long n = 0;
+    int count = 1;              // matrices of the same block structure held by the handle
+    int cur = 0;                // matrix addressed by the single-matrix entry points
     rgf_options opt{};
     int device = 0;
     hipStream_t stream = nullptr;
     int ldp = 0;                // panel leading dimension
-    long long pstride = 0;      // elements between panels
+    long long pstride = 0;      // elements between panels of one matrix
+    long long mstride = 0;      // elements between matrices (nt panels)
     int T = 0;                  // 64-wide tiles per diagonal block
     int ldf = 0;                // ld of Ftmp / W / Z / S buffers
-    // device buffers
+    long long cstride = 1;      // nd*nd (corner stride), >= 1
+    long long fstride = 0;      // Ftmp / W / Z stride per matrix: ldf * ns
+    long long sstride = 0;      // S_below stride per matrix: ldf * (ns + nd)
+    long long tstride = 0;      // trtri scratch per matrix
+    long long xstride = 0;      // inverted diagonal tiles per matrix: T * 64 * 64
+    long long partstride = 0;   // log-diagonal parts per matrix: nt*T + 1
+    // device buffers (all x count)
     double* Ap = nullptr;       // resident matrix panels (keep_matrix)
     double* Add = nullptr;      // resident matrix corner
     double* Lp = nullptr;       // factor panels
@@ -62,13 +71,13 @@ struct rgf_handle_s {
     double* xdiag = nullptr;    // inverted 64 x 64 diagonal tiles of the current block
     double* parts = nullptr;    // per-tile log-diagonal sums: nt*T + 1
     double* cpart = nullptr;    // corner partials nt * nd * nd
-    int* dinfo = nullptr;       // first non-positive pivot (INT_MAX = none)
+    int* dinfo = nullptr;       // first non-positive pivot per matrix (INT_MAX = none)
     // selected inversion
     double* Sb[2] = {nullptr, nullptr};
     double* W = nullptr;
     double* Z = nullptr;
-    double* diag = nullptr;
-    // solve
+    double* diag = nullptr;     // count * n
+    // solve (one matrix at a time)
     double* Y = nullptr;        // n x 16
     double* X = nullptr;        // n x 16
     double* Bv = nullptr;       // n x 16
@@ -76,6 +85,8 @@ struct rgf_handle_s {
     double* scal = nullptr;     // small device scalars
     bool factorized = false;
     bool broken = false;        // in-place factorization failed: A is gone
+    bool generated = false;
+    rgf_gen_params gen{};
     size_t bytes = 0;
     std::string err;
     std::vector<void*> allocs;
@@ -117,7 +128,11 @@ int dalloc(rgf_handle_s* h, T** p, size_t count)
     return RGF_OK;
 }
 
-__global__ void set_int_kernel(int* p, int v) { *p = v; }
+__global__ void set_int_kernel(int* p, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
 int validate_shape(size_t ns, size_t nt, size_t nd)
 {
@@ -129,33 +144,57 @@ int validate_shape(size_t ns, size_t nt, size_t nd)
     return RGF_OK;
 }
 
+// bytes a handle of this shape allocates up front (admission control of the serving loop)
+size_t footprint_bytes(int ns, int nt, int nd, int count, int keep)
+{
+    const size_t ldp = (size_t)r8(2LL * ns + nd), ldf = (size_t)r8((long long)ns + nd);
+    const size_t T = (size_t)(ns + 63) / 64;
+    size_t tt = 1;
+    for (size_t sz = 1; sz < T; sz *= 2) {
+        size_t G = (T - sz + 2 * sz - 1) / (2 * sz);
+        tt = std::max(tt, G * (sz * 64) * (sz * 64));
+    }
+    const size_t cnd = (size_t)std::max(nd * nd, 1);
+    size_t per = ldp * ns * nt * (keep ? 2 : 1) + cnd * (keep ? 3 : 2) + ldf * ns + tt + T * 4096 + nt * T + 1 +
+                 (size_t)std::max(nt * nd * nd, 1);
+    per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
+    size_t once = 3 * ((size_t)ns * nt + nd) * 16 + 64 * (size_t)std::max<long long>((long long)ns * nt + nd, 2LL * ns + nd) * 16;
+    return (per * count + once) * sizeof(double);
+}
+
 int alloc_core(rgf_handle_s* h)
 {
-    const int ns = h->ns, nt = h->nt, nd = h->nd;
+    const int ns = h->ns, nt = h->nt, nd = h->nd, B = h->count;
     h->ldp = r8(2LL * ns + nd);
     h->pstride = (long long)h->ldp * ns;
+    h->mstride = h->pstride * nt;
     h->T = (ns + 63) / 64;
     h->ldf = r8((long long)ns + nd);
-    int rc;
-    if (h->opt.keep_matrix) {
-        if ((rc = dalloc(h, &h->Ap, (size_t)h->pstride * nt))) return rc;
-        if ((rc = dalloc(h, &h->Add, (size_t)std::max(nd * nd, 1)))) return rc;
-    }
-    if ((rc = dalloc(h, &h->Lp, (size_t)h->pstride * nt))) return rc;
-    if ((rc = dalloc(h, &h->Ldd, (size_t)std::max(nd * nd, 1)))) return rc;
-    if ((rc = dalloc(h, &h->Xdd, (size_t)std::max(nd * nd, 1)))) return rc;
-    if ((rc = dalloc(h, &h->Ftmp, (size_t)h->ldf * ns))) return rc;
-    // trtri merge scratch: max over levels of (groups * (s*64)^2)
+    h->cstride = std::max(nd * nd, 1);
+    h->fstride = (long long)h->ldf * ns;
+    h->sstride = (long long)h->ldf * (ns + nd);
+    h->xstride = (long long)h->T * 64 * 64;
+    h->partstride = (long long)nt * h->T + 1;
     size_t tt = 1;
     for (int sz = 1; sz < h->T; sz *= 2) {
         size_t G = (size_t)((h->T - sz + 2 * sz - 1) / (2 * sz));
         tt = std::max(tt, G * (size_t)(sz * 64) * (size_t)(sz * 64));
     }
-    if ((rc = dalloc(h, &h->ttmp, tt))) return rc;
-    if ((rc = dalloc(h, &h->xdiag, (size_t)h->T * 64 * 64))) return rc;
-    if ((rc = dalloc(h, &h->parts, (size_t)nt * h->T + 1))) return rc;
-    if ((rc = dalloc(h, &h->cpart, (size_t)std::max(nt * nd * nd, 1)))) return rc;
-    if ((rc = dalloc(h, &h->dinfo, 1))) return rc;
+    h->tstride = (long long)tt;
+    int rc;
+    if (h->opt.keep_matrix) {
+        if ((rc = dalloc(h, &h->Ap, (size_t)h->mstride * B))) return rc;
+        if ((rc = dalloc(h, &h->Add, (size_t)h->cstride * B))) return rc;
+    }
+    if ((rc = dalloc(h, &h->Lp, (size_t)h->mstride * B))) return rc;
+    if ((rc = dalloc(h, &h->Ldd, (size_t)h->cstride * B))) return rc;
+    if ((rc = dalloc(h, &h->Xdd, (size_t)h->cstride * B))) return rc;
+    if ((rc = dalloc(h, &h->Ftmp, (size_t)h->fstride * B))) return rc;
+    if ((rc = dalloc(h, &h->ttmp, (size_t)h->tstride * B))) return rc;
+    if ((rc = dalloc(h, &h->xdiag, (size_t)h->xstride * B))) return rc;
+    if ((rc = dalloc(h, &h->parts, (size_t)h->partstride * B))) return rc;
+    if ((rc = dalloc(h, &h->cpart, (size_t)std::max(nt * nd * nd, 1) * B))) return rc;
+    if ((rc = dalloc(h, &h->dinfo, (size_t)B))) return rc;
     if ((rc = dalloc(h, &h->scal, 64))) return rc;
     return RGF_OK;
 }
@@ -177,23 +216,28 @@ int ensure_selinv_buffers(rgf_handle_s* h)
 {
     if (h->W) return RGF_OK;
     int rc;
-    const int m = h->ns + h->nd;
-    if ((rc = dalloc(h, &h->Sb[0], (size_t)h->ldf * m))) return rc;
-    if ((rc = dalloc(h, &h->Sb[1], (size_t)h->ldf * m))) return rc;
-    if ((rc = dalloc(h, &h->W, (size_t)h->ldf * h->ns))) return rc;
-    if ((rc = dalloc(h, &h->Z, (size_t)h->ldf * h->ns))) return rc;
-    if ((rc = dalloc(h, &h->diag, (size_t)h->n))) return rc;
+    const int B = h->count;
+    if ((rc = dalloc(h, &h->Sb[0], (size_t)h->sstride * B))) return rc;
+    if ((rc = dalloc(h, &h->Sb[1], (size_t)h->sstride * B))) return rc;
+    if ((rc = dalloc(h, &h->W, (size_t)h->fstride * B))) return rc;
+    if ((rc = dalloc(h, &h->Z, (size_t)h->fstride * B))) return rc;
+    if ((rc = dalloc(h, &h->diag, (size_t)h->n * B))) return rc;
     return RGF_OK;
 }
 
-rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, const rgf_options* opts, int* rc)
+rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, size_t count, const rgf_options* opts, int* rc)
 {
     *rc = validate_shape(ns, nt, nd);
     if (*rc) return nullptr;
+    if (count < 1 || count > 4096) {
+        *rc = set_err(nullptr, RGF_EINVAL, "batch count %zu out of range [1, 4096]", count);
+        return nullptr;
+    }
     auto* h = new rgf_handle_s();
     rgf_default_options(&h->opt);
     if (opts) h->opt = *opts;
     h->ns = (int)ns; h->nt = (int)nt; h->nd = (int)nd;
+    h->count = (int)count;
     h->n = (long long)ns * nt + nd;
     h->device = h->opt.device;
     if (h->opt.precision != RGF_FP64) {
@@ -217,59 +261,70 @@ rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, const rgf_options* opt
     return h;
 }
 
-// pointers into the resident matrix (A) storage: Ap if kept, else the factor panels
-double* a_panel(rgf_handle_s* h, int t) { return (h->Ap ? h->Ap : h->Lp) + (long long)t * h->pstride; }
-double* a_corner(rgf_handle_s* h) { return h->Add ? h->Add : h->Ldd; }
-double* l_panel(rgf_handle_s* h, int t) { return h->Lp + (long long)t * h->pstride; }
+// pointers into matrix b: the resident matrix (Ap if kept, else the factor panels)
+double* a_panel(rgf_handle_s* h, int t, int b = 0)
+{
+    return (h->Ap ? h->Ap : h->Lp) + (long long)b * h->mstride + (long long)t * h->pstride;
+}
+double* a_corner(rgf_handle_s* h, int b = 0) { return (h->Add ? h->Add : h->Ldd) + (long long)b * h->cstride; }
+double* l_panel(rgf_handle_s* h, int t, int b = 0) { return h->Lp + (long long)b * h->mstride + (long long)t * h->pstride; }
 
 // ------------------------------------------------------------------ factorize
-// FirstStageFactor (RGF.H:233-340) with every block resident:
+// FirstStageFactor (RGF.H:233-340) with every block resident, over all matrices
+// of the handle at once (each launch covers the whole batch):
 //   S_i = A_ii - B_{i-1} B_{i-1}^T ; A_di -= D_{i-1} B_{i-1}^T      (one GEMM launch)
-//   L_ii = chol(S_i)   (log-diagonal summed in the panel kernel)
+//   L_ii = chol(S_i)   (log-diagonal summed in the tile kernel)
 //   X_i  = L_ii^-1     (in place)
 //   [B_i; D_i] = [A_{i+1,i}; A_di] X_i^T                             (one GEMM launch)
 //   S_dd = A_dd - sum_i D_i D_i^T ; L_dd = chol(S_dd) ; X_dd = L_dd^-1
 int factor_sweep(rgf_handle_s* h)
 {
-    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, ldf = h->ldf;
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, ldf = h->ldf, B = h->count;
+    const long long ms = h->mstride, fs = h->fstride;
     hipStream_t s = h->stream;
     int rc;
-    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, h->dinfo, INT_MAX);
+    hipLaunchKernelGGL(set_int_kernel, dim3((B + 255) / 256), dim3(256), 0, s, h->dinfo, B, INT_MAX);
     if ((rc = (int)hipGetLastError())) return rc;
     for (int i = 0; i < nt; i++) {
         double* P = l_panel(h, i);
         double* A = a_panel(h, i);
         if (i == 0) {
-            if (h->Ap && (rc = copy2d(A, ldp, P, ldp, ns, ns, 1.0, s))) return rc;
-            if (nd > 0 && (rc = copy2d(A + 2 * ns, ldp, h->Ftmp + ns, ldf, nd, ns, 1.0, s))) return rc;
+            if (h->Ap && (rc = copy2d(A, ldp, P, ldp, ns, ns, 1.0, s, B, ms, ms))) return rc;
+            if (nd > 0 && (rc = copy2d(A + 2 * ns, ldp, h->Ftmp + ns, ldf, nd, ns, 1.0, s, B, ms, fs))) return rc;
         } else {
             const double* Pm = l_panel(h, i - 1);
             GemmDesc d[2];
             d[0] = gdesc(Pm + ns, ldp, Pm + ns, ldp, P, ldp, ns, ns, ns, -1.0, 1.0, G_CLOWER);
             d[0].Cin = A; d[0].ldcin = ldp;
+            batch2(d[0], B, ms, ms, ms);
             int ndesc = 1;
             if (nd > 0) {
                 d[1] = gdesc(Pm + 2 * ns, ldp, Pm + ns, ldp, h->Ftmp + ns, ldf, nd, ns, ns, -1.0, 1.0, 0);
                 d[1].Cin = A + 2 * ns; d[1].ldcin = ldp;
+                batch2(d[1], B, ms, ms, fs);
+                d[1].sCin2 = ms;
                 ndesc = 2;
             }
             if ((rc = gemm_f64(Op::N, Op::T, d, ndesc, s))) return rc;
         }
-        if ((rc = potrf_f64(P, ldp, ns, h->parts + (long long)i * h->T, h->dinfo, i * ns, h->xdiag, s))) return rc;
-        if ((rc = trtri_f64_inplace(P, ldp, ns, h->ttmp, h->xdiag, s))) return rc;
+        if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
+                            h->xdiag, h->xstride, s)))
+            return rc;
+        if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s))) return rc;
         const bool has_next = i < nt - 1;
-        if (has_next && (rc = copy2d(A + ns, ldp, h->Ftmp, ldf, ns, ns, 1.0, s))) return rc;
+        if (has_next && (rc = copy2d(A + ns, ldp, h->Ftmp, ldf, ns, ns, 1.0, s, B, ms, fs))) return rc;
         const int mF = (has_next ? ns : 0) + nd;
         if (mF > 0) {
             const double* src = h->Ftmp + (has_next ? 0 : ns);
             double* dst = P + (has_next ? ns : 2 * ns);
             GemmDesc d = gdesc(src, ldf, P, ldp, dst, ldp, mF, ns, ns, 1.0, 0.0, G_BUPPER);
+            batch2(d, B, fs, ms, ms);
             if ((rc = gemm_f64(Op::N, Op::T, &d, 1, s))) return rc;
         }
     }
     if (nd > 0) {
-        rc = corner_factor(h->Lp + 2 * ns, h->pstride, ldp, ns, nt, nd, a_corner(h), nd, h->Ldd, h->Xdd, h->cpart,
-                           h->parts + (long long)nt * h->T, h->dinfo, nt * ns, s);
+        rc = corner_factor(h->Lp + 2 * ns, h->pstride, ms, ldp, ns, nt, nd, a_corner(h), nd, h->Ldd, h->Xdd, h->cpart,
+                           h->parts + (long long)nt * h->T, h->partstride, h->dinfo, nt * ns, B, s);
         if (rc) return rc;
     }
     return 0;
@@ -278,18 +333,24 @@ int factor_sweep(rgf_handle_s* h)
 int do_factorize(rgf_handle_s* h)
 {
     if (!h->Ap && h->factorized) return RGF_OK;   // in place: A is gone, the factor is current
+    if (h->broken) return set_err(h, RGF_ESTATE, "factorization failed earlier and the matrix was not kept");
     hipSetDevice(h->device);
     int rc = factor_sweep(h);
     if (rc) return dev_status(h, rc, "factorize");
-    int info = 0;
-    hipError_t e = hipMemcpyAsync(&info, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    std::vector<int> info(h->count);
+    hipError_t e = hipMemcpyAsync(info.data(), h->dinfo, sizeof(int) * h->count, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return dev_status(h, (int)e, "factorize");
-    if (info != INT_MAX) {
-        h->factorized = false;
-        if (!h->Ap) h->broken = true;
-        return set_err(h, RGF_ENOTSPD, "matrix is not positive definite: non-positive pivot at column %d", info);
-    }
+    for (int b = 0; b < h->count; b++)
+        if (info[b] != INT_MAX) {
+            h->factorized = false;
+            if (!h->Ap) h->broken = true;
+            if (h->count == 1)
+                return set_err(h, RGF_ENOTSPD, "matrix is not positive definite: non-positive pivot at column %d",
+                               info[b]);
+            return set_err(h, RGF_ENOTSPD, "matrix %d of the batch is not positive definite: non-positive pivot at column %d",
+                           b, info[b]);
+        }
     h->factorized = true;
     return RGF_OK;
 }
@@ -297,33 +358,33 @@ int do_factorize(rgf_handle_s* h)
 int ensure_factorized(rgf_handle_s* h)
 {
     if (h->factorized) return RGF_OK;
-    if (h->broken)
-        return set_err(h, RGF_ESTATE, "factorization failed earlier and the matrix was not kept");
+    if (h->broken) return set_err(h, RGF_ESTATE, "factorization failed earlier and the matrix was not kept");
     return do_factorize(h);
 }
 
 // ------------------------------------------------------------------ solve
-// SecondStageSolve (RGF.H:344-418) as products with the stored inverses:
+// SecondStageSolve (RGF.H:344-418) as products with the stored inverses, for matrix b:
 //   forward   y_i = X_i (b_i - B_{i-1} y_{i-1}),  y_d = X_dd (b_d - sum D_i y_i)
 //   backward  x_d = X_dd^T y_d,  x_i = X_i^T (y_i - B_i^T x_{i+1} - D_i^T x_d)
 // x, b: device, n x nr (ld n), nr <= 16; Y is scratch.
-int solve_block(rgf_handle_s* h, double* x, const double* b, int nr)
+int solve_block(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp;
     const int n = (int)h->n;
     hipStream_t s = h->stream;
     double* Y = h->Y;
     double* w = h->work;
+    const double* Xdd = h->Xdd + (long long)mb * h->cstride;
     int rc;
     if (x != b && (rc = copy2d(b, n, x, n, n, nr, 1.0, s))) return rc;   // x holds r during the forward pass
     double* xd = x + (long long)nt * ns;
     double* yd = Y + (long long)nt * ns;
     for (int i = 0; i < nt; i++) {
-        double* P = l_panel(h, i);
+        double* P = l_panel(h, i, mb);
         double* ri = x + (long long)i * ns;
         double* yi = Y + (long long)i * ns;
         if (i > 0) {
-            const double* Pm = l_panel(h, i - 1);
+            const double* Pm = l_panel(h, i - 1, mb);
             if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, n, 1.0, ri, n,
                                  ri, n, w, s)))
                 return rc;
@@ -334,11 +395,11 @@ int solve_block(rgf_handle_s* h, double* x, const double* b, int nr)
             return rc;
     }
     if (nd > 0) {
-        if ((rc = skinny_f64(Op::N, 1, nd, nd, nr, 1.0, h->Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s))) return rc;
-        if ((rc = skinny_f64(Op::T, 2, nd, nd, nr, 1.0, h->Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s))) return rc;
     }
     for (int i = nt - 1; i >= 0; i--) {
-        double* P = l_panel(h, i);
+        double* P = l_panel(h, i, mb);
         double* yi = Y + (long long)i * ns;
         double* xi = x + (long long)i * ns;
         if (nd > 0 &&
@@ -361,33 +422,38 @@ int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
     const long long n = h->n;
     for (size_t c0 = 0; c0 < nrhs; c0 += 16) {
         const int nr = (int)std::min<size_t>(16, nrhs - c0);
-        rc = solve_block(h, x + c0 * n, b + c0 * n, nr);
+        rc = solve_block(h, h->cur, x + c0 * n, b + c0 * n, nr);
         if (rc) return dev_status(h, rc, "solve");
     }
     return RGF_OK;
 }
 
 // ------------------------------------------------------------------ selected inversion
-// ThirdStageRGF (RGF.H:422-506) in the G-formulation with X_i = L_ii^-1 resident:
+// ThirdStageRGF (RGF.H:422-506) in the G-formulation with X_i = L_ii^-1 resident,
+// over all matrices of the handle at once:
 //   W = F_i X_i, F_i = [B_i; D_i]            Z = Sb W   (Sb = [[S_{i+1}, S_{d,i+1}^T], [S_{d,i+1}, S_dd]])
 //   S_ii = X_i^T X_i + W^T Z                 S_{d,i} = -Z[D rows]
 // exact = 0 keeps Sb block-diagonal, which is the reference's formula.
+// diag_out: count x n (matrix b at b*n).
 int selinv_sweep(rgf_handle_s* h, double* diag_out)
 {
-    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, lds = h->ldf;
+    const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, lds = h->ldf, B = h->count;
+    const long long ms = h->mstride, ss = h->sstride, fs = h->fstride, n = h->n;
     const int m = ns + nd;
     const int exact = h->opt.selinv_mode == RGF_SELINV_EXACT;
     hipStream_t s = h->stream;
     int rc;
-    if ((rc = fill2d(h->Sb[0], lds, m, m, 0.0, s))) return rc;
-    if ((rc = fill2d(h->Sb[1], lds, m, m, 0.0, s))) return rc;
+    if ((rc = fill2d(h->Sb[0], lds, m, m, 0.0, s, B, ss))) return rc;
+    if ((rc = fill2d(h->Sb[1], lds, m, m, 0.0, s, B, ss))) return rc;
     if (nd > 0) {
         for (int q = 0; q < 2; q++) {
             double* Sdd = h->Sb[q] + (long long)ns * lds + ns;
             GemmDesc d = gdesc(h->Xdd, nd, h->Xdd, nd, Sdd, lds, nd, nd, nd, 1.0, 0.0, 0);
+            batch2(d, B, h->cstride, h->cstride, ss);
             if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
         }
-        if ((rc = extract_diag(h->Sb[0] + (long long)ns * lds + ns, lds, nd, diag_out + (long long)nt * ns, s))) return rc;
+        if ((rc = extract_diag(h->Sb[0] + (long long)ns * lds + ns, lds, nd, diag_out + (long long)nt * ns, s, B, ss, n)))
+            return rc;
     }
     int cur = 0;
     for (int i = nt - 1; i >= 0; i--) {
@@ -400,19 +466,24 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out)
         double* Sn = h->Sb[cur ^ 1];
         if (mF > 0) {
             GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
+            batch2(d, B, ms, ms, fs);
             if ((rc = gemm_f64(Op::N, Op::N, &d, 1, s))) return rc;
             d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
+            batch2(d, B, ss, fs, fs);
             if ((rc = gemm_f64(Op::N, Op::N, &d, 1, s))) return rc;
         }
         {
             GemmDesc d = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
+            batch2(d, B, ms, ms, ss);
             if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
         }
         if (mF > 0) {
             GemmDesc d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 1.0, G_CLOWER);
+            batch2(d, B, fs, fs, ss);
             if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
         }
-        if ((rc = selinv_finish(Sn, lds, ns, nd, h->Z, lds, mF - nd, exact && mF > 0, diag_out + (long long)i * ns, s)))
+        if ((rc = selinv_finish(Sn, lds, ss, ns, nd, h->Z, lds, fs, mF - nd, exact && mF > 0,
+                                diag_out + (long long)i * ns, n, B, s)))
             return rc;
         cur ^= 1;
     }
@@ -445,7 +516,7 @@ __global__ __launch_bounds__(1024) void norm2_kernel(const double* r, long long 
     if (threadIdx.x == 0) *out = sqrt(red[0]);
 }
 
-int block_matvec_residual(rgf_handle_s* h, const double* x, const double* b, double* r, int nr)
+int block_matvec_residual(rgf_handle_s* h, int mb, const double* x, const double* b, double* r, int nr)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, n = (int)h->n;
     hipStream_t s = h->stream;
@@ -455,7 +526,7 @@ int block_matvec_residual(rgf_handle_s* h, const double* x, const double* b, dou
     const double* xd = x + (long long)nt * ns;
     double* rd = r + (long long)nt * ns;
     for (int t = 0; t < nt; t++) {
-        const double* A = h->Ap + (long long)t * h->pstride;
+        const double* A = h->Ap + (long long)mb * h->mstride + (long long)t * h->pstride;
         const double* xt = x + (long long)t * ns;
         double* rt = r + (long long)t * ns;
         if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s))) return rc;
@@ -469,7 +540,10 @@ int block_matvec_residual(rgf_handle_s* h, const double* x, const double* b, dou
             if ((rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s))) return rc;
         }
     }
-    if (nd > 0 && (rc = skinny_f64(Op::N, 0, nd, nd, nr, -1.0, h->Add, nd, xd, n, 1.0, rd, n, rd, n, w, s))) return rc;
+    if (nd > 0 &&
+        (rc = skinny_f64(Op::N, 0, nd, nd, nr, -1.0, h->Add + (long long)mb * h->cstride, nd, xd, n, 1.0, rd, n, rd,
+                         n, w, s)))
+        return rc;
     return 0;
 }
 
@@ -484,7 +558,7 @@ int residual_norm_host(rgf_handle_s* h, const double* x, const double* b, double
     hipError_t e = hipMemcpyAsync(h->X, x, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(h->Bv, b, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return dev_status(h, (int)e, "residual upload");
-    rc = block_matvec_residual(h, h->X, h->Bv, h->Y, 1);
+    rc = block_matvec_residual(h, h->cur, h->X, h->Bv, h->Y, 1);
     if (rc) return dev_status(h, rc, "residual");
     hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, h->Y, h->n, h->scal);
     hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, h->Bv, h->n, h->scal + 1);
@@ -497,8 +571,8 @@ int residual_norm_host(rgf_handle_s* h, const double* x, const double* b, double
 }
 
 // ------------------------------------------------------------------ MF export
-// Packs panels into the reference's MF layout (RGF.H:95, 114-138).
-int export_mf(rgf_handle_s* h, const double* panels, const double* corner, double* mf, bool lower_diag)
+// Packs the panels of one matrix into the reference's MF layout (RGF.H:95, 114-138).
+int export_mf(rgf_handle_s* h, const double* panels, const double* corner, double* mf)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp;
     std::vector<double> buf((size_t)h->pstride);
@@ -512,7 +586,7 @@ int export_mf(rgf_handle_s* h, const double* panels, const double* corner, doubl
         for (int c = 0; c < ns; c++) {
             double* col = mf + off + (size_t)c * lda;
             const double* src = buf.data() + (size_t)c * ldp;
-            for (int r = 0; r < ns; r++) col[r] = (lower_diag && r < c) ? 0.0 : src[r];
+            for (int r = 0; r < ns; r++) col[r] = (r < c) ? 0.0 : src[r];
             if (!last)
                 for (int r = 0; r < ns; r++) col[ns + r] = src[ns + r];
             for (int d = 0; d < nd; d++) col[(last ? ns : 2 * ns) + d] = src[2 * ns + d];
@@ -526,6 +600,53 @@ int export_mf(rgf_handle_s* h, const double* panels, const double* corner, doubl
         for (int c = 0; c < nd; c++)
             for (int r = 0; r < nd; r++) mf[off + (size_t)c * nd + r] = (r < c) ? 0.0 : cb[(size_t)c * nd + r];
     }
+    return RGF_OK;
+}
+
+// ------------------------------------------------------------------ generation
+int generate_all(rgf_handle_s* h)
+{
+    const int ns = h->ns, nt = h->nt, nd = h->nd;
+    double *tmp = nullptr, *rows = nullptr;
+    hipError_t e = hipMalloc(&tmp, (size_t)ns * ns * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&rows, (size_t)h->n * sizeof(double));
+    int krc = 0;
+    for (int b = 0; b < h->count && e == hipSuccess && !krc; b++) {
+        const rgf_gen_params& gp = h->gen;
+        GenParams g{gp.seed, gp.matrix_id + (uint32_t)b, ns, nt, nd, gp.sigma_c, gp.sigma_a, gp.theta_d, gp.theta_c};
+        if (gp.theta_mode == 1) {   // per-instance scales theta_d ~ U(0.5, 2), theta_c ~ U(0.1, 1)
+            g.theta_d = 0.5 + 1.5 * gen_uniform_host(gp.seed, g.matrix_id, 0);
+            g.theta_c = 0.1 + 0.9 * gen_uniform_host(gp.seed, g.matrix_id, 1);
+        }
+        krc = generate_matrix(g, a_panel(h, 0, b), h->pstride, h->ldp, a_corner(h, b), tmp, rows, h->stream);
+    }
+    if (e == hipSuccess && !krc) e = hipStreamSynchronize(h->stream);
+    hipFree(tmp);
+    hipFree(rows);
+    if (e != hipSuccess || krc) return dev_status(h, e != hipSuccess ? (int)e : krc, "generate");
+    h->factorized = false;
+    h->broken = false;
+    return RGF_OK;
+}
+
+int create_generated(size_t ns, size_t nt, size_t nd, size_t count, const rgf_gen_params* gen, const rgf_options* opts,
+                     rgf_handle_t* out)
+{
+    if (!out) return set_err(nullptr, RGF_EINVAL, "null argument");
+    *out = nullptr;
+    int rc;
+    rgf_handle_s* h = new_handle(ns, nt, nd, count, opts, &rc);
+    if (!h) return rc;
+    rgf_default_gen_params(&h->gen);
+    if (gen) h->gen = *gen;
+    h->generated = true;
+    rc = generate_all(h);
+    if (rc) {
+        g_create_error = h->err;
+        rgf_destroy(h);
+        return rc;
+    }
+    *out = h;
     return RGF_OK;
 }
 
@@ -552,6 +673,7 @@ void rgf_default_gen_params(rgf_gen_params* p)
     p->sigma_a = 0.05;
     p->theta_d = 1.0;
     p->theta_c = 1.0;
+    p->theta_mode = 0;
 }
 
 int rgf_create_csc(const size_t* ia, const size_t* ja, const double* a, size_t ns, size_t nt, size_t nd,
@@ -576,7 +698,7 @@ int rgf_create_csc(const size_t* ia, const size_t* ja, const double* a, size_t n
                                "CSC entry %zu (row %zu, column %zu) is outside the lower block-arrow pattern", k, r, c);
         }
     }
-    rgf_handle_s* h = new_handle(ns, nt, nd, opts, &rc);
+    rgf_handle_s* h = new_handle(ns, nt, nd, 1, opts, &rc);
     if (!h) return rc;
     const size_t nnz = ia[n];
     uint64_t *dia = nullptr, *dja = nullptr;
@@ -611,32 +733,27 @@ int rgf_create_csc(const size_t* ia, const size_t* ja, const double* a, size_t n
 int rgf_create_generated(size_t ns, size_t nt, size_t nd, const rgf_gen_params* gen, const rgf_options* opts,
                          rgf_handle_t* out)
 {
-    if (!out) return set_err(nullptr, RGF_EINVAL, "null argument");
-    *out = nullptr;
-    int rc;
-    rgf_handle_s* h = new_handle(ns, nt, nd, opts, &rc);
-    if (!h) return rc;
-    rgf_gen_params gp;
-    rgf_default_gen_params(&gp);
-    if (gen) gp = *gen;
-    GenParams g{gp.seed, gp.matrix_id, (int)ns, (int)nt, (int)nd, gp.sigma_c, gp.sigma_a, gp.theta_d, gp.theta_c};
-    double *tmp = nullptr, *rows = nullptr;
-    hipError_t e = hipMalloc(&tmp, ns * ns * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&rows, (ns * nt + nd) * sizeof(double));
-    int krc = 0;
-    if (e == hipSuccess) {
-        krc = generate_matrix(g, a_panel(h, 0), h->pstride, h->ldp, a_corner(h), tmp, rows, h->stream);
-        if (!krc) e = hipStreamSynchronize(h->stream);
-    }
-    hipFree(tmp); hipFree(rows);
-    if (e != hipSuccess || krc) {
-        rc = dev_status(h, e != hipSuccess ? (int)e : krc, "generate");
-        g_create_error = h->err;
-        rgf_destroy(h);
-        return rc;
-    }
-    *out = h;
-    return RGF_OK;
+    return create_generated(ns, nt, nd, 1, gen, opts, out);
+}
+
+int rgf_create_generated_batch(size_t ns, size_t nt, size_t nd, size_t count, const rgf_gen_params* gen,
+                               const rgf_options* opts, rgf_handle_t* out)
+{
+    return create_generated(ns, nt, nd, count, gen, opts, out);
+}
+
+int rgf_regenerate(rgf_handle_t h)
+{
+    if (!h) return RGF_EINVAL;
+    if (!h->generated) return set_err(h, RGF_ESTATE, "handle was not created by the generator");
+    hipSetDevice(h->device);
+    return generate_all(h);
+}
+
+size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, int keep_matrix)
+{
+    if (validate_shape(ns, nt, nd)) return 0;
+    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, keep_matrix);
 }
 
 int rgf_destroy(rgf_handle_t h)
@@ -652,6 +769,16 @@ int rgf_destroy(rgf_handle_t h)
 
 const char* rgf_last_error(rgf_handle_t h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
+int rgf_batch_count(rgf_handle_t h) { return h ? h->count : 0; }
+
+int rgf_select(rgf_handle_t h, size_t index)
+{
+    if (!h) return RGF_EINVAL;
+    if (index >= (size_t)h->count) return set_err(h, RGF_EINVAL, "index %zu >= batch count %d", index, h->count);
+    h->cur = (int)index;
+    return RGF_OK;
+}
+
 int rgf_factorize(rgf_handle_t h)
 {
     if (!h) return RGF_EINVAL;
@@ -660,19 +787,32 @@ int rgf_factorize(rgf_handle_t h)
     return do_factorize(h);
 }
 
-int rgf_logdet(rgf_handle_t h, double* out)
+int rgf_logdet_batch(rgf_handle_t h, double* out)
 {
     if (!h || !out) return RGF_EINVAL;
     int rc = ensure_factorized(h);
     if (rc) return rc;
-    const size_t np = (size_t)h->nt * h->T + (h->nd > 0 ? 1 : 0);
+    const size_t np = (size_t)h->partstride * h->count;
     std::vector<double> parts(np);
     hipError_t e = hipMemcpyAsync(parts.data(), h->parts, np * sizeof(double), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return dev_status(h, (int)e, "logdet");
-    double s = 0.0;
-    for (double v : parts) s += v;   // fixed order: block by block, tile by tile, corner last
-    *out = 2.0 * s;
+    const size_t used = (size_t)h->nt * h->T + (h->nd > 0 ? 1 : 0);
+    for (int b = 0; b < h->count; b++) {
+        double s = 0.0;
+        for (size_t k = 0; k < used; k++) s += parts[(size_t)b * h->partstride + k];   // fixed order
+        out[b] = 2.0 * s;
+    }
+    return RGF_OK;
+}
+
+int rgf_logdet(rgf_handle_t h, double* out)
+{
+    if (!h || !out) return RGF_EINVAL;
+    std::vector<double> all(h->count);
+    int rc = rgf_logdet_batch(h, all.data());
+    if (rc) return rc;
+    *out = all[h->cur];
     return RGF_OK;
 }
 
@@ -697,7 +837,7 @@ int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
         const size_t bytes = (size_t)n * nr * sizeof(double);
         hipError_t e = hipMemcpyAsync(h->Bv, b + c0 * n, bytes, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) return dev_status(h, (int)e, "solve upload");
-        rc = solve_block(h, h->X, h->Bv, nr);
+        rc = solve_block(h, h->cur, h->X, h->Bv, nr);
         if (rc) return dev_status(h, rc, "solve");
         e = hipMemcpyAsync(x + c0 * n, h->X, bytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -713,6 +853,22 @@ int rgf_selinv_diag_device(rgf_handle_t h, double* diag_dev)
     return do_selinv_device(h, diag_dev);
 }
 
+int rgf_selinv_diag_batch(rgf_handle_t h, double* diag)
+{
+    if (!h || !diag) return RGF_EINVAL;
+    hipSetDevice(h->device);
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    if ((rc = ensure_selinv_buffers(h))) return rc;
+    rc = do_selinv_device(h, h->diag);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(diag, h->diag, (size_t)h->n * h->count * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return dev_status(h, (int)e, "selinv download");
+    return RGF_OK;
+}
+
 int rgf_selinv_diag(rgf_handle_t h, double* diag)
 {
     if (!h || !diag) return RGF_EINVAL;
@@ -722,7 +878,8 @@ int rgf_selinv_diag(rgf_handle_t h, double* diag)
     if ((rc = ensure_selinv_buffers(h))) return rc;
     rc = do_selinv_device(h, h->diag);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(diag, h->diag, (size_t)h->n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    hipError_t e = hipMemcpyAsync(diag, h->diag + (long long)h->cur * h->n, (size_t)h->n * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return dev_status(h, (int)e, "selinv download");
     return RGF_OK;
@@ -766,7 +923,7 @@ int rgf_export_matrix_mf(rgf_handle_t h, double* mf)
     if (!h->Ap) return set_err(h, RGF_ESTATE, "matrix not kept (keep_matrix=0)");
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
-    return export_mf(h, h->Ap, h->Add, mf, true);
+    return export_mf(h, h->Ap + (long long)h->cur * h->mstride, h->Add + (long long)h->cur * h->cstride, mf);
 }
 
 int rgf_export_factor_mf(rgf_handle_t h, double* mf)
@@ -776,8 +933,7 @@ int rgf_export_factor_mf(rgf_handle_t h, double* mf)
     int rc = ensure_factorized(h);
     if (rc) return rc;
     hipStreamSynchronize(h->stream);
-    rc = export_mf(h, h->Lp, h->Xdd, mf, true);
-    return rc;
+    return export_mf(h, h->Lp + (long long)h->cur * h->mstride, h->Xdd + (long long)h->cur * h->cstride, mf);
 }
 
 }  // extern "C"
@@ -834,8 +990,8 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
     int init = INT_MAX;
     if (e == hipSuccess) e = hipMemcpy(info.p, &init, sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: %s", hipGetErrorString(e));
-    int rc = potrf_f64((double*)a.p, lda, n, (double*)parts.p, (int*)info.p, 0, (double*)xd.p, nullptr);
-    if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, n, (double*)tmp.p, (const double*)xd.p, nullptr);
+    int rc = potrf_f64((double*)a.p, lda, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, nullptr);
+    if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, 0, n, 1, (double*)tmp.p, 0, (const double*)xd.p, 0, nullptr);
     if (rc) return dev_status(nullptr, rc, "debug_potrf");
     std::vector<double> hp(T);
     int hinfo = 0;

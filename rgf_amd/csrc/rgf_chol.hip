@@ -29,9 +29,17 @@ constexpr int NB = 64;
 // The tile is identity-padded beyond kb.  Writes L_kk back (zeros above the
 // diagonal), X_kk = L_kk^-1 to xout (64 x 64, ld 64), the tile's sum of log L_jj
 // (fixed-order tree) and the first non-positive pivot.
-__global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, int k0, int kb, double* xout,
-                                                         double* logdet_slot, int* info, int info_base)
+__global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, long long sA, int k0, int kb,
+                                                         double* xout, long long sX, double* logdet_slot,
+                                                         long long sP, int* info, int info_base)
 {
+    {
+        const long long b = blockIdx.x;   // batch entry
+        A += b * sA;
+        xout += b * sX;
+        logdet_slot += b * sP;
+        info += b;
+    }
     __shared__ double L[NB][NB + 1];
     __shared__ double X[NB][NB + 1];
     __shared__ double red[NB];
@@ -211,24 +219,26 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, int
 // the tile kernel, the rows below as an in-place product with X_kk^T (the output has
 // one column tile, so every workgroup reads only the rows it writes), and the
 // trailing lower update on MFMA.  xdiag receives the T inverted diagonal tiles.
-int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base, double* xdiag,
-              hipStream_t s)
+int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
+              int info_base, double* xdiag, long long sX, hipStream_t s)
 {
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int kb = min(NB, n - k0);
         const int below = n - k0 - kb;
         double* xt = xdiag + (long long)(k0 / NB) * NB * NB;
-        hipLaunchKernelGGL(potrf_tile_kernel, dim3(1), dim3(256), 0, s, A, lda, k0, kb, xt,
-                           logdet_parts + k0 / NB, info, info_base);
+        hipLaunchKernelGGL(potrf_tile_kernel, dim3(count), dim3(256), 0, s, A, lda, sA, k0, kb, xt, sX,
+                           logdet_parts + k0 / NB, sP, info, info_base);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
         if (below > 0) {
             double* Pn = A + (long long)k0 * lda + (k0 + kb);
             GemmDesc p = gdesc(Pn, lda, xt, NB, Pn, lda, below, kb, kb, 1.0, 0.0, G_BUPPER);
+            batch2(p, count, sA, sX, sA);
             int rc = gemm_f64(Op::N, Op::T, &p, 1, s);
             if (rc) return rc;
             double* Cn = A + (long long)(k0 + kb) * lda + (k0 + kb);
             GemmDesc d = gdesc(Pn, lda, Pn, lda, Cn, lda, below, below, kb, -1.0, 1.0, G_CLOWER);
+            batch2(d, count, sA, sA, sA);
             rc = gemm_f64(Op::N, Op::T, &d, 1, s);
             if (rc) return rc;
         }
@@ -239,8 +249,11 @@ int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int in
 // ------------------------------------------------------------------ trtri
 // grid.x = T*T tiles.  (ti < tj): zero the tile above the diagonal.  (ti == tj):
 // invert the diagonal tile in LDS.  (ti > tj): nothing (written by the merges).
-__global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int n, int T, const double* xdiag)
+__global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, long long sA, int n, int T,
+                                                         const double* xdiag, long long sX)
 {
+    A += (long long)blockIdx.y * sA;
+    if (xdiag) xdiag += (long long)blockIdx.y * sX;
     __shared__ double Ls[NB][NB + 1];
     __shared__ double Xs[NB][NB + 1];
     const int ti = blockIdx.x % T, tj = blockIdx.x / T;
@@ -287,10 +300,11 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, int
     }
 }
 
-int trtri_f64_inplace(double* A, int lda, int n, double* tmp, const double* xdiag, hipStream_t s)
+int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
+                      const double* xdiag, long long sX, hipStream_t s)
 {
     const int T = (n + NB - 1) / NB;
-    hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T), dim3(256), 0, s, A, lda, n, T, xdiag);
+    hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, xdiag, sX);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
     for (int sz = 1; sz < T; sz *= 2) {
@@ -317,9 +331,11 @@ int trtri_f64_inplace(double* A, int lda, int n, double* tmp, const double* xdia
             // T = L21 X11   (X11 lower)
             GemmDesc a = gdesc(L21, lda, X11, lda, Tg, n1, n2, n1, n1, 1.0, 0.0, G_BLOWER);
             a.count = cnt; a.sA = gstride; a.sB = gstride; a.sC = tstride; a.sCin = tstride;
+            batch2(a, count, sA, sA, sT);
             // X21 = -X22 T  (X22 lower)
             GemmDesc b = gdesc(X22, lda, Tg, n1, L21, lda, n2, n1, n2, -1.0, 0.0, G_ALOWER);
             b.count = cnt; b.sA = gstride; b.sB = tstride; b.sC = gstride; b.sCin = gstride;
+            batch2(b, count, sA, sT, sA);
             d1[nd1] = a; d2[nd1] = b; nd1++;
         }
         int rc = gemm_f64(Op::N, Op::N, d1, nd1, s);
@@ -331,9 +347,11 @@ int trtri_f64_inplace(double* A, int lda, int n, double* tmp, const double* xdia
 }
 
 // ------------------------------------------------------------------ small kernels
-__global__ void copy2d_kernel(const double* src, int lds, double* dst, int ldd, int rows, int cols,
-                              double alpha)
+__global__ void copy2d_kernel(const double* src, int lds, long long sS, double* dst, int ldd, long long sD,
+                              int rows, int cols, double alpha)
 {
+    src += (long long)blockIdx.y * sS;
+    dst += (long long)blockIdx.y * sD;
     const long long total = (long long)rows * cols;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
@@ -343,17 +361,19 @@ __global__ void copy2d_kernel(const double* src, int lds, double* dst, int ldd, 
 }
 
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
-           hipStream_t s)
+           hipStream_t s, int count, long long sS, long long sD)
 {
-    if (rows <= 0 || cols <= 0) return 0;
+    if (rows <= 0 || cols <= 0 || count <= 0) return 0;
     long long total = (long long)rows * cols;
     int grid = (int)min((total + 255) / 256, (long long)8192);
-    hipLaunchKernelGGL(copy2d_kernel, dim3(grid), dim3(256), 0, s, src, lds, dst, ldd, rows, cols, alpha);
+    hipLaunchKernelGGL(copy2d_kernel, dim3(grid, count), dim3(256), 0, s, src, lds, sS, dst, ldd, sD, rows, cols,
+                       alpha);
     return (int)hipGetLastError();
 }
 
-__global__ void fill2d_kernel(double* dst, int ldd, int rows, int cols, double v)
+__global__ void fill2d_kernel(double* dst, int ldd, long long sD, int rows, int cols, double v)
 {
+    dst += (long long)blockIdx.y * sD;
     const long long total = (long long)rows * cols;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
@@ -362,19 +382,22 @@ __global__ void fill2d_kernel(double* dst, int ldd, int rows, int cols, double v
     }
 }
 
-int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s)
+int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s, int count, long long sD)
 {
-    if (rows <= 0 || cols <= 0) return 0;
+    if (rows <= 0 || cols <= 0 || count <= 0) return 0;
     long long total = (long long)rows * cols;
     int grid = (int)min((total + 255) / 256, (long long)8192);
-    hipLaunchKernelGGL(fill2d_kernel, dim3(grid), dim3(256), 0, s, dst, ldd, rows, cols, v);
+    hipLaunchKernelGGL(fill2d_kernel, dim3(grid, count), dim3(256), 0, s, dst, ldd, sD, rows, cols, v);
     return (int)hipGetLastError();
 }
 
 // mirror the lower triangle into the upper one through 32x32 LDS tiles; optional
 // diagonal gather.
-__global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, int n, int T, double* diag)
+__global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, long long sA, int n, int T, double* diag,
+                                                     long long sDg)
 {
+    A += (long long)blockIdx.y * sA;
+    if (diag) diag += (long long)blockIdx.y * sDg;
     __shared__ double t[32][33];
     const int bi = blockIdx.x % T, bj = blockIdx.x / T;   // destination tile (bi row, bj col), bi <= bj
     if (bi > bj) return;
@@ -392,33 +415,37 @@ __global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, int n, 
     }
 }
 
-int mirror_lower_diag(double* A, int lda, int n, double* diag, hipStream_t s)
+int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s)
 {
-    if (n <= 0) return 0;
+    if (n <= 0 || count <= 0) return 0;
     const int T = (n + 31) / 32;
-    hipLaunchKernelGGL(mirror_kernel, dim3(T * T), dim3(256), 0, s, A, lda, n, T, diag);
+    hipLaunchKernelGGL(mirror_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, diag, sDg);
     return (int)hipGetLastError();
 }
 
-int mirror_lower(double* A, int lda, int n, hipStream_t s) { return mirror_lower_diag(A, lda, n, nullptr, s); }
+int mirror_lower(double* A, int lda, int n, hipStream_t s) { return mirror_lower_diag(A, lda, 0, n, nullptr, 0, 1, s); }
 
-__global__ void extract_diag_kernel(const double* A, int lda, int n, double* out)
+__global__ void extract_diag_kernel(const double* A, int lda, long long sA, int n, double* out, long long sO)
 {
+    A += (long long)blockIdx.y * sA;
+    out += (long long)blockIdx.y * sO;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = A[(long long)i * lda + i];
 }
 
-int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s)
+int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s, int count, long long sA, long long sO)
 {
-    if (n <= 0) return 0;
-    hipLaunchKernelGGL(extract_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, lda, n, out);
+    if (n <= 0 || count <= 0) return 0;
+    hipLaunchKernelGGL(extract_diag_kernel, dim3((n + 255) / 256, count), dim3(256), 0, s, A, lda, sA, n, out, sO);
     return (int)hipGetLastError();
 }
 
 // S_{d,i} = -Z[zrow0 + d, c] into S[ns + d, c] and S[c, ns + d]
-__global__ void selinv_offdiag_kernel(double* S, int lds, int ns, int nd, const double* Z, int ldz,
-                                      int zrow0)
+__global__ void selinv_offdiag_kernel(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz,
+                                      long long sZ, int zrow0)
 {
+    S += (long long)blockIdx.y * sS;
+    Z += (long long)blockIdx.y * sZ;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ns) return;
     for (int d = 0; d < nd; d++) {
@@ -428,14 +455,14 @@ __global__ void selinv_offdiag_kernel(double* S, int lds, int ns, int nd, const 
     }
 }
 
-int selinv_finish(double* S, int lds, int ns, int nd, const double* Z, int ldz, int zrow0, int exact,
-                  double* diag_out, hipStream_t s)
+int selinv_finish(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz, long long sZ,
+                  int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s)
 {
-    int rc = mirror_lower_diag(S, lds, ns, diag_out, s);
+    int rc = mirror_lower_diag(S, lds, sS, ns, diag_out, sDg, count, s);
     if (rc) return rc;
     if (exact && nd > 0) {
-        hipLaunchKernelGGL(selinv_offdiag_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, S, lds, ns,
-                           nd, Z, ldz, zrow0);
+        hipLaunchKernelGGL(selinv_offdiag_kernel, dim3((ns + 255) / 256, count), dim3(256), 0, s, S, lds, sS, ns,
+                           nd, Z, ldz, sZ, zrow0);
         return (int)hipGetLastError();
     }
     return 0;
@@ -444,10 +471,12 @@ int selinv_finish(double* S, int lds, int ns, int nd, const double* Z, int ldz, 
 // ------------------------------------------------------------------ corner
 // partial[i] = D_i D_i^T for panel i (nd x nd, full), one workgroup per panel.
 __global__ __launch_bounds__(256) void corner_partial_kernel(const double* Dbase, long long dstride,
-                                                             int ldd, int ns, int nd, double* partial)
+                                                             long long sM, int ldd, int ns, int nd, double* partial,
+                                                             long long sPart)
 {
     const int i = blockIdx.x;
-    const double* D = Dbase + (long long)i * dstride;
+    const double* D = Dbase + (long long)blockIdx.y * sM + (long long)i * dstride;
+    partial += (long long)blockIdx.y * sPart;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int npairs = nd * nd;
     for (int pq = wave; pq < npairs; pq += 4) {
@@ -464,11 +493,20 @@ __global__ __launch_bounds__(256) void corner_partial_kernel(const double* Dbase
 }
 
 // one workgroup: S = Cin - sum_i partial[i]; Cholesky; log-det; L_dd and X_dd = L_dd^-1.
-__global__ __launch_bounds__(256) void corner_finish_kernel(const double* partial, int nparts, int nd,
-                                                            const double* Cin, int ldcin, double* Ldd,
-                                                            double* Xdd, double* logdet_slot,
-                                                            int* info, int info_base)
+__global__ __launch_bounds__(256) void corner_finish_kernel(const double* partial, long long sPart, int nparts,
+                                                            int nd, const double* Cin, int ldcin, long long sC,
+                                                            double* Ldd, double* Xdd, double* logdet_slot,
+                                                            long long sP, int* info, int info_base)
 {
+    {
+        const long long b = blockIdx.x;
+        partial += b * sPart;
+        Cin += b * sC;
+        Ldd += b * sC;
+        Xdd += b * sC;
+        logdet_slot += b * sP;
+        info += b;
+    }
     __shared__ double Ls[NB][NB + 1];
     __shared__ double Xs[NB][NB + 1];
     __shared__ int bad_col;
@@ -520,18 +558,19 @@ __global__ __launch_bounds__(256) void corner_finish_kernel(const double* partia
     }
 }
 
-int corner_factor(const double* Dbase, long long dstride, int ldd, int ns, int nt, int nd,
-                  const double* Cin, int ldcin, double* Ldd, double* Xdd, double* partial,
-                  double* logdet_slot, int* info, int info_base, hipStream_t s)
+int corner_factor(const double* Dbase, long long dstride, long long sM, int ldd, int ns, int nt, int nd,
+                  const double* Cin, int ldcin, double* Ldd, double* Xdd, double* partial, double* logdet_slot,
+                  long long sP, int* info, int info_base, int count, hipStream_t s)
 {
     if (nd <= 0) return 0;
-    if (nd > NB) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(corner_partial_kernel, dim3(nt), dim3(256), 0, s, Dbase, dstride, ldd, ns, nd,
-                       partial);
+    if (nd > NB || ldcin != nd) return (int)hipErrorInvalidValue;
+    const long long sPart = (long long)nt * nd * nd, sC = (long long)nd * nd;
+    hipLaunchKernelGGL(corner_partial_kernel, dim3(nt, count), dim3(256), 0, s, Dbase, dstride, sM, ldd, ns, nd,
+                       partial, sPart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(corner_finish_kernel, dim3(1), dim3(256), 0, s, partial, nt, nd, Cin, ldcin, Ldd,
-                       Xdd, logdet_slot, info, info_base);
+    hipLaunchKernelGGL(corner_finish_kernel, dim3(count), dim3(256), 0, s, partial, sPart, nt, nd, Cin, ldcin, sC,
+                       Ldd, Xdd, logdet_slot, sP, info, info_base);
     return (int)hipGetLastError();
 }
 

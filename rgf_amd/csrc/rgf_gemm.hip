@@ -68,11 +68,12 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs args)
     const int i0 = ti * BM, j0 = tj * BN;
     if ((flags & G_CLOWER) && j0 > i0 + BM - 1) return;
 
-    const long long bz = (long long)(z - D.zbase);
-    const double* A = D.A + bz * D.sA;
-    const double* B = D.B + bz * D.sB;
-    double* C = D.C + bz * D.sC;
-    const double* Cin = D.Cin + bz * D.sCin;
+    const int bz = z - D.zbase;
+    const long long b1 = bz % D.count, b2 = bz / D.count;
+    const double* A = D.A + b1 * D.sA + b2 * D.sA2;
+    const double* B = D.B + b1 * D.sB + b2 * D.sB2;
+    double* C = D.C + b1 * D.sC + b2 * D.sC2;
+    const double* Cin = D.Cin + b1 * D.sCin + b2 * D.sCin2;
     const int M = D.m, N = D.n, K = D.k;
     const int lda = D.lda, ldb = D.ldb;
 
@@ -192,7 +193,7 @@ double canonical_flops(const GemmDesc& d)
 {
     const int atri = d.flags & (G_ALOWER | G_AUPPER), btri = d.flags & (G_BLOWER | G_BUPPER);
     const int cl = d.flags & G_CLOWER;
-    double f = 2.0 * d.m * (double)d.n * d.k * d.count;
+    double f = 2.0 * d.m * (double)d.n * d.k * d.count * d.count2;
     if (atri && btri && cl) return f / 6.0;
     if (atri) f *= 0.5;
     if (btri) f *= 0.5;
@@ -209,12 +210,13 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
     int z = 0, maxtiles = 0, nd = 0;
     for (int q = 0; q < ndesc; q++) {
         GemmDesc d = descs[q];
+        if (d.count2 < 1) d.count2 = 1;
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
         d.tiles_m = (d.m + BM - 1) / BM;
         d.tiles_n = (d.n + BN - 1) / BN;
         d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n;
         d.zbase = z;
-        z += d.count;
+        z += d.count * d.count2;
         const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
         if (nt > maxtiles) maxtiles = nt;
         args.d[nd++] = d;

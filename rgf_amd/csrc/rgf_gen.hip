@@ -22,13 +22,14 @@
 
 namespace rgf {
 
-__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
     }
@@ -44,6 +45,14 @@ __device__ double gen_normal(uint64_t seed, uint32_t mid, uint32_t kind, uint32_
     const double r = sqrt(-2.0 * log(1.0 - u1));
     const double t = 6.283185307179586476925286766559 * u2;
     return (e & 1) ? r * sin(t) : r * cos(t);
+}
+
+// Uniform in [0,1) of stream kind 5 (per-instance scale factors), index idx.
+double gen_uniform_host(uint64_t seed, uint32_t mid, uint32_t idx)
+{
+    uint32_t c[4] = {idx, 0u, 5u << 24, mid};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * (1.0 / 9007199254740992.0);
 }
 
 // dst[i + j*ld] = scale * N(e), e = (col0 + j) * rows + i

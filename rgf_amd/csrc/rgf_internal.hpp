@@ -37,7 +37,9 @@ struct GemmDesc {
     int m, n, k;
     int lda, ldb, ldc, ldcin;
     int flags;
-    int count;                    // batch entries described by this desc
+    int count;                    // batch entries described by this desc (inner level)
+    int count2;                   // outer batch level (matrices of a batched handle), >= 1
+    long long sA2, sB2, sC2, sCin2;
     double alpha, beta;
     // filled by the launcher
     int tiles_m, tiles_n, zbase, tri_tiles;
@@ -63,25 +65,34 @@ inline GemmDesc gdesc(const double* A, int lda, const double* B, int ldb, double
     GemmDesc d{};
     d.A = A; d.B = B; d.C = C; d.Cin = C;
     d.lda = lda; d.ldb = ldb; d.ldc = ldc; d.ldcin = ldc;
-    d.m = m; d.n = n; d.k = k; d.alpha = alpha; d.beta = beta; d.flags = flags; d.count = 1;
+    d.m = m; d.n = n; d.k = k; d.alpha = alpha; d.beta = beta; d.flags = flags; d.count = 1; d.count2 = 1;
     return d;
 }
 
+// Repeat a descriptor over `count` matrices at strides (A, B, C; Cin follows C).
+inline void batch2(GemmDesc& d, int count, long long sA, long long sB, long long sC)
+{
+    d.count2 = count; d.sA2 = sA; d.sB2 = sB; d.sC2 = sC; d.sCin2 = sC;
+}
+
 // ---------------------------------------------------------------- dense block kernels
-// xdiag: T = ceil(n/64) tiles of 64 x 64 receiving the inverted diagonal tiles.
-int potrf_f64(double* A, int lda, int n, double* logdet_parts, int* info, int info_base, double* xdiag,
-              hipStream_t s);
+// Every launcher below works on `count` matrices at the given element strides
+// (s* arguments); info is one int per matrix.
+// xdiag: T = ceil(n/64) tiles of 64 x 64 per matrix receiving the inverted diagonal tiles.
+int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
+              int info_base, double* xdiag, long long sX, hipStream_t s);
 // In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.
-int trtri_f64_inplace(double* A, int lda, int n, double* tmp, const double* xdiag, hipStream_t s);
+int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
+                      const double* xdiag, long long sX, hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
-           hipStream_t s);
-int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s);
+           hipStream_t s, int count = 1, long long sS = 0, long long sD = 0);
+int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s, int count = 1, long long sD = 0);
 // Corner: Cout = Cin - sum_i D_i D_i^T (D_i: nd x ns at Dbase + i*dstride, ld ldd),
 // then in-place Cholesky of the nd x nd corner, its inverse X_dd (written to Xdd),
-// log-det part and info.  nd <= 64.
-int corner_factor(const double* Dbase, long long dstride, int ldd, int ns, int nt, int nd,
-                  const double* Cin, int ldcin, double* Ldd, double* Xdd, double* partial,
-                  double* logdet_slot, int* info, int info_base, hipStream_t s);
+// log-det part and info.  nd <= 64; Cin, Ldd, Xdd are nd x nd (ld nd) per matrix.
+int corner_factor(const double* Dbase, long long dstride, long long sM, int ldd, int ns, int nt, int nd,
+                  const double* Cin, int ldcin, double* Ldd, double* Xdd, double* partial, double* logdet_slot,
+                  long long sP, int* info, int info_base, int count, hipStream_t s);
 
 // ---------------------------------------------------------------- skinny products (solves)
 // Y = alpha * op(A) * R + beta * Y0, A is m x k (N) or k x m (T) col-major,
@@ -104,16 +115,19 @@ struct GenParams {
 // rowsum: n doubles.
 int generate_matrix(const GenParams& g, double* P, long long pstride, int ldp, double* Cdd,
                     double* tmp, double* rowsum, hipStream_t s);
+double gen_uniform_host(uint64_t seed, uint32_t mid, uint32_t idx);
 int scatter_csc(const uint64_t* ia, const uint64_t* ja, const double* a, long long nnz, int ns,
                 int nt, int nd, double* P, long long pstride, int ldp, double* Cdd, hipStream_t s);
 int mirror_lower(double* A, int lda, int n, hipStream_t s);
+int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s);
 
 // ---------------------------------------------------------------- selinv helpers
 // Finish one Takahashi step: mirror the lower triangle of S (ns x ns, ld lds) into
 // its upper triangle, write diag(S) to diag_out, and (exact mode) store
 // S_{d,i} = -Z[zrow0:zrow0+nd, :] into S[ns:ns+nd, 0:ns] and its transpose.
-int selinv_finish(double* S, int lds, int ns, int nd, const double* Z, int ldz, int zrow0,
-                  int exact, double* diag_out, hipStream_t s);
-int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s);
+int selinv_finish(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz, long long sZ,
+                  int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s);
+int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s, int count = 1, long long sA = 0,
+                 long long sO = 0);
 
 }  // namespace rgf

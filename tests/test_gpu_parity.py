@@ -307,3 +307,46 @@ def test_cli_matches_reference_driver_format(gpu, tmp_path):
     x = np.array([float(l) for l in lines[-11:]])
     xg = np.array(G["closed_11x11"]["x_rhs_formula_2"])[:11]
     assert np.abs(x - xg).max() <= 1e-12 * np.abs(xg).max()
+
+
+# ------------------------------------------------------------------ batched handles
+@pytest.mark.parametrize("ns,nt,nd,count", [(100, 6, 5, 3), (65, 3, 0, 4), (8, 5, 3, 5)])
+def test_batch_matches_single_and_oracle(gpu, ns, nt, nd, count):
+    """c3-style batch (per-instance theta): every matrix of the batch gives bitwise the
+    same logdet and diag(A^-1) as that matrix alone, and matches the oracle."""
+    say(f"batch ns={ns} nt={nt} nd={nd} count={count}")
+    seed = 99
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=seed, matrix_id=10, count=count, theta_mode=1) as hb:
+        assert hb.count == count
+        lds = hb.logdet_batch()
+        diags = hb.selinv_diag_batch()
+        mats = []
+        for b in range(count):
+            hb.select(b)
+            mats.append(hb.export_matrix_mf())
+            assert hb.logDet() == lds[b]
+    for b in range(count):
+        td = 0.5 + 1.5 * oracle.gen_uniform(seed, 10 + b, 0)
+        tc = 0.1 + 0.9 * oracle.gen_uniform(seed, 10 + b, 1)
+        H = oracle.generate(seed, ns, nt, nd, matrix_id=10 + b, theta_d=td, theta_c=tc)
+        assert np.abs(mats[b] - H).max() <= 1e-13 * np.abs(H).max()
+        L = oracle.factor(mats[b], ns, nt, nd)
+        assert abs(lds[b] - oracle.logdet(L, ns, nt, nd)) <= 1e-11 * abs(lds[b])
+        assert np.abs(diags[b] / oracle.selinv(L, ns, nt, nd, exact=True) - 1).max() <= 1e-10
+        with rgf_amd.RGF.generated(ns, nt, nd, seed=seed, matrix_id=10 + b, theta_d=td, theta_c=tc) as h1:
+            assert np.array_equal(h1.export_matrix_mf(), mats[b])
+            assert h1.logDet() == lds[b]
+            assert np.array_equal(h1.RGFdiag(), diags[b])
+
+
+def test_batch_in_place_regenerate(gpu):
+    say("batch keep_matrix=0 + regenerate")
+    opts = rgf_amd.default_options(keep_matrix=0)
+    with rgf_amd.RGF.generated(40, 4, 2, seed=5, count=3, theta_mode=1, options=opts) as h:
+        l1 = h.logdet_batch()
+        d1 = h.selinv_diag_batch()
+        h.regenerate()
+        h.factorize()
+        assert np.array_equal(h.logdet_batch(), l1)
+        assert np.array_equal(h.selinv_diag_batch(), d1)
+    assert rgf_amd.footprint_bytes(2865, 100, 6, 1, 1) > 2 * 8 * 100 * 5736 * 2865
