@@ -135,6 +135,9 @@ int rgf_solve_device(rgf_handle_t h, double *x_dev, const double *b_dev, size_t 
 /* diag(A^-1) of every matrix of the handle into device memory (count x n). */
 int rgf_selinv_diag_device(rgf_handle_t h, double *diag_dev);
 int rgf_synchronize(rgf_handle_t h);
+/* Refinement of the last mixed-precision solve: steps taken and the final
+ * max_c ||b_c - A x_c|| / ||b_c||. */
+int rgf_refine_stats(rgf_handle_t h, int *iters, double *rel_residual);
 int rgf_get_info(rgf_handle_t h, rgf_info_t *info);
 
 /* A in the reference's MF layout (RGF.H:95 doubles; lower triangles of the diagonal

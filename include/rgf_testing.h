@@ -12,7 +12,8 @@ extern "C" {
 
 /* C = alpha*op(A)*op(B) + beta*C on the device FP64 MFMA GEMM kernel.  Host
  * arrays, column-major; ta/tb: 0 = N, 1 = T; flags as the kernel's GemmFlags
- * (1 lower-C stores, 2/4 op(A) lower/upper, 8/16 op(B) lower/upper).
+ * (1 lower-C stores, 2/4 op(A) lower/upper, 8/16 op(B) lower/upper); flag 256 runs the
+ * split-BF16 kernel of the mixed-precision path instead of the FP64 one.
  * A is (ta ? k x m : m x k) with ld lda, B is (tb ? n x k : k x n) with ld ldb. */
 int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int flags);

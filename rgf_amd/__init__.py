@@ -129,6 +129,12 @@ class RGF:
         _check(_lib.load().rgf_logdet_batch(self._h, _dptr(out)), self._h)
         return out
 
+    def refine_stats(self) -> tuple[int, float]:
+        """(steps, max relative residual) of the last mixed-precision solve's refinement."""
+        it, res = C.c_int(), C.c_double()
+        _check(_lib.load().rgf_refine_stats(self._h, C.byref(it), C.byref(res)), self._h)
+        return it.value, res.value
+
     def selinv_diag_batch(self) -> np.ndarray:
         out = np.empty(self.count * self.n)
         _check(_lib.load().rgf_selinv_diag_batch(self._h, _dptr(out)), self._h)

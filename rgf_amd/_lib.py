@@ -27,7 +27,7 @@ EXPORTS = [
     "rgf_selinv_diag_device", "rgf_synchronize", "rgf_get_info", "rgf_export_matrix_mf",
     "rgf_export_factor_mf", "rgf_kernel_timing_enable", "rgf_kernel_timing_read",
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
-    "rgf_logdet_batch", "rgf_selinv_diag_batch",
+    "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats",
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf"]
@@ -100,6 +100,7 @@ def load():
         lib.rgf_regenerate.argtypes = [H]
         lib.rgf_footprint_bytes.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]
         lib.rgf_logdet_batch.argtypes = [H, dp]
+        lib.rgf_refine_stats.argtypes = [H, C.POINTER(C.c_int), dp]
         lib.rgf_selinv_diag_batch.argtypes = [H, dp]
         lib.rgf_debug_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int,
                                        dp, C.c_int, C.c_double, dp, C.c_int, C.c_int]

@@ -81,7 +81,11 @@ struct rgf_handle_s {
     double* Y = nullptr;        // n x 16
     double* X = nullptr;        // n x 16
     double* Bv = nullptr;       // n x 16
+    double* R = nullptr;        // n x 16 residual of the refinement
+    double* Dc = nullptr;       // n x 16 correction of the refinement
     double* work = nullptr;     // skinny split-k partials
+    int last_refine_iters = 0;  // refinement steps taken by the last mixed solve
+    double last_refine_res = 0.0;
     double* scal = nullptr;     // small device scalars
     bool factorized = false;
     bool broken = false;        // in-place factorization failed: A is gone
@@ -207,6 +211,8 @@ int ensure_solve_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->Y, n16))) return rc;
     if ((rc = dalloc(h, &h->X, n16))) return rc;
     if ((rc = dalloc(h, &h->Bv, n16))) return rc;
+    if ((rc = dalloc(h, &h->R, n16))) return rc;
+    if ((rc = dalloc(h, &h->Dc, n16))) return rc;
     const int mx = std::max<int>((int)h->n, 2 * h->ns + h->nd);
     if ((rc = dalloc(h, &h->work, skinny_work_elems(mx, mx, 16)))) return rc;
     return RGF_OK;
@@ -240,7 +246,7 @@ rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, size_t count, const rg
     h->count = (int)count;
     h->n = (long long)ns * nt + nd;
     h->device = h->opt.device;
-    if (h->opt.precision != RGF_FP64) {
+    if (h->opt.precision != RGF_FP64 && h->opt.precision != RGF_MIXED_BF16X3) {
         *rc = set_err(nullptr, RGF_EINVAL, "precision mode %d is not available in this build", h->opt.precision);
         delete h;
         return nullptr;
@@ -268,6 +274,14 @@ double* a_panel(rgf_handle_s* h, int t, int b = 0)
 }
 double* a_corner(rgf_handle_s* h, int b = 0) { return (h->Add ? h->Add : h->Ldd) + (long long)b * h->cstride; }
 double* l_panel(rgf_handle_s* h, int t, int b = 0) { return h->Lp + (long long)b * h->mstride + (long long)t * h->pstride; }
+
+// Products that the mixed path moves to split-BF16 MFMA (Schur and triangular-solve
+// updates, the Takahashi products); everything else stays on the FP64 kernel.
+int prod(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d, int nd)
+{
+    if (h->opt.precision == RGF_MIXED_BF16X3) return gemm_bf16x3(ta, tb, d, nd, h->stream);
+    return gemm_f64(ta, tb, d, nd, h->stream);
+}
 
 // ------------------------------------------------------------------ factorize
 // FirstStageFactor (RGF.H:233-340) with every block resident, over all matrices
@@ -305,7 +319,7 @@ int factor_sweep(rgf_handle_s* h)
                 d[1].sCin2 = ms;
                 ndesc = 2;
             }
-            if ((rc = gemm_f64(Op::N, Op::T, d, ndesc, s))) return rc;
+            if ((rc = prod(h, Op::N, Op::T, d, ndesc))) return rc;
         }
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, s)))
@@ -319,7 +333,7 @@ int factor_sweep(rgf_handle_s* h)
             double* dst = P + (has_next ? ns : 2 * ns);
             GemmDesc d = gdesc(src, ldf, P, ldp, dst, ldp, mF, ns, ns, 1.0, 0.0, G_BUPPER);
             batch2(d, B, fs, ms, ms);
-            if ((rc = gemm_f64(Op::N, Op::T, &d, 1, s))) return rc;
+            if ((rc = prod(h, Op::N, Op::T, &d, 1))) return rc;
         }
     }
     if (nd > 0) {
@@ -414,15 +428,82 @@ int solve_block(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
     return 0;
 }
 
+int block_matvec_residual(rgf_handle_s* h, int mb, const double* x, const double* b, double* r, int nr);
+
+// column 2-norms of an n x nr array (ld n), one workgroup per column, fixed-order tree
+__global__ __launch_bounds__(1024) void colnorm2_kernel(const double* r, long long n, double* out)
+{
+    __shared__ double red[1024];
+    r += (long long)blockIdx.x * n;
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 1024) s += r[i] * r[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0]);
+}
+
+__global__ void axpy_kernel(double* y, const double* x, long long total)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x)
+        y[i] += x[i];
+}
+
+// Solve with the (mixed-precision) factor, then FP64 iterative refinement against
+// the resident FP64 matrix: r = b - A x; x += M^-1 r, until max_c ||r_c|| / ||b_c||
+// <= refine_tol or refine_iters steps.  b must not alias x.
+int solve_refined(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
+{
+    const long long n = h->n;
+    hipStream_t s = h->stream;
+    int rc = solve_block(h, mb, x, b, nr);
+    if (rc) return rc;
+    h->last_refine_iters = 0;
+    h->last_refine_res = 0.0;
+    if (!h->Ap || h->opt.refine_iters <= 0) return 0;
+    hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, b, n, h->scal + 32);
+    for (int it = 0; it <= h->opt.refine_iters; it++) {
+        if ((rc = block_matvec_residual(h, mb, x, b, h->R, nr))) return rc;
+        hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, h->R, n, h->scal + 16);
+        double v[32];
+        hipError_t e = hipMemcpyAsync(v, h->scal + 16, 32 * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return (int)e;
+        double worst = 0.0;
+        for (int c = 0; c < nr; c++) worst = std::max(worst, v[16 + c] > 0 ? v[c] / v[16 + c] : v[c]);
+        h->last_refine_res = worst;
+        if (worst <= h->opt.refine_tol || it == h->opt.refine_iters) break;
+        if ((rc = solve_block(h, mb, h->Dc, h->R, nr))) return rc;
+        const long long total = n * nr;
+        hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)), dim3(256), 0, s,
+                           x, h->Dc, total);
+        h->last_refine_iters = it + 1;
+    }
+    return 0;
+}
+
 int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
 {
     int rc = ensure_factorized(h);
     if (rc) return rc;
     if ((rc = ensure_solve_buffers(h))) return rc;
     const long long n = h->n;
+    const bool refine = h->opt.precision != RGF_FP64;
     for (size_t c0 = 0; c0 < nrhs; c0 += 16) {
         const int nr = (int)std::min<size_t>(16, nrhs - c0);
-        rc = solve_block(h, h->cur, x + c0 * n, b + c0 * n, nr);
+        if (refine) {
+            const double* bb = b + c0 * n;
+            if (bb == x + c0 * n) {   // in place: keep b for the residual
+                if ((rc = copy2d(bb, (int)n, h->Bv, (int)n, (int)n, nr, 1.0, h->stream))) return dev_status(h, rc, "solve");
+                bb = h->Bv;
+            }
+            rc = solve_refined(h, h->cur, x + c0 * n, bb, nr);
+        } else {
+            rc = solve_block(h, h->cur, x + c0 * n, b + c0 * n, nr);
+        }
         if (rc) return dev_status(h, rc, "solve");
     }
     return RGF_OK;
@@ -467,20 +548,20 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out)
         if (mF > 0) {
             GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
             batch2(d, B, ms, ms, fs);
-            if ((rc = gemm_f64(Op::N, Op::N, &d, 1, s))) return rc;
+            if ((rc = prod(h, Op::N, Op::N, &d, 1))) return rc;
             d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
             batch2(d, B, ss, fs, fs);
-            if ((rc = gemm_f64(Op::N, Op::N, &d, 1, s))) return rc;
+            if ((rc = prod(h, Op::N, Op::N, &d, 1))) return rc;
         }
         {
             GemmDesc d = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
             batch2(d, B, ms, ms, ss);
-            if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
+            if ((rc = prod(h, Op::T, Op::N, &d, 1))) return rc;
         }
         if (mF > 0) {
             GemmDesc d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 1.0, G_CLOWER);
             batch2(d, B, fs, fs, ss);
-            if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
+            if ((rc = prod(h, Op::T, Op::N, &d, 1))) return rc;
         }
         if ((rc = selinv_finish(Sn, lds, ss, ns, nd, h->Z, lds, fs, mF - nd, exact && mF > 0,
                                 diag_out + (long long)i * ns, n, B, s)))
@@ -837,7 +918,8 @@ int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
         const size_t bytes = (size_t)n * nr * sizeof(double);
         hipError_t e = hipMemcpyAsync(h->Bv, b + c0 * n, bytes, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) return dev_status(h, (int)e, "solve upload");
-        rc = solve_block(h, h->cur, h->X, h->Bv, nr);
+        rc = h->opt.precision != RGF_FP64 ? solve_refined(h, h->cur, h->X, h->Bv, nr)
+                                          : solve_block(h, h->cur, h->X, h->Bv, nr);
         if (rc) return dev_status(h, rc, "solve");
         e = hipMemcpyAsync(x + c0 * n, h->X, bytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -895,6 +977,14 @@ int rgf_residual_norm_normalized(rgf_handle_t h, const double* x, const double* 
 {
     if (!h || !x || !b || !out) return RGF_EINVAL;
     return residual_norm_host(h, x, b, out, true);
+}
+
+int rgf_refine_stats(rgf_handle_t h, int* iters, double* rel_residual)
+{
+    if (!h) return RGF_EINVAL;
+    if (iters) *iters = h->last_refine_iters;
+    if (rel_residual) *rel_residual = h->last_refine_res;
+    return RGF_OK;
 }
 
 int rgf_synchronize(rgf_handle_t h)
@@ -962,8 +1052,10 @@ extern "C" int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha,
     if (e == hipSuccess && nb) e = hipMemcpy(b.p, B, nb * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c.p, C, nc * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_gemm: %s", hipGetErrorString(e));
-    GemmDesc d = gdesc((const double*)a.p, lda, (const double*)b.p, ldb, (double*)c.p, ldc, m, n, k, alpha, beta, flags);
-    int rc = gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s);
+    GemmDesc d = gdesc((const double*)a.p, lda, (const double*)b.p, ldb, (double*)c.p, ldc, m, n, k, alpha, beta,
+                       flags & 0xff);
+    int rc = (flags & 0x100) ? gemm_bf16x3(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s)
+                             : gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s);
     if (rc) return dev_status(nullptr, rc, "debug_gemm");
     e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(C, c.p, nc * 8, hipMemcpyDeviceToHost);

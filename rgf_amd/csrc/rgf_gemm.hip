@@ -225,25 +225,35 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
     args.nd = nd;
     dim3 grid(maxtiles, z), block(256);
     const int cls = (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0);
-    TimedLaunch tl{};
-    const bool timed = cls == g_timing_class;
-    if (timed) {
-        hipEventCreate(&tl.a);
-        hipEventCreate(&tl.b);
-        tl.flops = 0.0;
-        for (int q = 0; q < nd; q++) tl.flops += canonical_flops(args.d[q]);
-        hipEventRecord(tl.a, s);
-    }
+    void* tok = timing_begin(cls, args.d, nd, s);
     if (ta == Op::N && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, args);
     else if (ta == Op::N && tb == Op::T) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, args);
     else if (ta == Op::T && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, args);
     else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, args);
     hipError_t err = hipGetLastError();
-    if (timed) {
-        hipEventRecord(tl.b, s);
-        g_timed.push_back(tl);
-    }
+    timing_end(tok, s);
     return (int)err;
+}
+
+void* timing_begin(int op_class, const GemmDesc* d, int nd, hipStream_t s)
+{
+    if (op_class != g_timing_class) return nullptr;
+    auto* tl = new TimedLaunch{};
+    hipEventCreate(&tl->a);
+    hipEventCreate(&tl->b);
+    tl->flops = 0.0;
+    for (int q = 0; q < nd; q++) tl->flops += canonical_flops(d[q]);
+    hipEventRecord(tl->a, s);
+    return tl;
+}
+
+void timing_end(void* token, hipStream_t s)
+{
+    if (!token) return;
+    auto* tl = (TimedLaunch*)token;
+    hipEventRecord(tl->b, s);
+    g_timed.push_back(*tl);
+    delete tl;
 }
 
 }  // namespace rgf

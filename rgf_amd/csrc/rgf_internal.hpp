@@ -58,6 +58,15 @@ enum class Op { N, T };
 // transposes (all entries are independent).  Returns hipError_t as int.
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
 
+// Same interface on split-BF16 MFMA (three bf16 products per FP64 product, FP32
+// accumulation inside a call, FP64 alpha/beta epilogue).
+int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
+
+// Kernel-timing hooks shared by the GEMM launchers (op class: 0..3 FP64 NN/NT/TN/TT,
+// 4..7 the same transposes on split-BF16).
+void* timing_begin(int op_class, const GemmDesc* d, int nd, hipStream_t s);
+void timing_end(void* token, hipStream_t s);
+
 // Convenience for a single, non-batched product.
 inline GemmDesc gdesc(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int m,
                       int n, int k, double alpha, double beta, int flags = 0)

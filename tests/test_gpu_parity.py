@@ -350,3 +350,71 @@ def test_batch_in_place_regenerate(gpu):
         assert np.array_equal(h.logdet_batch(), l1)
         assert np.array_equal(h.selinv_diag_batch(), d1)
     assert rgf_amd.footprint_bytes(2865, 100, 6, 1, 1) > 2 * 8 * 100 * 5736 * 2865
+
+
+# ------------------------------------------------------------------ mixed precision (split-BF16 products)
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(16, 16, 32), (130, 70, 33), (64, 200, 300), (300, 260, 129)])
+def test_gemm_bf16x3_kernel(gpu, ta, tb, m, n, k):
+    """Split-BF16 product error is bounded by ~2^-16 of sum_k |a_ik||b_kj| (hi*hi + hi*lo +
+    lo*hi kept, FP32 accumulation); the layout of the output is checked exactly by the
+    same bound (a misplaced element would be off by O(1))."""
+    say(f"gemm bf16x3 ta={ta} tb={tb} m={m} n={n} k={k}")
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    lda, ldb, ldc = (k if ta else m) + 1, (n if tb else k) + 2, m + 3
+    A = np.asfortranarray(rng.standard_normal((lda, m if ta else k)))
+    B = np.asfortranarray(rng.standard_normal((ldb, k if tb else n)))
+    C = np.asfortranarray(rng.standard_normal((ldc, n)))
+    opA = (A[:k, :m].T if ta else A[:m, :k])
+    opB = (B[:n, :k].T if tb else B[:k, :n])
+    ref = C.copy()
+    ref[:m, :n] = 1.5 * opA @ opB - 0.5 * C[:m, :n]
+    bound = 1.5 * np.abs(opA) @ np.abs(opB)
+    Cf = C.reshape(-1, order="F").copy()
+    rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.5, dptr(A.reshape(-1, order="F").copy()), lda,
+                                    dptr(B.reshape(-1, order="F").copy()), ldb, -0.5, dptr(Cf), ldc, 256)
+    assert rc == 0
+    got = Cf.reshape(ldc, n, order="F")
+    err = np.abs(got[:m, :n] - ref[:m, :n])
+    assert (err <= 4e-5 * bound + 1e-12).all(), (err / (bound + 1e-300)).max()
+    assert np.array_equal(got[m:, :], C[m:, :])
+
+
+@pytest.mark.parametrize("ns,nt,nd", [(200, 6, 6), (65, 4, 5), (300, 3, 0)])
+def test_mixed_precision_vs_fp64(gpu, ns, nt, nd):
+    """Mixed path (split-BF16 Schur / solve / Takahashi products, FP64 potrf, logdet and
+    refinement): logdet within 1e-8 relative of FP64, refined solves at FP64 accuracy,
+    diag(A^-1) within 1e-4 relative (stated, not refined)."""
+    say(f"mixed ns={ns} nt={nt} nd={nd}")
+    opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3, refine_tol=1e-14)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=31, options=opts) as s:
+        A_mf = s.export_matrix_mf()
+        ld = s.logDet()
+        n = s.n
+        b = np.random.default_rng(2).standard_normal((n, 16))
+        x = s.solve(None, b, 16)
+        its, res = s.refine_stats()
+        d = s.RGFdiag()
+    L = oracle.factor(A_mf, ns, nt, nd)
+    ld0 = oracle.logdet(L, ns, nt, nd)
+    assert abs(ld - ld0) <= 1e-8 * abs(ld0)
+    x0 = oracle.solve(L, ns, nt, nd, b)
+    assert np.abs(x - x0).max() <= 1e-12 * np.abs(x0).max()
+    assert res <= 1e-14 and its >= 1
+    d0 = oracle.selinv(L, ns, nt, nd, exact=True)
+    assert np.abs(d / d0 - 1).max() <= 1e-4
+    say(f"  logdet rel {abs(ld - ld0) / abs(ld0):.2e}  diag rel {np.abs(d / d0 - 1).max():.2e}  refine {its} steps")
+
+
+def test_mixed_batch_matches_single(gpu):
+    say("mixed batch vs single")
+    opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3)
+    with rgf_amd.RGF.generated(150, 5, 4, seed=8, count=3, theta_mode=1, options=opts) as hb:
+        lds = hb.logdet_batch()
+        diags = hb.selinv_diag_batch()
+    for b in range(3):
+        td = 0.5 + 1.5 * oracle.gen_uniform(8, b, 0)
+        tc = 0.1 + 0.9 * oracle.gen_uniform(8, b, 1)
+        with rgf_amd.RGF.generated(150, 5, 4, seed=8, matrix_id=b, theta_d=td, theta_c=tc, options=opts) as h1:
+            assert h1.logDet() == lds[b]
+            assert np.array_equal(h1.RGFdiag(), diags[b])
