@@ -41,6 +41,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-blocks", type=int, default=16, help="blocks of the CPU baseline sample")
     p.add_argument("--no-kernel-timing", action="store_true")
+    p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--batch", type=int, default=4, help="matrices per batched launch (0/1: skip)")
+    p.add_argument("--only-batched", choices=["fp64", "mixed"], default=None,
+                   help="run only the batched measurement of one precision (for profiling)")
     return p.parse_args()
 
 
@@ -70,6 +74,31 @@ def cpu_baseline(ns, nt, nd, seed, blocks):
     }
 
 
+def time_batch(rgf_amd, ns, nt, nd, count, precision, seed, steps, warmup):
+    """Batched handle of `count` c2-shaped matrices (in place, regenerated before every
+    step outside the timed region); returns (s per step, logdets, diag rows)."""
+    import numpy as np
+    opts = rgf_amd.default_options(precision=precision, keep_matrix=0)
+    h = rgf_amd.RGF.generated(ns, nt, nd, seed=seed, count=count, options=opts)
+    diag = rgf_amd.DeviceBuffer(h.n * count)
+    times = []
+    for it in range(warmup + steps):
+        if it > 0:
+            h.regenerate()
+        h.synchronize()
+        t0 = time.perf_counter()
+        h.factorize()
+        lds = h.logdet_batch()
+        h.selinv_diag_device(diag)
+        h.synchronize()
+        if it >= warmup:
+            times.append(time.perf_counter() - t0)
+    d = diag.download().reshape(count, h.n)
+    diag.free()
+    h.close()
+    return float(np.median(times)), lds, d
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -83,6 +112,12 @@ def main():
 
     ns, nt, nd = WORKLOADS[args.workload]
     F, S = rgf_amd.canonical_flops(ns, nt, nd)
+    if args.only_batched:
+        prec = rgf_amd.RGF_FP64 if args.only_batched == "fp64" else rgf_amd.RGF_MIXED_BF16X3
+        tb, lds, _ = time_batch(rgf_amd, ns, nt, nd, args.batch, prec, args.seed, args.steps, args.warmup)
+        print(json.dumps({"only_batched": args.only_batched, "count": args.batch, "s_per_matrix": tb / args.batch,
+                          "TFLOPs_canonical": args.batch * (F + S) / tb / 1e12}), flush=True)
+        return
     t_gen = time.perf_counter()
     s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed)
     s.synchronize()
@@ -113,17 +148,71 @@ def main():
     t = time.perf_counter(); s.factorize(); s.synchronize(); ph["factorize_ms"] = (time.perf_counter() - t) * 1e3
     t = time.perf_counter(); s.logDet(); ph["logdet_ms"] = (time.perf_counter() - t) * 1e3
     t = time.perf_counter(); s.selinv_diag_device(diag); s.synchronize(); ph["selinv_ms"] = (time.perf_counter() - t) * 1e3
-    b = rgf_amd.DeviceBuffer(s.n)
-    x = rgf_amd.DeviceBuffer(s.n)
-    b.upload(np.random.default_rng(1).standard_normal(s.n))
+    b = rgf_amd.DeviceBuffer(s.n * 16)
+    x = rgf_amd.DeviceBuffer(s.n * 16)
+    b.upload(np.random.default_rng(1).standard_normal(s.n * 16))
     s.solve_device(x, b, 1)
     s.synchronize()
     t = time.perf_counter(); s.solve_device(x, b, 1); s.synchronize(); ph["solve_1rhs_ms"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter(); s.solve_device(x, b, 16); s.synchronize(); ph["solve_16rhs_ms"] = (time.perf_counter() - t) * 1e3
     info = s.info()
     ph["solve_1rhs_GBps"] = info.bytes_solve_per_rhs / (ph["solve_1rhs_ms"] * 1e-3) / 1e9
     ph["factorize_TFLOPs"] = F / (ph["factorize_ms"] * 1e-3) / 1e12
     ph["selinv_TFLOPs"] = S / (ph["selinv_ms"] * 1e-3) / 1e12
     ph = {k: round(v, 4) for k, v in ph.items()}
+    x64 = x.download()
+    fp64_diag = diag.download()
+    s.close()
+    diag.free()
+
+    # mixed precision, same matrix: split-BF16 products, FP64 potrf / logdet / refinement
+    mixed = None
+    if not args.no_mixed:
+        opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3, refine_tol=1e-13)
+        m = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=opts)
+        md = rgf_amd.DeviceBuffer(m.n)
+        for _ in range(args.warmup):
+            m.factorize(); m.logDet(); m.selinv_diag_device(md)
+        m.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            m.factorize(); ldm = m.logDet(); m.selinv_diag_device(md)
+        m.synchronize()
+        dtm = (time.perf_counter() - t0) / args.steps
+        xm = rgf_amd.DeviceBuffer(m.n * 16)
+        t = time.perf_counter(); m.solve_device(xm, b, 16); m.synchronize(); t_solve16 = time.perf_counter() - t
+        its, res = m.refine_stats()
+        xmh = xm.download()
+        dmix = md.download()
+        mixed = {
+            "s_per_matrix": round(dtm, 4),
+            "TFLOPs_canonical": round((F + S) / dtm / 1e12, 4),
+            "speedup_vs_fp64": round(dt / dtm, 3),
+            "logdet_rel_err_vs_fp64": abs(ldm - ld) / abs(ld),
+            "diag_max_rel_err_vs_fp64": float(np.max(np.abs(dmix / fp64_diag - 1))),
+            "solve_16rhs_refined_ms": round(t_solve16 * 1e3, 3),
+            "refine_steps": its, "refine_rel_residual": res,
+            "x_16rhs_max_rel_err_vs_fp64": float(np.max(np.abs(xmh - x64)) / np.max(np.abs(x64))),
+        }
+        m.close(); md.free(); xm.free()
+    b.free(); x.free()
+
+    # batched: B matrices of the c2 shape per launch, FP64 and mixed
+    batched = None
+    if args.batch > 1:
+        tb64, ld64, d64 = time_batch(rgf_amd, ns, nt, nd, args.batch, rgf_amd.RGF_FP64, args.seed, 2, 1)
+        batched = {"count": args.batch, "fp64_s_per_matrix": round(tb64 / args.batch, 4),
+                   "fp64_TFLOPs": round(args.batch * (F + S) / tb64 / 1e12, 4)}
+        if not args.no_mixed:
+            tbm, ldm_b, dm_b = time_batch(rgf_amd, ns, nt, nd, args.batch, rgf_amd.RGF_MIXED_BF16X3, args.seed, 2, 1)
+            batched.update({
+                "mixed_s_per_matrix": round(tbm / args.batch, 4),
+                "mixed_TFLOPs_canonical": round(args.batch * (F + S) / tbm / 1e12, 4),
+                "mixed_speedup_vs_fp64_batched": round(tb64 / tbm, 3),
+                "mixed_speedup_vs_fp64_single": round(dt * args.batch / tbm, 3),
+                "logdet_max_rel_err": float(np.max(np.abs(ldm_b - ld64) / np.abs(ld64))),
+                "diag_max_rel_err": float(np.max(np.abs(dm_b / d64 - 1))),
+            })
 
     achieved = (kfl / (kms * 1e-3) / 1e12) if kms > 0 else None
     roofline = {
@@ -156,11 +245,12 @@ def main():
                    "canonical_flops": F + S, "precision": "fp64", "selinv": "exact",
                    "logdet": ld, "device_bytes": info.device_bytes, "generate_s": round(t_gen, 3)},
         "phases": ph,
+        "mixed_bf16x3": mixed,
+        "batched": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    s.close()
 
 
 if __name__ == "__main__":
