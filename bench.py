@@ -43,6 +43,11 @@ def parse():
     p.add_argument("--no-kernel-timing", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--batch", type=int, default=4, help="matrices per batched launch (0/1: skip)")
+    p.add_argument("--serve", action="store_true", help="c4: Poisson request stream through the serving loop")
+    p.add_argument("--requests", type=int, default=2000)
+    p.add_argument("--rate", type=float, default=20.0, help="requests per second (Poisson)")
+    p.add_argument("--slots", type=int, default=8)
+    p.add_argument("--c3", action="store_true", help="c3: 64 matrices (1500, 60, 5), per-instance theta")
     p.add_argument("--only-batched", choices=["fp64", "mixed"], default=None,
                    help="run only the batched measurement of one precision (for profiling)")
     return p.parse_args()
@@ -99,8 +104,67 @@ def time_batch(rgf_amd, ns, nt, nd, count, precision, seed, steps, warmup):
     return float(np.median(times)), lds, d
 
 
+def run_serve(args):
+    import rgf_amd
+    from rgf_amd import serve
+    out = {"metric": "serving req/s + p50/p99 latency (factorize+logdet+solve per request)",
+           "workload": f"c4: {args.requests} requests, Poisson rate {args.rate}/s, ns~U{{300..3000}}, "
+                       f"nt~U{{10..120}}, nd~U{{0..8}}, 10% follow-up solves, {args.slots} slots",
+           "data": "synthetic (device generator, per-request seed)"}
+    for name, prec in (("fp64", rgf_amd.RGF_FP64), ("mixed_bf16x3", rgf_amd.RGF_MIXED_BF16X3)):
+        reqs = serve.poisson_workload(args.requests, args.rate, seed=args.seed, followup_frac=0.1)
+        srv = serve.Server(serve.GPUExecutor(prec), slots=args.slots, budget_bytes=200 << 30)
+        stats = srv.run(reqs)
+        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in stats.items()}
+        print(f"serve {name}: {out[name]}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
+def run_c3(args):
+    import numpy as np
+    import rgf_amd
+    ns, nt, nd, count = 1500, 60, 5, 64
+    F, S = rgf_amd.canonical_flops(ns, nt, nd)
+    res = {"metric": "batched factorize+logdet+selinv s/matrix & TFLOP/s",
+           "workload": f"c3: {count} matrices ns={ns} nt={nt} nd={nd}, theta_d~U(0.5,2), theta_c~U(0.1,1)"}
+    lds = {}
+    diags = {}
+    for name, prec in (("fp64", rgf_amd.RGF_FP64), ("mixed_bf16x3", rgf_amd.RGF_MIXED_BF16X3)):
+        opts = rgf_amd.default_options(precision=prec, keep_matrix=0)
+        h = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, count=count, theta_mode=1, options=opts)
+        dg = rgf_amd.DeviceBuffer(h.n * count)
+        times = []
+        for it in range(args.warmup + args.steps):
+            if it > 0:
+                h.regenerate()
+            h.synchronize()
+            t0 = time.perf_counter()
+            h.factorize()
+            lds[name] = h.logdet_batch()
+            h.selinv_diag_device(dg)
+            h.synchronize()
+            if it >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        diags[name] = dg.download().reshape(count, h.n)
+        res[name] = {"s_per_batch": round(t, 4), "s_per_matrix": round(t / count, 5),
+                     "TFLOPs_canonical": round(count * (F + S) / t / 1e12, 3),
+                     "device_GB": round(h.info().device_bytes / 1e9, 2)}
+        dg.free()
+        h.close()
+        print(f"c3 {name}: {res[name]}", file=sys.stderr, flush=True)
+    res["mixed_speedup"] = round(res["fp64"]["s_per_batch"] / res["mixed_bf16x3"]["s_per_batch"], 3)
+    res["logdet_max_rel_err"] = float(np.max(np.abs(lds["mixed_bf16x3"] / lds["fp64"] - 1)))
+    res["diag_max_rel_err"] = float(np.max(np.abs(diags["mixed_bf16x3"] / diags["fp64"] - 1)))
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
+    if args.serve:
+        return run_serve(args)
+    if args.c3:
+        return run_c3(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not launched here.", file=sys.stderr)
