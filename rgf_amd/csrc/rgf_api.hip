@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -93,7 +95,7 @@ struct rgf_handle_s {
     rgf_gen_params gen{};
     size_t bytes = 0;
     std::string err;
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;   // cached device blocks (pointer, bytes)
 };
 
 namespace {
@@ -116,18 +118,74 @@ int dev_status(rgf_handle_s* h, int hip_rc, const char* where)
                    where, hip_rc, hipGetErrorString((hipError_t)hip_rc));
 }
 
+// Device memory comes from a per-device block cache: destroying a handle returns its
+// blocks to the cache (after its stream has drained) instead of calling hipFree, and a
+// new handle reuses a cached block of a fitting size before calling hipMalloc.  The
+// cache is bounded (kCacheCap bytes; beyond that the largest blocks are freed), and all
+// access is serialized by one mutex, so concurrent requests of the serving loop can
+// create and destroy handles without device-wide synchronization on every free.
+constexpr size_t kCacheCap = 48ull << 30;
+constexpr size_t kGranule = 2ull << 20;
+std::mutex g_cache_mu;
+std::multimap<size_t, void*> g_cache[64];
+size_t g_cached[64] = {};
+
+hipError_t cache_alloc(int dev, size_t bytes, void** out, size_t* got)
+{
+    const size_t want = (std::max<size_t>(bytes, 1) + kGranule - 1) / kGranule * kGranule;
+    if (dev >= 0 && dev < 64) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache[dev].lower_bound(want);
+        if (it != g_cache[dev].end() && it->first <= want + want / 4) {
+            *out = it->second;
+            *got = it->first;
+            g_cached[dev] -= it->first;
+            g_cache[dev].erase(it);
+            return hipSuccess;
+        }
+    }
+    *got = want;
+    hipError_t e = hipMalloc(out, want);
+    if (e == hipErrorOutOfMemory && dev >= 0 && dev < 64) {   // give the cache back and retry once
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (auto& kv : g_cache[dev]) hipFree(kv.second);
+        g_cache[dev].clear();
+        g_cached[dev] = 0;
+        e = hipMalloc(out, want);
+    }
+    return e;
+}
+
+// p must no longer be in use by any stream.
+void cache_free(int dev, void* p, size_t bytes)
+{
+    if (!p) return;
+    if (dev < 0 || dev >= 64) { hipFree(p); return; }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache[dev].emplace(bytes, p);
+    g_cached[dev] += bytes;
+    while (g_cached[dev] > kCacheCap && !g_cache[dev].empty()) {
+        auto it = std::prev(g_cache[dev].end());
+        hipFree(it->second);
+        g_cached[dev] -= it->first;
+        g_cache[dev].erase(it);
+    }
+}
+
 template <class T>
 int dalloc(rgf_handle_s* h, T** p, size_t count)
 {
     void* q = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
+    size_t got = 0;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = cache_alloc(h->device, bytes, &q, &got);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        return set_err(h, RGF_ENOMEM, "hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+        return set_err(h, RGF_ENOMEM, "device allocation of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
     }
-    h->allocs.push_back(q);
-    h->bytes += bytes;
+    h->allocs.push_back({q, got});
+    h->bytes += got;
     *p = (T*)q;
     return RGF_OK;
 }
@@ -689,8 +747,9 @@ int generate_all(rgf_handle_s* h)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd;
     double *tmp = nullptr, *rows = nullptr;
-    hipError_t e = hipMalloc(&tmp, (size_t)ns * ns * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&rows, (size_t)h->n * sizeof(double));
+    size_t gt = 0, gr = 0;
+    hipError_t e = cache_alloc(h->device, (size_t)ns * ns * sizeof(double), (void**)&tmp, &gt);
+    if (e == hipSuccess) e = cache_alloc(h->device, (size_t)h->n * sizeof(double), (void**)&rows, &gr);
     int krc = 0;
     for (int b = 0; b < h->count && e == hipSuccess && !krc; b++) {
         const rgf_gen_params& gp = h->gen;
@@ -701,9 +760,10 @@ int generate_all(rgf_handle_s* h)
         }
         krc = generate_matrix(g, a_panel(h, 0, b), h->pstride, h->ldp, a_corner(h, b), tmp, rows, h->stream);
     }
-    if (e == hipSuccess && !krc) e = hipStreamSynchronize(h->stream);
-    hipFree(tmp);
-    hipFree(rows);
+    hipError_t es = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = es;
+    cache_free(h->device, tmp, gt);
+    cache_free(h->device, rows, gr);
     if (e != hipSuccess || krc) return dev_status(h, e != hipSuccess ? (int)e : krc, "generate");
     h->factorized = false;
     h->broken = false;
@@ -784,9 +844,10 @@ int rgf_create_csc(const size_t* ia, const size_t* ja, const double* a, size_t n
     const size_t nnz = ia[n];
     uint64_t *dia = nullptr, *dja = nullptr;
     double* da = nullptr;
-    hipError_t e = hipMalloc(&dia, (n + 1) * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(&dja, std::max<size_t>(nnz, 1) * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(&da, std::max<size_t>(nnz, 1) * sizeof(double));
+    size_t g1 = 0, g2 = 0, g3 = 0;
+    hipError_t e = cache_alloc(h->device, (n + 1) * sizeof(uint64_t), (void**)&dia, &g1);
+    if (e == hipSuccess) e = cache_alloc(h->device, std::max<size_t>(nnz, 1) * sizeof(uint64_t), (void**)&dja, &g2);
+    if (e == hipSuccess) e = cache_alloc(h->device, std::max<size_t>(nnz, 1) * sizeof(double), (void**)&da, &g3);
     if (e == hipSuccess) e = hipMemcpy(dia, ia, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && nnz) e = hipMemcpy(dja, ja, nnz * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && nnz) e = hipMemcpy(da, a, nnz * sizeof(double), hipMemcpyHostToDevice);
@@ -800,7 +861,10 @@ int rgf_create_csc(const size_t* ia, const size_t* ja, const double* a, size_t n
                               a_corner(h), h->stream);
         if (!krc) e = hipStreamSynchronize(h->stream);
     }
-    hipFree(dia); hipFree(dja); hipFree(da);
+    hipStreamSynchronize(h->stream);
+    cache_free(h->device, dia, g1);
+    cache_free(h->device, dja, g2);
+    cache_free(h->device, da, g3);
     if (e != hipSuccess || krc) {
         rc = dev_status(h, e != hipSuccess ? (int)e : krc, "CSC upload");
         g_create_error = h->err;
@@ -842,7 +906,7 @@ int rgf_destroy(rgf_handle_t h)
     if (!h) return RGF_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (void* p : h->allocs) hipFree(p);
+    for (auto& a : h->allocs) cache_free(h->device, a.first, a.second);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return RGF_OK;

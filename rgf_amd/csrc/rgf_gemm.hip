@@ -33,7 +33,7 @@ __device__ __forceinline__ void select_desc(const GemmArgs& args, int z, GemmDes
 }
 
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs args)
+__global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
 {
     __shared__ double As[BK][BM + PADL];
     __shared__ double Bs[BK][BN + PADL];

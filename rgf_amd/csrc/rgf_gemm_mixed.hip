@@ -44,7 +44,7 @@ __device__ __forceinline__ void select_desc_m(const GemmArgs& args, int z, GemmD
 }
 
 template <bool TA, bool TB, int BT>
-__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs args)
+__global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3_kernel(GemmArgs args)
 {
     constexpr int WT = BT / 2;        // wave tile edge
     constexpr int NF = WT / 16;       // 16x16 MFMA blocks per wave edge
