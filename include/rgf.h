@@ -94,8 +94,9 @@ int rgf_batch_count(rgf_handle_t h);
 int rgf_select(rgf_handle_t h, size_t index);
 /* Regenerate the matrices of a generated handle (e.g. after an in-place factorization). */
 int rgf_regenerate(rgf_handle_t h);
-/* Device bytes a handle of this shape holds (0 for an invalid shape). */
-size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, int keep_matrix);
+/* Device bytes a handle of this shape and these options (NULL: defaults) holds after
+ * factorize, solve and selected inversion (0 for an invalid shape). */
+size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, const rgf_options *opts);
 
 /* RGF<T>::~RGF(), RGF.H:151-170 */
 int rgf_destroy(rgf_handle_t h);

@@ -13,7 +13,8 @@ extern "C" {
 /* C = alpha*op(A)*op(B) + beta*C on the device FP64 MFMA GEMM kernel.  Host
  * arrays, column-major; ta/tb: 0 = N, 1 = T; flags as the kernel's GemmFlags
  * (1 lower-C stores, 2/4 op(A) lower/upper, 8/16 op(B) lower/upper); flag 256 runs the
- * split-BF16 kernel of the mixed-precision path instead of the FP64 one.
+ * split-BF16 kernel of the mixed-precision path instead of the FP64 one, flag 512
+ * the split-BF16 product with operands split into bf16 hi/lo copies first.
  * A is (ta ? k x m : m x k) with ld lda, B is (tb ? n x k : k x n) with ld ldb. */
 int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int flags);
@@ -22,6 +23,12 @@ int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const doub
  * L^-1 in place when invert != 0; returns the log-determinant of the input in
  * *logdet and RGF_ENOTSPD for a non-positive pivot. */
 int rgf_debug_potrf(int n, double *A, int lda, int invert, double *logdet);
+
+/* Tuning loops: average device time (ms) of one GEMM launch of the given shape
+ * (operands allocated and filled once; flags as rgf_debug_gemm), and of the blocked
+ * Cholesky of an n x n block and of its 64 x 64 tile kernel alone. */
+int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int flags, int reps, double *ms_per_launch);
+int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
 
 #ifdef __cplusplus
 }

@@ -231,9 +231,10 @@ class RGF:
         self.close()
 
 
-def footprint_bytes(ns: int, nt: int, nd: int, count: int = 1, keep_matrix: int = 1) -> int:
-    """Device bytes a handle of this shape allocates (admission control)."""
-    return int(_lib.load().rgf_footprint_bytes(ns, nt, nd, count, keep_matrix))
+def footprint_bytes(ns: int, nt: int, nd: int, count: int = 1, options: Options | None = None) -> int:
+    """Device bytes a handle of this shape and options holds (admission control)."""
+    opt = C.byref(options) if options is not None else None
+    return int(_lib.load().rgf_footprint_bytes(ns, nt, nd, count, opt))
 
 
 def kernel_timing_enable(op_class: int) -> None:

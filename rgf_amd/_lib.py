@@ -30,7 +30,7 @@ EXPORTS = [
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats",
 ]
 # include/rgf_testing.h
-TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf"]
+TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf"]
 
 
 class Options(C.Structure):
@@ -98,7 +98,7 @@ def load():
         lib.rgf_batch_count.argtypes = [H]
         lib.rgf_select.argtypes = [H, C.c_size_t]
         lib.rgf_regenerate.argtypes = [H]
-        lib.rgf_footprint_bytes.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]
+        lib.rgf_footprint_bytes.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(Options)]
         lib.rgf_logdet_batch.argtypes = [H, dp]
         lib.rgf_refine_stats.argtypes = [H, C.POINTER(C.c_int), dp]
         lib.rgf_selinv_diag_batch.argtypes = [H, dp]
@@ -107,6 +107,10 @@ def load():
         lib.rgf_debug_gemm.restype = C.c_int
         lib.rgf_debug_potrf.argtypes = [C.c_int, dp, C.c_int, C.c_int, dp]
         lib.rgf_debug_potrf.restype = C.c_int
+        lib.rgf_debug_bench_gemm.argtypes = [C.c_int] * 7 + [dp]
+        lib.rgf_debug_bench_gemm.restype = C.c_int
+        lib.rgf_debug_bench_potrf.argtypes = [C.c_int, C.c_int, dp, dp]
+        lib.rgf_debug_bench_potrf.restype = C.c_int
         for name in EXPORTS:
             getattr(lib, name).restype = getattr(lib, name).restype if name in (
                 "rgf_last_error", "rgf_default_options", "rgf_default_gen_params") else C.c_int

@@ -89,6 +89,8 @@ struct rgf_handle_s {
     int last_refine_iters = 0;  // refinement steps taken by the last mixed solve
     double last_refine_res = 0.0;
     double* scal = nullptr;     // small device scalars
+    __bf16* split = nullptr;    // bf16 hi/lo operand copies of the split-BF16 products
+    size_t split_elems = 0;
     bool factorized = false;
     bool broken = false;        // in-place factorization failed: A is gone
     bool generated = false;
@@ -207,7 +209,14 @@ int validate_shape(size_t ns, size_t nt, size_t nd)
 }
 
 // bytes a handle of this shape allocates up front (admission control of the serving loop)
-size_t footprint_bytes(int ns, int nt, int nd, int count, int keep)
+// bf16 elements of the operand copies of the largest split-BF16 product of a sweep
+// (every product has m, n, k <= ns + nd)
+size_t split_need(int ns, int nd, int count)
+{
+    return split_scratch_elems(ns + nd, ns + nd, ns + nd, count);
+}
+
+size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision)
 {
     const size_t ldp = (size_t)r8(2LL * ns + nd), ldf = (size_t)r8((long long)ns + nd);
     const size_t T = (size_t)(ns + 63) / 64;
@@ -221,7 +230,9 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep)
                  (size_t)std::max(nt * nd * nd, 1);
     per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
     size_t once = 3 * ((size_t)ns * nt + nd) * 16 + 64 * (size_t)std::max<long long>((long long)ns * nt + nd, 2LL * ns + nd) * 16;
-    return (per * count + once) * sizeof(double);
+    size_t bytes = (per * count + once) * sizeof(double);
+    if (precision == RGF_MIXED_BF16X3) bytes += split_need(ns, nd, count) * 2;
+    return bytes;
 }
 
 int alloc_core(rgf_handle_s* h)
@@ -258,6 +269,10 @@ int alloc_core(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->cpart, (size_t)std::max(nt * nd * nd, 1) * B))) return rc;
     if ((rc = dalloc(h, &h->dinfo, (size_t)B))) return rc;
     if ((rc = dalloc(h, &h->scal, 64))) return rc;
+    if (h->opt.precision == RGF_MIXED_BF16X3) {
+        h->split_elems = split_need(ns, nd, B);
+        if ((rc = dalloc(h, &h->split, h->split_elems))) return rc;
+    }
     return RGF_OK;
 }
 
@@ -337,8 +352,19 @@ double* l_panel(rgf_handle_s* h, int t, int b = 0) { return h->Lp + (long long)b
 // updates, the Takahashi products); everything else stays on the FP64 kernel.
 int prod(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d, int nd)
 {
-    if (h->opt.precision == RGF_MIXED_BF16X3) return gemm_bf16x3(ta, tb, d, nd, h->stream);
-    return gemm_f64(ta, tb, d, nd, h->stream);
+    if (h->opt.precision != RGF_MIXED_BF16X3) return gemm_f64(ta, tb, d, nd, h->stream);
+    // products with both output edges >= 128 split their operands once into bf16
+    // copies; thin ones (the arrow rows) split while staging
+    GemmDesc rest[GEMM_MAXD];
+    int nrest = 0, rc;
+    for (int q = 0; q < nd; q++) {
+        if (d[q].m >= 128 && d[q].n >= 128) {
+            if ((rc = gemm_bf16x3_split(ta, tb, &d[q], h->split, h->split_elems, h->stream))) return rc;
+        } else {
+            rest[nrest++] = d[q];
+        }
+    }
+    return nrest ? gemm_bf16x3(ta, tb, rest, nrest, h->stream) : 0;
 }
 
 // ------------------------------------------------------------------ factorize
@@ -895,10 +921,13 @@ int rgf_regenerate(rgf_handle_t h)
     return generate_all(h);
 }
 
-size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, int keep_matrix)
+size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, const rgf_options* opts)
 {
     if (validate_shape(ns, nt, nd)) return 0;
-    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, keep_matrix);
+    rgf_options o;
+    rgf_default_options(&o);
+    if (opts) o = *opts;
+    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, o.keep_matrix, o.precision);
 }
 
 int rgf_destroy(rgf_handle_t h)
@@ -1118,8 +1147,16 @@ extern "C" int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha,
     if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_gemm: %s", hipGetErrorString(e));
     GemmDesc d = gdesc((const double*)a.p, lda, (const double*)b.p, ldb, (double*)c.p, ldc, m, n, k, alpha, beta,
                        flags & 0xff);
-    int rc = (flags & 0x100) ? gemm_bf16x3(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s)
+    DevBuf sc;
+    int rc;
+    if (flags & 0x200) {
+        const size_t sce = split_scratch_elems(m, n, k, 1);
+        if (hipMalloc(&sc.p, sce * 2) != hipSuccess) return set_err(nullptr, RGF_ENOMEM, "debug_gemm: scratch");
+        rc = gemm_bf16x3_split(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, (__bf16*)sc.p, sce, s);
+    } else {
+        rc = (flags & 0x100) ? gemm_bf16x3(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s)
                              : gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s);
+    }
     if (rc) return dev_status(nullptr, rc, "debug_gemm");
     e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(C, c.p, nc * 8, hipMemcpyDeviceToHost);
@@ -1160,5 +1197,108 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
     for (double v : hp) sum += v;
     if (logdet) *logdet = 2.0 * sum;
     if (hinfo != INT_MAX) return set_err(nullptr, RGF_ENOTSPD, "non-positive pivot at column %d", hinfo);
+    return RGF_OK;
+}
+
+// Device-timed loops over single kernels, for tuning (buffers allocated once, filled with
+// a deterministic pattern; results are discarded).
+__global__ void bench_fill_kernel(double* p, long long n, double scale)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = scale * (double)((i * 2654435761LL) % 1000) / 1000.0;
+}
+
+__global__ void diag_shift_kernel(double* A, int n, double v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(long long)i * n + i] += v;
+}
+
+extern "C" int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int flags, int reps, double* ms_per_launch)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || reps <= 0 || !ms_per_launch) return RGF_EINVAL;
+    const int lda = ta ? k : m, ldb = tb ? n : k, ldc = m;
+    const size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
+    DevBuf a, b, c;
+    hipError_t e = hipMalloc(&a.p, na * 8);
+    if (e == hipSuccess) e = hipMalloc(&b.p, nb * 8);
+    if (e == hipSuccess) e = hipMalloc(&c.p, nc * 8);
+    if (e != hipSuccess) return set_err(nullptr, RGF_ENOMEM, "bench_gemm: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(bench_fill_kernel, dim3(1024), dim3(256), 0, nullptr, (double*)a.p, (long long)na, 1.0);
+    hipLaunchKernelGGL(bench_fill_kernel, dim3(1024), dim3(256), 0, nullptr, (double*)b.p, (long long)nb, 1.0);
+    hipLaunchKernelGGL(bench_fill_kernel, dim3(1024), dim3(256), 0, nullptr, (double*)c.p, (long long)nc, 1.0);
+    GemmDesc d = gdesc((const double*)a.p, lda, (const double*)b.p, ldb, (double*)c.p, ldc, m, n, k, 1.0, 1.0, flags & 0xff);
+    DevBuf sc;
+    size_t sce = 0;
+    if (flags & 0x200) {
+        sce = split_scratch_elems(m, n, k, 1);
+        if (hipMalloc(&sc.p, sce * 2) != hipSuccess) return set_err(nullptr, RGF_ENOMEM, "bench_gemm: scratch");
+    }
+    auto launch = [&]() {
+        const Op oa = ta ? Op::T : Op::N, ob = tb ? Op::T : Op::N;
+        if (flags & 0x200) return gemm_bf16x3_split(oa, ob, &d, (__bf16*)sc.p, sce, nullptr);
+        return (flags & 0x100) ? gemm_bf16x3(oa, ob, &d, 1, nullptr) : gemm_f64(oa, ob, &d, 1, nullptr);
+    };
+    int rc = launch();
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && !rc; r++) rc = launch();
+    hipEventRecord(e1, nullptr);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return dev_status(nullptr, rc, "bench_gemm");
+    *ms_per_launch = ms / reps;
+    return RGF_OK;
+}
+
+extern "C" int rgf_debug_bench_potrf(int n, int reps, double* ms_potrf, double* ms_tile)
+{
+    if (n <= 0 || reps <= 0) return RGF_EINVAL;
+    const int T = (n + 63) / 64;
+    DevBuf a, a0, parts, info, xd;
+    const size_t na = (size_t)n * n;
+    hipError_t e = hipMalloc(&a.p, na * 8);
+    if (e == hipSuccess) e = hipMalloc(&a0.p, na * 8);
+    if (e == hipSuccess) e = hipMalloc(&parts.p, (size_t)T * 8);
+    if (e == hipSuccess) e = hipMalloc(&info.p, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&xd.p, (size_t)T * 64 * 64 * 8);
+    if (e != hipSuccess) return set_err(nullptr, RGF_ENOMEM, "bench_potrf: %s", hipGetErrorString(e));
+    // SPD test block: small symmetric noise + n on the diagonal
+    hipLaunchKernelGGL(bench_fill_kernel, dim3(1024), dim3(256), 0, nullptr, (double*)a0.p, (long long)na, 0.001);
+    int rc = mirror_lower((double*)a0.p, n, n, nullptr);
+    hipLaunchKernelGGL(diag_shift_kernel, dim3((n + 255) / 256), dim3(256), 0, nullptr, (double*)a0.p, n, (double)n);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float tot = 0.f;
+    for (int r = 0; r < reps && !rc; r++) {
+        rc = copy2d((const double*)a0.p, n, (double*)a.p, n, n, n, 1.0, nullptr);
+        hipEventRecord(e0, nullptr);
+        if (!rc) rc = potrf_f64((double*)a.p, n, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, nullptr);
+        hipEventRecord(e1, nullptr);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        tot += ms;
+    }
+    // the tile kernel alone, repeated on the first tile
+    float mst = 0.f;
+    if (!rc) {
+        hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps * 8 && !rc; r++) rc = potrf_tile_only((double*)a.p, n, (double*)xd.p, (double*)parts.p, (int*)info.p, nullptr);
+        hipEventRecord(e1, nullptr);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&mst, e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return dev_status(nullptr, rc, "bench_potrf");
+    if (ms_potrf) *ms_potrf = tot / reps;
+    if (ms_tile) *ms_tile = mst / (reps * 8);
     return RGF_OK;
 }

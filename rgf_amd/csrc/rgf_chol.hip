@@ -246,6 +246,14 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
     return 0;
 }
 
+// one launch of the tile kernel on the leading tile (tuning only)
+int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s)
+{
+    hipLaunchKernelGGL(potrf_tile_kernel, dim3(1), dim3(256), 0, s, A, lda, 0LL, 0, NB, xdiag, 0LL, logdet_slot, 0LL,
+                       info, 0);
+    return (int)hipGetLastError();
+}
+
 // ------------------------------------------------------------------ trtri
 // grid.x = T*T tiles.  (ti < tj): zero the tile above the diagonal.  (ti == tj):
 // invert the diagonal tile in LDS.  (ti > tj): nothing (written by the merges).

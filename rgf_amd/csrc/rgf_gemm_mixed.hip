@@ -268,3 +268,251 @@ int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
 }
 
 }  // namespace rgf
+
+// ======================================================================== pre-split operands
+// The kernel above converts every operand element once per output tile that reads it.
+// Here each operand is split once into bf16 hi/lo arrays in "k-contiguous" layout
+// (op(A) as [row][k], op(B)^T as [col][k], row pitch ldk = roundup(K, 64), zero
+// padded), and the product kernel streams them with 16-byte loads straight into LDS.
+namespace rgf {
+
+constexpr int SPLIT_PITCH = 64;
+
+// dst[e][r*ldk + k] = split(op(X)[r][k]) for r < R, k < ldk (0 beyond K); hi at dst,
+// lo at dst + lo_off.  op(X)[r][k] = X[r + k*ld] (kcontig = false) or X[k + r*ld].
+// grid: (ceil(ldk/32), ceil(R/32), entries); 32x32 tiles through LDS for the transpose.
+__global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld, int R, int K, bool kcontig,
+                                                         __bf16* dst, int ldk, long long lo_off, int count,
+                                                         long long sX1, long long sX2, long long sD)
+{
+    __shared__ double t[32][33];
+    const int e = blockIdx.z;
+    const long long b1 = e % count, b2 = e / count;
+    X += b1 * sX1 + b2 * sX2;
+    dst += (long long)e * sD;
+    const int k0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int q = ty; q < 32; q += 8) {
+        double v = 0.0;
+        if (kcontig) {   // row r0+q, k k0+tx contiguous in k
+            const int r = r0 + q, k = k0 + tx;
+            if (r < R && k < K) v = X[(long long)r * ld + k];
+            t[q][tx] = v;
+        } else {         // element (r0+tx, k0+q): contiguous in r
+            const int r = r0 + tx, k = k0 + q;
+            if (r < R && k < K) v = X[(long long)k * ld + r];
+            t[tx][q] = v;
+        }
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const int r = r0 + q, k = k0 + tx;
+        if (r < R && k < ldk) {
+            __bf16 h, l;
+            split_bf16(t[q][tx], h, l);
+            dst[(long long)r * ldk + k] = h;
+            dst[lo_off + (long long)r * ldk + k] = l;
+        }
+    }
+}
+
+template <int BT>
+__global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
+                                                                              const __bf16* Bs, long long sEA,
+                                                                              long long sEB, long long loA,
+                                                                              long long loB, int ldka, int ldkb)
+{
+    constexpr int WT = BT / 2;
+    constexpr int NF = WT / 16;
+    constexpr int CH = BT * MBK / 8 / 256;   // 16-byte chunks per thread per array per k-step
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[BT][LROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[BT][LROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[BT][LROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[BT][LROW];
+
+    const int z = blockIdx.y;
+    GemmDesc D;
+    select_desc_m(args, z, D);
+    const int flags = D.flags;
+    int g = blockIdx.x;
+    {
+        const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
+        g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
+    }
+    int ti, tj;
+    if (D.tri_tiles) {
+        if (g >= D.tiles_m * (D.tiles_m + 1) / 2) return;
+        ti = (int)((sqrt(8.0 * g + 1.0) - 1.0) * 0.5);
+        while (ti * (ti + 1) / 2 > g) ti--;
+        while ((ti + 1) * (ti + 2) / 2 <= g) ti++;
+        tj = g - ti * (ti + 1) / 2;
+    } else {
+        if (g >= D.tiles_m * D.tiles_n) return;
+        const int band = g / (8 * D.tiles_n);
+        const int ti0 = band * 8, gsz = min(D.tiles_m - ti0, 8);
+        const int in = g - band * 8 * D.tiles_n;
+        ti = ti0 + in % gsz;
+        tj = in / gsz;
+    }
+    const int i0 = ti * BT, j0 = tj * BT;
+    if ((flags & G_CLOWER) && j0 > i0 + BT - 1) return;
+
+    const int bz = z - D.zbase;   // entry index inside this descriptor; split copies are per entry
+    const long long b1 = bz % D.count, b2 = bz / D.count;
+    const __bf16* Ahg = As + (long long)(D.zbase + bz) * sEA;
+    const __bf16* Bhg = Bs + (long long)(D.zbase + bz) * sEB;
+    double* C = D.C + b1 * D.sC + b2 * D.sC2;
+    const double* Cin = D.Cin + b1 * D.sCin + b2 * D.sCin2;
+    const int M = D.m, N = D.n, K = D.k;
+
+    int klo = 0, khi = K;
+    if (flags & G_ALOWER) khi = min(khi, i0 + BT);
+    if (flags & G_AUPPER) klo = max(klo, i0);
+    if (flags & G_BLOWER) klo = max(klo, j0);
+    if (flags & G_BUPPER) khi = min(khi, j0 + BT);
+    klo = (klo / MBK) * MBK;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    v4f acc[NF][NF];
+#pragma unroll
+    for (int a = 0; a < NF; a++)
+#pragma unroll
+        for (int b = 0; b < NF; b++) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    // chunk c (0..BT*4-1) of a [BT][32] tile: row c/4, k offset 8*(c%4)
+    bf16x8 rah[CH], ral[CH], rbh[CH], rbl[CH];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            const int c = tid + 256 * q, r = c >> 2, kk = 8 * (c & 3);
+            const int ga = i0 + r, gb = j0 + r;
+            const long long oa = (long long)ga * ldka + k0 + kk, ob = (long long)gb * ldkb + k0 + kk;
+            const bf16x8 zero = {};
+            rah[q] = ga < M ? *(const bf16x8*)(Ahg + oa) : zero;
+            ral[q] = ga < M ? *(const bf16x8*)(Ahg + loA + oa) : zero;
+            rbh[q] = gb < N ? *(const bf16x8*)(Bhg + ob) : zero;
+            rbl[q] = gb < N ? *(const bf16x8*)(Bhg + loB + ob) : zero;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            const int c = tid + 256 * q, r = c >> 2, kk = 8 * (c & 3);
+            *(bf16x8*)&Ah[r][kk] = rah[q];
+            *(bf16x8*)&Al[r][kk] = ral[q];
+            *(bf16x8*)&Bh[r][kk] = rbh[q];
+            *(bf16x8*)&Bl[r][kk] = rbl[q];
+        }
+    };
+    // the split copies are zero beyond K up to their 64-aligned pitch, so a k-step may
+    // read past khi (the values there are zero or belong to skipped triangles: both exact)
+    if (klo < khi) load(klo);
+    for (int k0 = klo; k0 < khi; k0 += MBK) {
+        store();
+        __syncthreads();
+        if (k0 + MBK < khi) load(k0 + MBK);
+        bf16x8 ah[NF], al[NF], bh[NF], bl[NF];
+#pragma unroll
+        for (int a = 0; a < NF; a++) {
+            const int r = wm * WT + a * 16 + l16;
+            ah[a] = *(const bf16x8*)&Ah[r][8 * lq];
+            al[a] = *(const bf16x8*)&Al[r][8 * lq];
+        }
+#pragma unroll
+        for (int b = 0; b < NF; b++) {
+            const int c = wn * WT + b * 16 + l16;
+            bh[b] = *(const bf16x8*)&Bh[c][8 * lq];
+            bl[b] = *(const bf16x8*)&Bl[c][8 * lq];
+        }
+#pragma unroll
+        for (int a = 0; a < NF; a++)
+#pragma unroll
+            for (int b = 0; b < NF; b++) {
+                v4f t = acc[a][b];
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[b], ah[a], t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], al[a], t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], ah[a], t, 0, 0, 0);
+                acc[a][b] = t;
+            }
+        __syncthreads();
+    }
+
+    const double alpha = D.alpha, beta = D.beta;
+    const int ldc = D.ldc, ldcin = D.ldcin;
+#pragma unroll
+    for (int a = 0; a < NF; a++) {
+        const int gi = i0 + wm * WT + a * 16 + l16;
+#pragma unroll
+        for (int b = 0; b < NF; b++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gj = j0 + wn * WT + b * 16 + 4 * lq + r;
+                if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
+                    double v = alpha * (double)acc[a][b][r];
+                    if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
+                    C[(long long)gj * ldc + gi] = v;
+                }
+            }
+        }
+    }
+}
+
+size_t split_scratch_elems(int m, int n, int k, int entries)
+{
+    const size_t ldk = (size_t)((k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH);
+    const size_t mp = (size_t)((m + 127) / 128 * 128), np = (size_t)((n + 127) / 128 * 128);
+    return (size_t)entries * 2 * (mp + np) * ldk;
+}
+
+// One product on split-BF16 MFMA with operands split first into `scratch` (bf16
+// elements, >= split_scratch_elems of the launch).  One descriptor per call.
+int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d0, __bf16* scratch, size_t scratch_elems, hipStream_t s)
+{
+    GemmDesc d = *d0;
+    if (d.count2 < 1) d.count2 = 1;
+    if (d.m <= 0 || d.n <= 0 || d.count <= 0) return 0;
+    const int entries = d.count * d.count2;
+    const int ldk = (d.k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+    const long long mp = (d.m + 127) / 128 * 128, np = (d.n + 127) / 128 * 128;
+    const long long sEA = 2 * mp * ldk, sEB = 2 * np * ldk;
+    if ((size_t)entries * (sEA + sEB) > scratch_elems) return (int)hipErrorInvalidValue;
+    __bf16* As = scratch;
+    __bf16* Bs = scratch + (size_t)entries * sEA;
+    if (d.k > 0) {
+        dim3 ga((ldk + 31) / 32, (d.m + 31) / 32, entries), gb((ldk + 31) / 32, (d.n + 31) / 32, entries);
+        // op(A)[i][k]: k contiguous iff TA; op(B)^T[j][k] = op(B)[k][j]: k contiguous iff !TB
+        hipLaunchKernelGGL(split_bf16_kernel, ga, dim3(256), 0, s, d.A, d.lda, d.m, d.k, ta == Op::T, As, ldk,
+                           mp * ldk, d.count, d.sA, d.sA2, sEA);
+        hipLaunchKernelGGL(split_bf16_kernel, gb, dim3(256), 0, s, d.B, d.ldb, d.n, d.k, tb == Op::N, Bs, ldk,
+                           np * ldk, d.count, d.sB, d.sB2, sEB);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+    }
+    long long work128 = (long long)((d.m + 127) / 128) * ((d.n + 127) / 128) * entries;
+    const int BT = work128 >= 768 ? 128 : 64;
+    GemmArgs args{};
+    d.tiles_m = (d.m + BT - 1) / BT;
+    d.tiles_n = (d.n + BT - 1) / BT;
+    d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n;
+    d.zbase = 0;
+    const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
+    args.d[0] = d;
+    args.nd = 1;
+    dim3 grid(nt, entries);
+    void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
+    if (BT == 128)
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128>), grid, dim3(256), 0, s, args, As, Bs, sEA, sEB, mp * ldk,
+                           np * ldk, ldk, ldk);
+    else
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64>), grid, dim3(256), 0, s, args, As, Bs, sEA, sEB, mp * ldk,
+                           np * ldk, ldk, ldk);
+    hipError_t e = hipGetLastError();
+    timing_end(tok, s);
+    return (int)e;
+}
+
+}  // namespace rgf

@@ -62,6 +62,13 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
 // accumulation inside a call, FP64 alpha/beta epilogue).
 int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
 
+// Split-BF16 product with the operands split once, before the product, into bf16
+// hi/lo copies in `scratch` (k-contiguous, pitch roundup(k, 64)).  One descriptor
+// (count x count2 entries); scratch must hold split_scratch_elems(m, n, k, entries)
+// bf16 values.  Same result as gemm_bf16x3 to FP32 accumulation order.
+size_t split_scratch_elems(int m, int n, int k, int entries);
+int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d, __bf16* scratch, size_t scratch_elems, hipStream_t s);
+
 // Kernel-timing hooks shared by the GEMM launchers (op class: 0..3 FP64 NN/NT/TN/TT,
 // 4..7 the same transposes on split-BF16).
 void* timing_begin(int op_class, const GemmDesc* d, int nd, hipStream_t s);
@@ -93,6 +100,7 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
 // In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
                       const double* xdiag, long long sX, hipStream_t s);
+int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
            hipStream_t s, int count = 1, long long sS = 0, long long sD = 0);
 int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s, int count = 1, long long sD = 0);

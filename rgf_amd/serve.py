@@ -91,9 +91,10 @@ class GPUExecutor(Executor):
         self.precision = precision
         self.keep = 1 if precision != rgf_amd.RGF_FP64 else 0   # refinement needs A
         self.refine_tol = refine_tol
+        self.opts = rgf_amd.default_options(precision=precision, keep_matrix=self.keep, refine_tol=refine_tol)
 
     def footprint(self, req: Request) -> int:
-        return self.rgf.footprint_bytes(req.ns, req.nt, req.nd, 1, self.keep)
+        return self.rgf.footprint_bytes(req.ns, req.nt, req.nd, 1, self.opts)
 
     def rhs(self, req: Request) -> np.ndarray:
         n = req.ns * req.nt + req.nd
@@ -105,8 +106,7 @@ class GPUExecutor(Executor):
             h = cached
             req.reused_factor = True
         else:
-            opts = rgf.default_options(precision=self.precision, keep_matrix=self.keep, refine_tol=self.refine_tol)
-            h = rgf.RGF.generated(req.ns, req.nt, req.nd, seed=req.seed, options=opts)
+            h = rgf.RGF.generated(req.ns, req.nt, req.nd, seed=req.seed, options=self.opts)
             h.factorize()
         req.logdet = h.logDet()
         req.x = h.solve(None, self.rhs(req), 1)

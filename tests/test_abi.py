@@ -61,3 +61,19 @@ def test_canonical_flops_closed_form():
     F1, S1 = rgf_amd.canonical_flops(1000, 50, 4)
     assert F1 + S1 == pytest.approx(3.456e11, rel=2e-3)
     assert rgf_amd.mf_size(42, 3, 0) == 42 * 42 * 5
+
+
+def test_footprint_bytes_host_only():
+    """Admission footprints come from the shape and options alone (no device call)."""
+    keep = rgf_amd.default_options(keep_matrix=1)
+    inplace = rgf_amd.default_options(keep_matrix=0)
+    mixed = rgf_amd.default_options(keep_matrix=1, precision=rgf_amd.RGF_MIXED_BF16X3)
+    f = rgf_amd.footprint_bytes(500, 10, 4, 1, keep)
+    assert f == rgf_amd.footprint_bytes(500, 10, 4, 1, None)
+    ldp = 2 * 500 + 8
+    # the factor and the kept matrix: two copies of nt panels of ldp x ns doubles
+    assert f - rgf_amd.footprint_bytes(500, 10, 4, 1, inplace) >= 8 * 10 * ldp * 500
+    assert rgf_amd.footprint_bytes(500, 10, 4, 1, mixed) > f
+    assert rgf_amd.footprint_bytes(500, 10, 4, 3, keep) > 2 * f
+    assert rgf_amd.footprint_bytes(0, 10, 4, 1, keep) == 0
+    assert rgf_amd.footprint_bytes(5, 10, 6, 1, keep) == 0     # nd > ns

@@ -349,17 +349,24 @@ def test_batch_in_place_regenerate(gpu):
         h.factorize()
         assert np.array_equal(h.logdet_batch(), l1)
         assert np.array_equal(h.selinv_diag_batch(), d1)
-    assert rgf_amd.footprint_bytes(2865, 100, 6, 1, 1) > 2 * 8 * 100 * 5736 * 2865
+    keep = rgf_amd.default_options(keep_matrix=1)
+    assert rgf_amd.footprint_bytes(2865, 100, 6, 1, keep) > 2 * 8 * 100 * 5736 * 2865
+    mixed = rgf_amd.default_options(keep_matrix=1, precision=rgf_amd.RGF_MIXED_BF16X3)
+    # the mixed path adds bf16 hi/lo copies of both operands of its largest product
+    assert rgf_amd.footprint_bytes(2865, 100, 6, 1, mixed) >= rgf_amd.footprint_bytes(2865, 100, 6, 1, keep) + \
+        2 * 2 * 2 * 2871 * 2865
 
 
 # ------------------------------------------------------------------ mixed precision (split-BF16 products)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("m,n,k", [(16, 16, 32), (130, 70, 33), (64, 200, 300), (300, 260, 129)])
-def test_gemm_bf16x3_kernel(gpu, ta, tb, m, n, k):
+@pytest.mark.parametrize("kind", [256, 512])
+def test_gemm_bf16x3_kernel(gpu, ta, tb, m, n, k, kind):
     """Split-BF16 product error is bounded by ~2^-16 of sum_k |a_ik||b_kj| (hi*hi + hi*lo +
     lo*hi kept, FP32 accumulation); the layout of the output is checked exactly by the
-    same bound (a misplaced element would be off by O(1))."""
-    say(f"gemm bf16x3 ta={ta} tb={tb} m={m} n={n} k={k}")
+    same bound (a misplaced element would be off by O(1)).  kind 256 splits while staging,
+    512 splits the operands into bf16 copies first."""
+    say(f"gemm bf16x3 kind={kind} ta={ta} tb={tb} m={m} n={n} k={k}")
     rng = np.random.default_rng(m + 3 * n + 7 * k)
     lda, ldb, ldc = (k if ta else m) + 1, (n if tb else k) + 2, m + 3
     A = np.asfortranarray(rng.standard_normal((lda, m if ta else k)))
@@ -372,12 +379,35 @@ def test_gemm_bf16x3_kernel(gpu, ta, tb, m, n, k):
     bound = 1.5 * np.abs(opA) @ np.abs(opB)
     Cf = C.reshape(-1, order="F").copy()
     rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.5, dptr(A.reshape(-1, order="F").copy()), lda,
-                                    dptr(B.reshape(-1, order="F").copy()), ldb, -0.5, dptr(Cf), ldc, 256)
+                                    dptr(B.reshape(-1, order="F").copy()), ldb, -0.5, dptr(Cf), ldc, kind)
     assert rc == 0
     got = Cf.reshape(ldc, n, order="F")
     err = np.abs(got[:m, :n] - ref[:m, :n])
     assert (err <= 4e-5 * bound + 1e-12).all(), (err / (bound + 1e-300)).max()
     assert np.array_equal(got[m:, :], C[m:, :])
+
+
+@pytest.mark.parametrize("ta,tb,flags", [(0, 1, 1), (0, 0, 8), (0, 1, 16), (1, 0, 1), (1, 0, 1 | 4 | 8),
+                                         (0, 0, 2), (1, 1, 0)])
+@pytest.mark.parametrize("m,n,k", [(200, 200, 200), (333, 333, 333), (70, 70, 70)])
+def test_gemm_bf16x3_presplit_matches_staged(gpu, ta, tb, flags, m, n, k):
+    """The pre-split product runs the same MFMA sequence on the same hi/lo values as the
+    staged one, over the same k ranges (triangle flags included), so both give the
+    same bits."""
+    say(f"gemm bf16x3 presplit vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k}")
+    rng = np.random.default_rng(flags + m)
+    lda, ldb, ldc = (k if ta else m), (n if tb else k), m + 1
+    A = rng.standard_normal(lda * (m if ta else k))
+    B = rng.standard_normal(ldb * (k if tb else n))
+    C = rng.standard_normal(ldc * n)
+    out = []
+    for kind in (256, 512):
+        Cf = C.copy()
+        rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.0, dptr(A.copy()), lda, dptr(B.copy()), ldb, 1.0,
+                                        dptr(Cf), ldc, kind | flags)
+        assert rc == 0
+        out.append(Cf)
+    assert np.array_equal(out[0], out[1])
 
 
 @pytest.mark.parametrize("ns,nt,nd", [(200, 6, 6), (65, 4, 5), (300, 3, 0)])
