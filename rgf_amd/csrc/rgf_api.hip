@@ -406,7 +406,7 @@ int factor_sweep(rgf_handle_s* h)
             if ((rc = prod(h, Op::N, Op::T, d, ndesc))) return rc;
         }
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
-                            h->xdiag, h->xstride, s)))
+                            h->xdiag, h->xstride, false, s)))
             return rc;
         if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s))) return rc;
         const bool has_next = i < nt - 1;
@@ -1183,7 +1183,7 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
     int init = INT_MAX;
     if (e == hipSuccess) e = hipMemcpy(info.p, &init, sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: %s", hipGetErrorString(e));
-    int rc = potrf_f64((double*)a.p, lda, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, nullptr);
+    int rc = potrf_f64((double*)a.p, lda, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, true, nullptr);
     if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, 0, n, 1, (double*)tmp.p, 0, (const double*)xd.p, 0, nullptr);
     if (rc) return dev_status(nullptr, rc, "debug_potrf");
     std::vector<double> hp(T);
@@ -1279,7 +1279,7 @@ extern "C" int rgf_debug_bench_potrf(int n, int reps, double* ms_potrf, double* 
     for (int r = 0; r < reps && !rc; r++) {
         rc = copy2d((const double*)a0.p, n, (double*)a.p, n, n, n, 1.0, nullptr);
         hipEventRecord(e0, nullptr);
-        if (!rc) rc = potrf_f64((double*)a.p, n, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, nullptr);
+        if (!rc) rc = potrf_f64((double*)a.p, n, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, false, nullptr);
         hipEventRecord(e1, nullptr);
         hipEventSynchronize(e1);
         float ms = 0.f;

@@ -18,228 +18,334 @@ namespace rgf {
 constexpr int NB = 64;
 
 // ------------------------------------------------------------------ potrf
-// One workgroup factors the kb x kb (kb <= 64) diagonal tile at A[k0, k0] and
-// inverts it.  Each of the 256 lanes owns a 4x4 register block (bi, bj) of the
-// 16x16 block grid (lower blocks only do work), so both sweeps are 16 blocked
-// steps of register math with one or two barriers each:
-//   factor   step K: lane (K,K) factors its 4x4 block; lanes (i,K) solve
-//            L_iK = A_iK L_KK^-T; lanes (i,j), j > K, update A_ij -= L_iK L_jK^T
-//   inverse  step K: lanes (K,J) finish X_KJ = -X_KK T_KJ (X_KK = L_KK^-1);
-//            lanes (I,J), I > K >= J, accumulate T_IJ += L_IK X_KJ
-// The tile is identity-padded beyond kb.  Writes L_kk back (zeros above the
-// diagonal), X_kk = L_kk^-1 to xout (64 x 64, ld 64), the tile's sum of log L_jj
-// (fixed-order tree) and the first non-positive pivot.
+// One workgroup (4 waves) factors the kb x kb (kb <= 64) diagonal tile at A[k0, k0]
+// and inverts it.  The tile is identity-padded beyond kb and held in LDS.
+//   factor   four 16-column panels.  Wave 0 factors a panel with one row per lane in
+//            registers: step j broadcasts the pivot of lane j (readlane), scales by
+//            1/sqrt(pivot) (v_rsq_f64 + two Newton steps) and updates the rest of
+//            the panel with column j (published in LDS).  Then the four waves apply
+//            the rank-16 update to the trailing 16 x 16 blocks on FP64 MFMA.
+//   inverse  X = L^-1 by blocks: the four 16 x 16 diagonal blocks by forward
+//            substitution (one column per lane), then the merges
+//            X21 = -X22 (L21 X11) at 16 -> 32 and 32 -> 64.
+// Writes X_kk to xout (64 x 64, ld 64), the tile's sum of log L_jj (fixed shuffle
+// tree), the first non-positive pivot and, if store_l, L_kk back into the tile (zeros
+// above the diagonal).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int LP = NB + 2;   // LDS row pitch in doubles (16-byte aligned rows)
+
+__device__ __forceinline__ double bcast_lane(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/sqrt(d) for d > 0: hardware estimate y0 and one third-order correction
+// y = y0 (1 + e/2 + 3 e^2/8), e = 1 - d y0^2 (relative error O(e^3))
+__device__ __forceinline__ double rsqrt_nr(double d)
+{
+    const double y0 = __builtin_amdgcn_rsq(d);
+    const double e = fma(-d * y0, y0, 1.0);
+    return fma(y0, e * fma(e, 0.375, 0.5), y0);
+}
+
 __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, long long sA, int k0, int kb,
-                                                         double* xout, long long sX, double* logdet_slot,
-                                                         long long sP, int* info, int info_base)
+                                                         int below, double* xout, long long sX, double* logdet_slot,
+                                                         long long sP, int* info, int info_base, bool store_l)
 {
     {
-        const long long b = blockIdx.x;   // batch entry
+        const long long b = blockIdx.y;   // batch entry
         A += b * sA;
         xout += b * sX;
         logdet_slot += b * sP;
         info += b;
     }
-    __shared__ double L[NB][NB + 1];
-    __shared__ double X[NB][NB + 1];
-    __shared__ double red[NB];
-    __shared__ int bad_col;
-    const int tid = threadIdx.x;
-    const int bi = tid >> 4, bj = tid & 15;
-    const bool lower = bi >= bj;
-    if (tid == 0) bad_col = INT_MAX;
-    double a[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = 4 * bi + u, c = 4 * bj + v;
-            double x = 0.0;
-            if (r < kb && c < kb) { if (r >= c) x = A[(long long)(k0 + c) * lda + (k0 + r)]; }
-            else if (r == c) x = 1.0;
-            a[u][v] = x;
-        }
-    __syncthreads();
-    // ---- factor
-    for (int K = 0; K < 16; K++) {
-        if (bi == K && bj == K) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const double d = a[v][v];
-                const bool bad = !(d > 0.0);
-                const double sd = bad ? 1.0 : sqrt(d);
-                if (bad && 4 * K + v < kb && bad_col == INT_MAX) bad_col = 4 * K + v;
-                a[v][v] = sd;
-                const double ri = 1.0 / sd;
-#pragma unroll
-                for (int u = v + 1; u < 4; u++) a[u][v] *= ri;
-#pragma unroll
-                for (int u = v + 1; u < 4; u++)
-#pragma unroll
-                    for (int w = v + 1; w <= u; w++) a[u][w] -= a[u][v] * a[w][v];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) L[4 * K + u][4 * K + v] = (u >= v) ? a[u][v] : 0.0;
-        }
-        __syncthreads();
-        if (bj == K && bi > K) {
-            double lk[4][4];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v <= u; v++) lk[u][v] = L[4 * K + u][4 * K + v];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const double ri = 1.0 / lk[v][v];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    double x = a[u][v];
-#pragma unroll
-                    for (int w = 0; w < v; w++) x -= a[u][w] * lk[v][w];
-                    a[u][v] = x * ri;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) L[4 * bi + u][4 * K + v] = a[u][v];
-        }
-        __syncthreads();
-        if (bj > K && lower) {
-            double li[4][4], lj[4][4];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int w = 0; w < 4; w++) {
-                    li[u][w] = L[4 * bi + u][4 * K + w];
-                    lj[u][w] = L[4 * bj + u][4 * K + w];
-                }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    double x = a[u][v];
-#pragma unroll
-                    for (int w = 0; w < 4; w++) x -= li[u][w] * lj[v][w];
-                    a[u][v] = x;
-                }
-        }
-    }
-    __syncthreads();
-    // ---- inverse: diagonal blocks first (lanes (K,K)), T accumulators in registers
-    double t[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) t[u][v] = 0.0;
-    double xd[4][4];   // X_ii of this lane's block row (used by lanes with bi == K at step K)
+    const bool lead = blockIdx.x == 0;   // writes X_kk, the log-determinant part and info
+    __shared__ __attribute__((aligned(16))) double Ls[NB][LP];
+    __shared__ __attribute__((aligned(16))) double Xs[NB][LP];
+    __shared__ __attribute__((aligned(16))) double Ts[2][NB / 2][NB / 2 + 2];
+    __shared__ double colb[NB];
+    __shared__ double rinv[NB];
+    __shared__ double dgl[NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    int bad = INT_MAX;   // first non-positive pivot (wave 0, uniform)
     {
-        double l[4][4];
+        // all 16 loads of a lane in flight at once (masked lanes read the tile's corner)
+        double v[NB * NB / 256];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int q = 0; q < NB * NB / 256; q++) {
+            const int r = lane, c = 4 * q + wave;
+            const bool in = r < kb && c < kb;
+            v[q] = A[in ? (long long)(k0 + c) * lda + (k0 + r) : (long long)k0 * lda + k0];
+        }
 #pragma unroll
-            for (int v = 0; v < 4; v++) l[u][v] = (u >= v) ? L[4 * bi + u][4 * bi + v] : 0.0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                double x = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-                for (int q = 0; q < r; q++) x -= l[r][q] * xd[q][c];
-                xd[r][c] = (r >= c) ? x / l[r][r] : 0.0;
-            }
+        for (int q = 0; q < NB * NB / 256; q++) {
+            const int r = lane, c = 4 * q + wave;
+            Ls[r][c] = (r < kb && c < kb) ? (c <= r ? v[q] : 0.0) : (r == c ? 1.0 : 0.0);
+            Xs[r][c] = 0.0;
         }
     }
-    for (int K = 0; K < 16; K++) {
-        if (bi == K && bj <= K) {   // finish X_KJ
-            double xk[4][4];
-            if (bj == K) {
+    __syncthreads();
+
+    // ---- factor
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+    for (int K = 0; K < 4; K++) {
+        const int o = 16 * K;
+        if (wave == 0) {
+            // Step j needs only uniform values on its critical path: the pivot d, and
+            // a = A[j+1][j+1], b = A[j+1][j] of lane j+1 (current).  With y = 1/sqrt(d),
+            // L[j+1][j] = b y and the next pivot is a - (b y)^2, the very operations
+            // lane j+1 performs on its own row.  Each step first starts the next pivot's
+            // 1/sqrt (region A), then does the row updates that hide its latency (region
+            // B): columns j+1, j+2 with readlane values, the rest with column j read
+            // from LDS, applied one step later so no LDS wait stalls the chain.
+            const int r = lane;
+            double p[16], cbv[16];
 #pragma unroll
-                    for (int v = 0; v < 4; v++) xk[u][v] = xd[u][v];
-            } else {
+            for (int c = 0; c < 16; c++) p[c] = Ls[r][o + c];
+            double d = bcast_lane(p[0], o);
+            double b = bcast_lane(p[0], o + 1), a = bcast_lane(p[1], o + 1);
+            bool ok = d > 0.0;
+            double y = rsqrt_nr(ok ? d : 1.0);
+            double lprev = 0.0;
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+            for (int jj = 0; jj < 16; jj++) {
+                const int j = o + jj;
+                // region A: next pivot
+                double s1 = 0.0, dn = 1.0, yn = 1.0;
+                bool okn = true;
+                if (jj < 15) {
+                    s1 = b * y;                     // L[j+1][j]
+                    dn = fma(-s1, s1, a);           // lane j+1's updated diagonal
+                    okn = dn > 0.0;
+                    yn = rsqrt_nr(okn ? dn : 1.0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                // region B: row updates
+                if (jj >= 1 && jj <= 13) {
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        double x = 0.0;
-#pragma unroll
-                        for (int w = 0; w <= u; w++) x -= xd[u][w] * t[w][v];
-                        xk[u][v] = x;
+                    for (int k = jj + 2; k < 16; k++) p[k] = fma(-lprev, cbv[k], p[k]);
+                }
+                const double sd = ok ? d * y : 1.0;
+                if (!ok && j < kb) bad = min(bad, j);
+                if (r == 0) {
+                    rinv[j] = ok ? y : 1.0;
+                    dgl[j] = sd;
+                }
+                // lane j holds p[jj] == d, so its entry is sqrt(d) = d y
+                const double l = r >= j ? p[jj] * y : 0.0;
+                p[jj] = l;
+                if (jj < 15) {
+                    p[jj + 1] = fma(-l, s1, p[jj + 1]);
+                    if (jj < 14) {
+                        const double s2 = bcast_lane(l, j + 2);   // L[j+2][j]
+                        p[jj + 2] = fma(-l, s2, p[jj + 2]);
+                        b = bcast_lane(p[jj + 1], j + 2);
+                        a = bcast_lane(p[jj + 2], j + 2);
                     }
+                    if (jj < 13) {
+                        colb[r] = l;
+                        // one wave's LDS operations complete in issue order, so the reads
+                        // below return every lane's write (and the next step's write comes
+                        // after them); the fence only keeps the compiler from reordering
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+                        for (int k = jj + 3; k < 16; k++) cbv[k] = colb[o + k];
+                        lprev = l;
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                d = dn;
+                y = yn;
+                ok = okn;
             }
+            if (r >= o) {
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) X[4 * K + u][4 * bj + v] = xk[u][v];
+                for (int c = 0; c < 16; c++) Ls[r][o + c] = p[c];
+            }
         }
         __syncthreads();
-        if (bi > K && bj <= K) {    // T_IJ += L_IK X_KJ
-            double li[4][4], xk[4][4];
+        // trailing blocks (I, J), K < J <= I: T_IJ -= L_IK L_JK^T, one MFMA block per wave
+        // (accumulator item q of lane (l16, lq) is T[16 I + l16][16 J + 4 q + lq])
+        const int nbr = 3 - K, nblk = nbr * (nbr + 1) / 2;
+        for (int q = wave; q < nblk; q += 4) {
+            int ii = 0;
+            while ((ii + 1) * (ii + 2) / 2 <= q) ii++;
+            const int I = K + 1 + ii, J = K + 1 + (q - ii * (ii + 1) / 2);
+            v4d acc;
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int t = 0; t < 4; t++) acc[t] = Ls[16 * I + l16][16 * J + 4 * t + lq];
 #pragma unroll
-                for (int w = 0; w < 4; w++) {
-                    li[u][w] = L[4 * bi + u][4 * K + w];
-                    xk[u][w] = X[4 * K + u][4 * bj + w];
-                }
+            for (int ks = 0; ks < 4; ks++) {
+                const double xi = -Ls[16 * I + l16][o + 4 * ks + lq];
+                const double yj = Ls[16 * J + l16][o + 4 * ks + lq];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(yj, xi, acc, 0, 0, 0);
+            }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    double x = t[u][v];
-#pragma unroll
-                    for (int w = 0; w < 4; w++) x += li[u][w] * xk[w][v];
-                    t[u][v] = x;
-                }
+            for (int t = 0; t < 4; t++) Ls[16 * I + l16][16 * J + 4 * t + lq] = acc[t];
         }
+        __syncthreads();
+    }
+    if (wave == 0) {
+        double v = lane < kb ? log(dgl[lane]) : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0 && lead) {
+            *logdet_slot = v;
+            if (bad != INT_MAX) atomicMin(info, info_base + k0 + bad + 1);
+        }
+    }
+
+    // ---- inverse, level 0: wave w inverts diagonal block w (lane c < 16 owns column c)
+    if (lane < 16) {
+        const int o = 16 * wave, c = lane;
+        double x[16];
+#pragma unroll
+        for (int rr = 0; rr < 16; rr++) {
+            double s = rr == c ? 1.0 : 0.0;
+#pragma unroll
+            for (int q = 0; q < rr; q++) s = fma(-Ls[o + rr][o + q], x[q], s);
+            x[rr] = s * rinv[o + rr];
+        }
+#pragma unroll
+        for (int rr = 0; rr < 16; rr++) Xs[o + rr][o + c] = x[rr];
+    }
+    __syncthreads();
+    // ---- level 1 (waves 0, 1: pair p = wave of 16-blocks) and level 2 (four waves, one
+    // 16 x 16 block each): T = L21 X11, then X21 = -X22 T, on FP64 MFMA.  Accumulator
+    // item t of lane (l16, lq) is row l16, column 4 t + lq of the 16 x 16 output block.
+    if (wave < 2) {
+        const int o = 32 * wave;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[o + 4 * ks + lq][o + l16], Ls[o + 16 + l16][o + 4 * ks + lq],
+                                                       acc, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) Ts[wave][l16][4 * t + lq] = acc[t];
+    }
+    __syncthreads();
+    if (wave < 2) {
+        const int o = 32 * wave;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[wave][4 * ks + lq][l16], Xs[o + 16 + l16][o + 16 + 4 * ks + lq],
+                                                       acc, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) Xs[o + 16 + l16][o + 4 * t + lq] = -acc[t];
+    }
+    __syncthreads();
+    {
+        const int bi = 16 * (wave >> 1), bj = 16 * (wave & 1);
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 8; ks++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[4 * ks + lq][bj + l16], Ls[32 + bi + l16][4 * ks + lq], acc, 0,
+                                                       0, 0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) Ts[0][bi + l16][bj + 4 * t + lq] = acc[t];
+        __syncthreads();
+        acc = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 8; ks++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[0][4 * ks + lq][bj + l16], Xs[32 + bi + l16][32 + 4 * ks + lq],
+                                                       acc, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) Xs[32 + bi + l16][bj + 4 * t + lq] = -acc[t];
     }
     __syncthreads();
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e & (NB - 1), c = e >> 6;
-        xout[c * NB + r] = (r >= c) ? X[r][c] : 0.0;
-        if (r < kb && c < kb) A[(long long)(k0 + c) * lda + (k0 + r)] = (r >= c) ? L[r][c] : 0.0;
+        if (lead) {
+            xout[c * NB + r] = Xs[r][c];
+            if (store_l && r < kb && c < kb) A[(long long)(k0 + c) * lda + (k0 + r)] = r >= c ? Ls[r][c] : 0.0;
+        }
     }
-    if (tid < NB) red[tid] = tid < kb ? log(L[tid][tid]) : 0.0;
-    __syncthreads();
-    for (int off = NB / 2; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
+
+    // ---- the panel below the tile, in place: L[R][k] = A[R][k] X_kk^T for row blocks R
+    // of 64 (block rb by workgroup rb mod gridDim.x).  Each wave computes a 32 x 32
+    // quarter on FP64 MFMA; item t of lane (l16, lq) is row l16, column 4 t + lq of
+    // each 16 x 16 block.
+    const int nrb = (below + NB - 1) / NB;
+    const int wm = wave & 1, wn = wave >> 1;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int r0 = k0 + kb + rb * NB, rows = min(NB, below - rb * NB);
+        __syncthreads();   // Ls is free (inverse, stores or the previous block done)
+        {
+            double v[NB * NB / 256];
+#pragma unroll
+            for (int q = 0; q < NB * NB / 256; q++) {
+                const int r = lane, c = 4 * q + wave;
+                const bool in = r < rows && c < kb;
+                v[q] = A[in ? (long long)(k0 + c) * lda + (r0 + r) : (long long)k0 * lda + r0];
+            }
+#pragma unroll
+            for (int q = 0; q < NB * NB / 256; q++) {
+                const int r = lane, c = 4 * q + wave;
+                Ls[r][c] = (r < rows && c < kb) ? v[q] : 0.0;
+            }
+        }
         __syncthreads();
-    }
-    if (tid == 0) {
-        *logdet_slot = red[0];
-        if (bad_col != INT_MAX) atomicMin(info, info_base + k0 + bad_col + 1);
+        v4d acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < NB / 4; ks++) {
+            double af[2], bf[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) af[a] = Ls[32 * wm + 16 * a + l16][4 * ks + lq];
+#pragma unroll
+            for (int b = 0; b < 2; b++) bf[b] = Xs[32 * wn + 16 * b + l16][4 * ks + lq];   // X^T[k][j] = X[j][k]
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            const int row = 32 * wm + 16 * a + l16;
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int col = 32 * wn + 16 * b + 4 * t + lq;
+                    if (row < rows && col < kb) A[(long long)(k0 + col) * lda + (r0 + row)] = acc[a][b][t];
+                }
+        }
     }
 }
 
 // Blocked right-looking Cholesky of the n x n block at A (lower): per 64-wide panel
-// the tile kernel, the rows below as an in-place product with X_kk^T (the output has
-// one column tile, so every workgroup reads only the rows it writes), and the
-// trailing lower update on MFMA.  xdiag receives the T inverted diagonal tiles.
+// one launch of the tile kernel (factor + inverse of the diagonal tile, then the rows
+// below as an in-place product with X_kk^T, spread over the launch's workgroups) and
+// the trailing lower update on MFMA.  xdiag receives the T inverted diagonal tiles.
 int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
-              int info_base, double* xdiag, long long sX, hipStream_t s)
+              int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s)
 {
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int kb = min(NB, n - k0);
         const int below = n - k0 - kb;
         double* xt = xdiag + (long long)(k0 / NB) * NB * NB;
-        hipLaunchKernelGGL(potrf_tile_kernel, dim3(count), dim3(256), 0, s, A, lda, sA, k0, kb, xt, sX,
-                           logdet_parts + k0 / NB, sP, info, info_base);
+        // every workgroup of a matrix factors the tile (in LDS) and then produces its
+        // share of the 64-row blocks of the panel below: no separate panel launch
+        const int nrb = (below + NB - 1) / NB;
+        const int nf = std::max(1, std::min(nrb, std::max(1, 256 / count)));
+        hipLaunchKernelGGL(potrf_tile_kernel, dim3(nf, count), dim3(256), 0, s, A, lda, sA, k0, kb, below, xt, sX,
+                           logdet_parts + k0 / NB, sP, info, info_base, store_diag_l);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
         if (below > 0) {
             double* Pn = A + (long long)k0 * lda + (k0 + kb);
-            GemmDesc p = gdesc(Pn, lda, xt, NB, Pn, lda, below, kb, kb, 1.0, 0.0, G_BUPPER);
-            batch2(p, count, sA, sX, sA);
-            int rc = gemm_f64(Op::N, Op::T, &p, 1, s);
-            if (rc) return rc;
             double* Cn = A + (long long)(k0 + kb) * lda + (k0 + kb);
             GemmDesc d = gdesc(Pn, lda, Pn, lda, Cn, lda, below, below, kb, -1.0, 1.0, G_CLOWER);
             batch2(d, count, sA, sA, sA);
-            rc = gemm_f64(Op::N, Op::T, &d, 1, s);
+            int rc = gemm_f64(Op::N, Op::T, &d, 1, s);
             if (rc) return rc;
         }
     }
@@ -249,8 +355,8 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
 // one launch of the tile kernel on the leading tile (tuning only)
 int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s)
 {
-    hipLaunchKernelGGL(potrf_tile_kernel, dim3(1), dim3(256), 0, s, A, lda, 0LL, 0, NB, xdiag, 0LL, logdet_slot, 0LL,
-                       info, 0);
+    hipLaunchKernelGGL(potrf_tile_kernel, dim3(1, 1), dim3(256), 0, s, A, lda, 0LL, 0, NB, 0, xdiag, 0LL, logdet_slot,
+                       0LL, info, 0, false);
     return (int)hipGetLastError();
 }
 

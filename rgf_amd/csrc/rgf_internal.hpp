@@ -95,8 +95,10 @@ inline void batch2(GemmDesc& d, int count, long long sA, long long sB, long long
 // Every launcher below works on `count` matrices at the given element strides
 // (s* arguments); info is one int per matrix.
 // xdiag: T = ceil(n/64) tiles of 64 x 64 per matrix receiving the inverted diagonal tiles.
+// The diagonal tiles of A receive L only if store_diag_l (the factor sweep replaces
+// them by their inverses from xdiag and skips the store).
 int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
-              int info_base, double* xdiag, long long sX, hipStream_t s);
+              int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s);
 // In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
                       const double* xdiag, long long sX, hipStream_t s);

@@ -107,16 +107,19 @@ def test_potrf_trtri_kernels(gpu, n):
     assert np.abs(X - Xr).max() < 1e-11 * np.abs(Xr).max()
 
 
-def test_potrf_not_spd(gpu):
-    say("potrf not SPD")
+@pytest.mark.parametrize("bad", [70, 0, 20, 63, 99])
+def test_potrf_not_spd(gpu, bad):
+    """The first non-positive pivot is reported 1-based, wherever it falls in a tile."""
+    say(f"potrf not SPD at {bad}")
     n = 100
     A = np.eye(n)
-    A[70, 70] = -1.0
+    A[bad, bad] = -1.0
+    A[min(bad + 3, n - 1), min(bad + 3, n - 1)] = 0.0   # a later bad pivot is not reported
     flat = A.reshape(-1, order="F").copy()
     import ctypes as C
     ld = C.c_double()
     assert _lib.load().rgf_debug_potrf(n, dptr(flat), n, 0, C.byref(ld)) == rgf_amd.RGF_ENOTSPD
-    assert b"column 71" in _lib.load().rgf_last_error(None)
+    assert f"column {bad + 1}".encode() in _lib.load().rgf_last_error(None)
 
 
 # ------------------------------------------------------------------ full path vs oracle
