@@ -14,7 +14,8 @@ extern "C" {
  * arrays, column-major; ta/tb: 0 = N, 1 = T; flags as the kernel's GemmFlags
  * (1 lower-C stores, 2/4 op(A) lower/upper, 8/16 op(B) lower/upper); flag 256 runs the
  * split-BF16 kernel of the mixed-precision path instead of the FP64 one, flag 512
- * the split-BF16 product with operands split into bf16 hi/lo copies first.
+ * the split-BF16 product with operands split into bf16 hi/lo copies first; bits 12-13
+ * force the FP64 tile shape (1 = 64x64, 2 = 128x64, 3 = 128x128; 0 automatic).
  * A is (ta ? k x m : m x k) with ld lda, B is (tb ? n x k : k x n) with ld ldb. */
 int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int flags);

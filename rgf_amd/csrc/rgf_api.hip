@@ -327,6 +327,7 @@ rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, size_t count, const rg
     hipError_t e = hipSetDevice(h->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (h->stream) hipStreamDestroy(h->stream);
         *rc = set_err(nullptr, RGF_EDEVICE, "no usable HIP device %d (%s)", h->device, hipGetErrorString(e));
         delete h;
         return nullptr;
@@ -1155,7 +1156,7 @@ extern "C" int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha,
         rc = gemm_bf16x3_split(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, (__bf16*)sc.p, sce, s);
     } else {
         rc = (flags & 0x100) ? gemm_bf16x3(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s)
-                             : gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s);
+                             : gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s, ((flags >> 12) & 3) - 1);
     }
     if (rc) return dev_status(nullptr, rc, "debug_gemm");
     e = hipDeviceSynchronize();
@@ -1237,7 +1238,9 @@ extern "C" int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int fla
     auto launch = [&]() {
         const Op oa = ta ? Op::T : Op::N, ob = tb ? Op::T : Op::N;
         if (flags & 0x200) return gemm_bf16x3_split(oa, ob, &d, (__bf16*)sc.p, sce, nullptr);
-        return (flags & 0x100) ? gemm_bf16x3(oa, ob, &d, 1, nullptr) : gemm_f64(oa, ob, &d, 1, nullptr);
+        // bits 12-13: FP64 tile shape + 1 (0 = automatic)
+        return (flags & 0x100) ? gemm_bf16x3(oa, ob, &d, 1, nullptr)
+                               : gemm_f64(oa, ob, &d, 1, nullptr, ((flags >> 12) & 3) - 1);
     };
     int rc = launch();
     hipEvent_t e0, e1;

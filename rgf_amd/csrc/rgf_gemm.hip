@@ -1,7 +1,8 @@
 // FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64).
 //
-// One workgroup (4 waves, 256 lanes) computes a 64x64 tile of C; each wave a
-// 32x32 quarter as 2x2 MFMA 16x16 blocks.  K advances in steps of 16 through
+// One workgroup (4 waves, 256 lanes) computes a TM x TN tile of C (64x64, 128x64 or
+// 128x128, chosen per launch from the tile count); each wave a quarter as MFMA
+// 16x16 blocks.  K advances in steps of 16 through
 // LDS (op(A) and op(B) stored k-major, one double of padding per row), with the
 // next step's global loads issued into registers before the current step's
 // MFMAs.  The MFMA is fed (op(B) fragment, op(A) fragment), so each lane's four
@@ -22,7 +23,10 @@ namespace rgf {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64, BN = 64, BK = 16, PADL = 1;
+constexpr int BK = 16, PADL = 1;
+
+// resident workgroups per CU the register budget of each tile shape allows
+constexpr int gemm_occupancy(int tm, int tn) { return tm * tn >= 128 * 128 ? 2 : (tm * tn >= 128 * 64 ? 3 : 5); }
 
 __device__ __forceinline__ void select_desc(const GemmArgs& args, int z, GemmDesc& D)
 {
@@ -32,11 +36,14 @@ __device__ __forceinline__ void select_desc(const GemmArgs& args, int z, GemmDes
         if (q < args.nd && z >= args.d[q].zbase) D = args.d[q];
 }
 
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
+template <bool TA, bool TB, int TM, int TN>
+__global__ __launch_bounds__(256, gemm_occupancy(TM, TN)) void gemm_f64_kernel(GemmArgs args)
 {
-    __shared__ double As[BK][BM + PADL];
-    __shared__ double Bs[BK][BN + PADL];
+    constexpr int WM = TM / 2, WN = TN / 2;          // wave tile (waves in 2 x 2)
+    constexpr int FA = WM / 16, FB = WN / 16;        // MFMA blocks per wave edge
+    constexpr int LA = TM * BK / 256, LB = TN * BK / 256;   // loads per lane per k-step
+    __shared__ double As[BK][TM + PADL];
+    __shared__ double Bs[BK][TN + PADL];
 
     const int z = blockIdx.y;
     GemmDesc D;
@@ -65,8 +72,8 @@ __global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
         ti = ti0 + in % gsz;
         tj = in / gsz;
     }
-    const int i0 = ti * BM, j0 = tj * BN;
-    if ((flags & G_CLOWER) && j0 > i0 + BM - 1) return;
+    const int i0 = ti * TM, j0 = tj * TN;
+    if ((flags & G_CLOWER) && j0 > i0 + TM - 1) return;
 
     const int bz = z - D.zbase;
     const long long b1 = bz % D.count, b2 = bz / D.count;
@@ -78,10 +85,10 @@ __global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
     const int lda = D.lda, ldb = D.ldb;
 
     int klo = 0, khi = K;
-    if (flags & G_ALOWER) khi = min(khi, i0 + BM);
+    if (flags & G_ALOWER) khi = min(khi, i0 + TM);
     if (flags & G_AUPPER) klo = max(klo, i0);
     if (flags & G_BLOWER) klo = max(klo, j0);
-    if (flags & G_BUPPER) khi = min(khi, j0 + BN);
+    if (flags & G_BUPPER) khi = min(khi, j0 + TN);
     klo = (klo / BK) * BK;
 
     const int tid = threadIdx.x;
@@ -89,45 +96,55 @@ __global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lq = lane >> 4;
 
-    v4d acc[2][2];
+    v4d acc[FA][FB];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < FA; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < FB; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
 
-    double ra[4], rb[4];
+    double ra[LA], rb[LB];
+    // lane -> (row or column, k) of the staged tiles: rows (columns) contiguous in
+    // memory take (tid % TM, tid / TM + (256 / TM) q), k-contiguous ones (tid / 16 + 16 q, tid % 16)
+    auto a_rk = [&](int q, int& r, int& kk) {
+        if (!TA) { r = tid % TM; kk = tid / TM + (256 / TM) * q; }
+        else     { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+    };
+    auto b_ck = [&](int q, int& c, int& kk) {
+        if (!TB) { kk = tid & 15; c = (tid >> 4) + 16 * q; }
+        else     { c = tid % TN; kk = tid / TN + (256 / TN) * q; }
+    };
 
     // global -> register loads for the k-step starting at k0
     auto load_tiles = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (!TA) {   // op(A)[i][k] = A[i + k*lda]; rows contiguous
-                int r = tid & 63, kk = (tid >> 6) + 4 * q;
-                int gi = i0 + r, gk = k0 + kk;
-                ra[q] = (gi < M && gk < khi) ? A[(long long)gk * lda + gi] : 0.0;
-            } else {     // op(A)[i][k] = A[k + i*lda]; k contiguous
-                int kk = tid & 15, r = (tid >> 4) + 16 * q;
-                int gi = i0 + r, gk = k0 + kk;
-                ra[q] = (gi < M && gk < khi) ? A[(long long)gi * lda + gk] : 0.0;
-            }
-            if (!TB) {   // op(B)[k][j] = B[k + j*ldb]; k contiguous
-                int kk = tid & 15, c = (tid >> 4) + 16 * q;
-                int gj = j0 + c, gk = k0 + kk;
-                rb[q] = (gj < N && gk < khi) ? B[(long long)gj * ldb + gk] : 0.0;
-            } else {     // op(B)[k][j] = B[j + k*ldb]; cols contiguous
-                int c = tid & 63, kk = (tid >> 6) + 4 * q;
-                int gj = j0 + c, gk = k0 + kk;
-                rb[q] = (gj < N && gk < khi) ? B[(long long)gk * ldb + gj] : 0.0;
-            }
+        for (int q = 0; q < LA; q++) {
+            int r, kk;
+            a_rk(q, r, kk);
+            const int gi = i0 + r, gk = k0 + kk;
+            // op(A)[i][k] = A[i + k*lda] (N) or A[k + i*lda] (T)
+            ra[q] = (gi < M && gk < khi) ? A[TA ? (long long)gi * lda + gk : (long long)gk * lda + gi] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < LB; q++) {
+            int c, kk;
+            b_ck(q, c, kk);
+            const int gj = j0 + c, gk = k0 + kk;
+            // op(B)[k][j] = B[k + j*ldb] (N) or B[j + k*ldb] (T)
+            rb[q] = (gj < N && gk < khi) ? B[TB ? (long long)gk * ldb + gj : (long long)gj * ldb + gk] : 0.0;
         }
     };
     auto store_tiles = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (!TA) As[(tid >> 6) + 4 * q][tid & 63] = ra[q];
-            else     As[tid & 15][(tid >> 4) + 16 * q] = ra[q];
-            if (!TB) Bs[tid & 15][(tid >> 4) + 16 * q] = rb[q];
-            else     Bs[(tid >> 6) + 4 * q][tid & 63] = rb[q];
+        for (int q = 0; q < LA; q++) {
+            int r, kk;
+            a_rk(q, r, kk);
+            As[kk][r] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < LB; q++) {
+            int c, kk;
+            b_ck(q, c, kk);
+            Bs[kk][c] = rb[q];
         }
     };
 
@@ -139,15 +156,15 @@ __global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ks++) {
             const int kr = ks * 4 + lq;
-            double af[2], bf[2];
+            double af[FA], bf[FB];
 #pragma unroll
-            for (int a = 0; a < 2; a++) af[a] = As[kr][wm * 32 + a * 16 + l16];
+            for (int a = 0; a < FA; a++) af[a] = As[kr][wm * WM + a * 16 + l16];
 #pragma unroll
-            for (int b = 0; b < 2; b++) bf[b] = Bs[kr][wn * 32 + b * 16 + l16];
+            for (int b = 0; b < FB; b++) bf[b] = Bs[kr][wn * WN + b * 16 + l16];
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+            for (int a = 0; a < FA; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++)
+                for (int b = 0; b < FB; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
@@ -159,13 +176,13 @@ __global__ __launch_bounds__(256, 5) void gemm_f64_kernel(GemmArgs args)
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
-        const int gi = i0 + wm * 32 + a * 16 + l16;
+    for (int a = 0; a < FA; a++) {
+        const int gi = i0 + wm * WM + a * 16 + l16;
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
+        for (int b = 0; b < FB; b++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gj = j0 + wn * 32 + b * 16 + 4 * r + lq;
+                const int gj = j0 + wn * WN + b * 16 + 4 * r + lq;
                 if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
                     double v = alpha * acc[a][b][r];
                     if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
@@ -202,19 +219,35 @@ double canonical_flops(const GemmDesc& d)
 }
 }  // namespace
 
-int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
+template <int TM, int TN>
+static void launch_f64(Op ta, Op tb, const GemmArgs& args, dim3 grid, hipStream_t s)
+{
+    const dim3 block(256);
+    if (ta == Op::N && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<false, false, TM, TN>), grid, block, 0, s, args);
+    else if (ta == Op::N && tb == Op::T) hipLaunchKernelGGL((gemm_f64_kernel<false, true, TM, TN>), grid, block, 0, s, args);
+    else if (ta == Op::T && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<true, false, TM, TN>), grid, block, 0, s, args);
+    else hipLaunchKernelGGL((gemm_f64_kernel<true, true, TM, TN>), grid, block, 0, s, args);
+}
+
+// Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, -1 = automatic.
+// Measured on MI355X at the sweep's shapes (ns = 2865, single and 4-matrix batches):
+// 64x64 at 5 workgroups per CU is the fastest or within a few percent everywhere
+// (the larger tiles lose occupancy), so the automatic choice is 64x64.
+int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
 {
     if (ndesc <= 0) return 0;
     if (ndesc > GEMM_MAXD) return (int)hipErrorInvalidValue;
+    if (cfg < 0) cfg = 0;
+    const int TM = cfg >= 1 ? 128 : 64, TN = cfg == 2 ? 128 : 64;
     GemmArgs args{};
     int z = 0, maxtiles = 0, nd = 0;
     for (int q = 0; q < ndesc; q++) {
         GemmDesc d = descs[q];
         if (d.count2 < 1) d.count2 = 1;
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
-        d.tiles_m = (d.m + BM - 1) / BM;
-        d.tiles_n = (d.n + BN - 1) / BN;
-        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n;
+        d.tiles_m = (d.m + TM - 1) / TM;
+        d.tiles_n = (d.n + TN - 1) / TN;
+        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n && TM == TN;
         d.zbase = z;
         z += d.count * d.count2;
         const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
@@ -223,13 +256,12 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
     }
     if (nd == 0) return 0;
     args.nd = nd;
-    dim3 grid(maxtiles, z), block(256);
+    dim3 grid(maxtiles, z);
     const int cls = (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0);
     void* tok = timing_begin(cls, args.d, nd, s);
-    if (ta == Op::N && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, args);
-    else if (ta == Op::N && tb == Op::T) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, args);
-    else if (ta == Op::T && tb == Op::N) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, args);
-    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, args);
+    if (cfg == 2) launch_f64<128, 128>(ta, tb, args, grid, s);
+    else if (cfg == 1) launch_f64<128, 64>(ta, tb, args, grid, s);
+    else launch_f64<64, 64>(ta, tb, args, grid, s);
     hipError_t err = hipGetLastError();
     timing_end(tok, s);
     return (int)err;

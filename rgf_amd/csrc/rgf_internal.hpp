@@ -56,7 +56,8 @@ enum class Op { N, T };
 
 // Launch one GEMM "call" made of up to GEMM_MAXD descriptors that share the
 // transposes (all entries are independent).  Returns hipError_t as int.
-int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
+// cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128; -1 automatic).
+int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
 
 // Same interface on split-BF16 MFMA (three bf16 products per FP64 product, FP32
 // accumulation inside a call, FP64 alpha/beta epilogue).

@@ -31,9 +31,12 @@ def dptr(a):
 
 # ------------------------------------------------------------------ kernels
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(16, 16, 4), (64, 64, 16), (130, 70, 33), (5, 200, 129)])
-def test_gemm_kernel(gpu, ta, tb, m, n, k):
-    say(f"gemm ta={ta} tb={tb} m={m} n={n} k={k}")
+@pytest.mark.parametrize("m,n,k", [(16, 16, 4), (64, 64, 16), (130, 70, 33), (5, 200, 129), (300, 270, 40)])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+def test_gemm_kernel(gpu, ta, tb, m, n, k, tile):
+    """FP64 product against numpy, for the automatic tile shape (0) and each forced
+    one (1 = 64x64, 2 = 128x64, 3 = 128x128; bits 12-13 of the debug flags)."""
+    say(f"gemm tile={tile} ta={ta} tb={tb} m={m} n={n} k={k}")
     rng = np.random.default_rng(m * 7 + n + k)
     lda, ldb, ldc = (k if ta else m) + 3, (n if tb else k) + 1, m + 2
     A = np.asfortranarray(rng.standard_normal((lda, m if ta else k)))
@@ -45,15 +48,16 @@ def test_gemm_kernel(gpu, ta, tb, m, n, k):
     ref[:m, :n] = 1.5 * opA @ opB - 0.5 * C[:m, :n]
     Cf = C.reshape(-1, order="F").copy()
     rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.5, dptr(A.reshape(-1, order="F").copy()), lda,
-                                    dptr(B.reshape(-1, order="F").copy()), ldb, -0.5, dptr(Cf), ldc, 0)
+                                    dptr(B.reshape(-1, order="F").copy()), ldb, -0.5, dptr(Cf), ldc, tile << 12)
     assert rc == 0
     got = Cf.reshape(ldc, n, order="F")
     assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * k
 
 
 @pytest.mark.parametrize("flags", [1, 2, 8, 16, 1 | 4 | 8])
-def test_gemm_kernel_triangular_flags(gpu, flags):
-    say(f"gemm flags={flags}")
+@pytest.mark.parametrize("tile", [0, 2, 3])
+def test_gemm_kernel_triangular_flags(gpu, flags, tile):
+    say(f"gemm tile={tile} flags={flags}")
     rng = np.random.default_rng(flags)
     m = n = k = 150
     L = np.tril(rng.standard_normal((k, k)))
@@ -77,7 +81,7 @@ def test_gemm_kernel_triangular_flags(gpu, flags):
         ref = np.where(np.tril(np.ones((m, n))) > 0, ref, 7.0)
     Cf = C0.reshape(-1, order="F").copy()
     rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 1.0, dptr(A.reshape(-1, order="F").copy()), k,
-                                    dptr(B.reshape(-1, order="F").copy()), k, 0.0, dptr(Cf), m, flags)
+                                    dptr(B.reshape(-1, order="F").copy()), k, 0.0, dptr(Cf), m, flags | (tile << 12))
     assert rc == 0
     assert np.abs(Cf.reshape(m, n, order="F") - ref).max() < 1e-11
 
