@@ -50,7 +50,36 @@ def parse():
     p.add_argument("--c3", action="store_true", help="c3: 64 matrices (1500, 60, 5), per-instance theta")
     p.add_argument("--only-batched", choices=["fp64", "mixed"], default=None,
                    help="run only the batched measurement of one precision (for profiling)")
+    p.add_argument("--profile-step", action="store_true",
+                   help="only generate the matrix and run one step (for profiler counter runs)")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB three times (scale for the HBM "
+                        "byte counters FETCH_SIZE / WRITE_SIZE of a profiler run)")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy kernel of known size (reads and writes mib MiB per
+    launch), to scale the profiler's FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": n * 8, "launches": 3}), flush=True)
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the NN GEMM class from the committed profiler summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_gemm_nn_c2.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(ns, nt, nd, seed, blocks):
@@ -161,14 +190,16 @@ def run_c3(args):
 
 def main():
     args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the solver runs one "
+              "handle per GPU and has no multi-GPU path to launch.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.serve:
         return run_serve(args)
     if args.c3:
         return run_c3(args)
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and not launched here.", file=sys.stderr)
-        sys.exit(2)
     # host threads for the CPU baseline: the box's share (16), set before MKL loads
     os.environ.setdefault("MKL_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
     import numpy as np
@@ -193,6 +224,20 @@ def main():
         ld = s.logDet()
         s.selinv_diag_device(diag)
         return ld
+
+    if args.profile_step:   # one step, NN class timed with HIP events (compare with a profiler)
+        rgf_amd.kernel_timing_enable(0)
+        t0 = time.perf_counter()
+        step()
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        kms, kfl, kn = rgf_amd.kernel_timing_read()
+        rgf_amd.kernel_timing_enable(-1)
+        print(json.dumps({"profile_step_s": round(dt, 4), "nn_launches": kn,
+                          "nn_avg_launch_ms": round(kms / max(kn, 1), 4),
+                          "nn_TFLOPs": round(kfl / (kms * 1e-3) / 1e12, 3) if kms > 0 else None}), flush=True)
+        s.close()
+        return
 
     for _ in range(args.warmup):
         step()
@@ -286,7 +331,7 @@ def main():
         "peak": PEAK_FP64_MFMA,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_MFMA, 4) if achieved else None,
-        "traffic": None,
+        "traffic": pmc_traffic(),
         "launches": kn,
         "avg_launch_ms": round(kms / kn, 4) if kn else None,
     }
