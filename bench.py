@@ -84,27 +84,47 @@ def pmc_traffic():
 
 def cpu_baseline(ns, nt, nd, seed, blocks):
     """The oracle (reference block algorithm on host MKL) on `blocks` of the workload's
-    nt blocks; throughput in the same unit as the GPU value."""
+    nt blocks with the box's threads, and on a quarter of them with one thread (the
+    reference links sequential MKL); throughput in the same unit as the GPU value."""
     import numpy as np  # noqa: F401
     from oracle import oracle
     import rgf_amd
-    nts = max(2, min(blocks, nt))
-    mf = oracle.generate(seed, ns, nts, nd)
-    t0 = time.perf_counter()
-    L = oracle.factor(mf, ns, nts, nd)
-    oracle.logdet(L, ns, nts, nd)
-    oracle.selinv(L, ns, nts, nd, exact=True)
-    dt = time.perf_counter() - t0
-    F, S = rgf_amd.canonical_flops(ns, nts, nd)
+
+    def run(nts, threads):
+        oracle.set_threads(threads)
+        mf = oracle.generate(seed, ns, nts, nd)
+        t0 = time.perf_counter()
+        L = oracle.factor(mf, ns, nts, nd)
+        oracle.logdet(L, ns, nts, nd)
+        oracle.selinv(L, ns, nts, nd, exact=True)
+        dt = time.perf_counter() - t0
+        F, S = rgf_amd.canonical_flops(ns, nts, nd)
+        return dt, (F + S) / dt / 1e12
+
     threads = int(os.environ.get("MKL_NUM_THREADS", "1"))
+    nts = max(2, min(blocks, nt))
+    run(2, threads)   # warm-up: MKL thread pool, first-touch of the generator's pages
+    dt, tf = run(nts, threads)
+    nts1 = max(2, nts // 4)
+    dt1, tf1 = run(nts1, 1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
-        "value": round((F + S) / dt / 1e12, 4),
+        "value": round(tf, 4),
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
         "sample": f"factorize+logdet+selinv(exact) of ns={ns} nt={nts} nd={nd} ({nts} of {nt} blocks) "
-                  f"on host MKL, {dt:.2f} s; s/matrix extrapolated to nt={nt}: {dt / nts * nt:.2f}",
+                  f"on host MKL, {threads} threads, {dt:.2f} s; s/matrix extrapolated to nt={nt}: "
+                  f"{dt / nts * nt:.2f}",
         "s_per_matrix_extrapolated": round(dt / nts * nt, 3),
+        "one_thread": {"value": round(tf1, 4), "blocks": nts1, "s": round(dt1, 2),
+                       "s_per_matrix_extrapolated": round(dt1 / nts1 * nt, 2)},
+        "host": {"nproc": os.cpu_count(), "model": model},
     }
 
 
