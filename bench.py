@@ -82,30 +82,39 @@ def pmc_traffic():
         return None
 
 
+def _cpu_sample(ns, nts, nd, seed):
+    """Child-process body: time the oracle on nts blocks (threads from MKL_NUM_THREADS)."""
+    from oracle import oracle
+    oracle.factor(oracle.generate(seed + 1, ns, 2, nd), ns, 2, nd)   # warm-up: thread pool, pages
+    mf = oracle.generate(seed, ns, nts, nd)
+    t0 = time.perf_counter()
+    L = oracle.factor(mf, ns, nts, nd)
+    oracle.logdet(L, ns, nts, nd)
+    oracle.selinv(L, ns, nts, nd, exact=True)
+    print(json.dumps({"s": time.perf_counter() - t0}), flush=True)
+
+
 def cpu_baseline(ns, nt, nd, seed, blocks):
     """The oracle (reference block algorithm on host MKL) on `blocks` of the workload's
-    nt blocks with the box's threads, and on a quarter of them with one thread (the
-    reference links sequential MKL); throughput in the same unit as the GPU value."""
-    import numpy as np  # noqa: F401
-    from oracle import oracle
+    nt blocks with the box's threads, and on an eighth of them with one thread (the
+    reference links sequential MKL); each sample runs in a child process whose MKL
+    thread count is fixed by its environment.  Throughput in the GPU value's unit."""
+    import subprocess
     import rgf_amd
 
     def run(nts, threads):
-        oracle.set_threads(threads)
-        mf = oracle.generate(seed, ns, nts, nd)
-        t0 = time.perf_counter()
-        L = oracle.factor(mf, ns, nts, nd)
-        oracle.logdet(L, ns, nts, nd)
-        oracle.selinv(L, ns, nts, nd, exact=True)
-        dt = time.perf_counter() - t0
+        env = dict(os.environ, MKL_NUM_THREADS=str(threads), OMP_NUM_THREADS=str(threads))
+        code = (f"import sys; sys.path.insert(0, {ROOT!r}); import bench; "
+                f"bench._cpu_sample({ns}, {nts}, {nd}, {seed})")
+        out = subprocess.run([sys.executable, "-c", code], env=env, check=True, capture_output=True, text=True)
+        dt = json.loads(out.stdout.strip().splitlines()[-1])["s"]
         F, S = rgf_amd.canonical_flops(ns, nts, nd)
         return dt, (F + S) / dt / 1e12
 
-    threads = int(os.environ.get("MKL_NUM_THREADS", "1"))
+    threads = min(16, os.cpu_count() or 1)
     nts = max(2, min(blocks, nt))
-    run(2, threads)   # warm-up: MKL thread pool, first-touch of the generator's pages
     dt, tf = run(nts, threads)
-    nts1 = max(2, nts // 4)
+    nts1 = max(2, nts // 8)
     dt1, tf1 = run(nts1, 1)
     model = ""
     try:
@@ -220,8 +229,6 @@ def main():
         return run_serve(args)
     if args.c3:
         return run_c3(args)
-    # host threads for the CPU baseline: the box's share (16), set before MKL loads
-    os.environ.setdefault("MKL_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
     import numpy as np
     import rgf_amd
 

@@ -76,11 +76,6 @@ def load():
         return lib
 
 
-def set_threads(n: int) -> None:
-    """Host threads of MKL for the following oracle calls (the library the oracle links)."""
-    load().rgf_oracle_set_threads(C.c_int(int(n)))
-
-
 # ---------------------------------------------------------------- layout
 def mf_size(ns: int, nt: int, nd: int) -> int:
     """Length of the MF array, RGF.H:95."""

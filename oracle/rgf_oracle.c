@@ -42,11 +42,6 @@ extern void dtrtri_(const char *uplo, const char *diag, const int *n, double *a,
 extern void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag,
                    const int *m, const int *n, const double *alpha, const double *a, const int *lda,
                    double *b, const int *ldb);
-extern void MKL_Set_Num_Threads(int n);
-
-/* Host threads of the MKL instance this library calls (timing of the CPU baseline). */
-void rgf_oracle_set_threads(int n) { MKL_Set_Num_Threads(n); }
-
 extern void dgemm_(const char *ta, const char *tb, const int *m, const int *n, const int *k,
                    const double *alpha, const double *a, const int *lda, const double *b,
                    const int *ldb, const double *beta, double *c, const int *ldc);
