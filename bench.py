@@ -52,6 +52,8 @@ def parse():
                    help="run only the batched measurement of one precision (for profiling)")
     p.add_argument("--profile-step", action="store_true",
                    help="only generate the matrix and run one step (for profiler counter runs)")
+    p.add_argument("--profile-precision", choices=["fp64", "mixed"], default="fp64",
+                   help="precision of the --profile-step run")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB three times (scale for the HBM "
                         "byte counters FETCH_SIZE / WRITE_SIZE of a profiler run)")
@@ -241,7 +243,11 @@ def main():
                           "TFLOPs_canonical": args.batch * (F + S) / tb / 1e12}), flush=True)
         return
     t_gen = time.perf_counter()
-    s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed)
+    if args.profile_step and args.profile_precision == "mixed":
+        s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
+                                  options=rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3))
+    else:
+        s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed)
     s.synchronize()
     t_gen = time.perf_counter() - t_gen
     diag = rgf_amd.DeviceBuffer(s.n)
@@ -253,14 +259,14 @@ def main():
         return ld
 
     if args.profile_step:   # one step, NN class timed with HIP events (compare with a profiler)
-        rgf_amd.kernel_timing_enable(0)
+        rgf_amd.kernel_timing_enable(0 if args.profile_precision == "fp64" else 4)
         t0 = time.perf_counter()
         step()
         s.synchronize()
         dt = time.perf_counter() - t0
         kms, kfl, kn = rgf_amd.kernel_timing_read()
         rgf_amd.kernel_timing_enable(-1)
-        print(json.dumps({"profile_step_s": round(dt, 4), "nn_launches": kn,
+        print(json.dumps({"precision": args.profile_precision, "profile_step_s": round(dt, 4), "nn_launches": kn,
                           "nn_avg_launch_ms": round(kms / max(kn, 1), 4),
                           "nn_TFLOPs": round(kfl / (kms * 1e-3) / 1e12, 3) if kms > 0 else None}), flush=True)
         s.close()
