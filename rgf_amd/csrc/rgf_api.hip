@@ -86,6 +86,7 @@ struct rgf_handle_s {
     double* R = nullptr;        // n x 16 residual of the refinement
     double* Dc = nullptr;       // n x 16 correction of the refinement
     double* work = nullptr;     // skinny split-k partials
+    size_t work_elems = 0;
     int last_refine_iters = 0;  // refinement steps taken by the last mixed solve
     double last_refine_res = 0.0;
     double* scal = nullptr;     // small device scalars
@@ -287,7 +288,8 @@ int ensure_solve_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->R, n16))) return rc;
     if ((rc = dalloc(h, &h->Dc, n16))) return rc;
     const int mx = std::max<int>((int)h->n, 2 * h->ns + h->nd);
-    if ((rc = dalloc(h, &h->work, skinny_work_elems(mx, mx, 16)))) return rc;
+    h->work_elems = skinny_work_elems(mx, mx, 16);
+    if ((rc = dalloc(h, &h->work, h->work_elems))) return rc;
     return RGF_OK;
 }
 
@@ -392,19 +394,20 @@ int factor_sweep(rgf_handle_s* h)
             if (nd > 0 && (rc = copy2d(A + 2 * ns, ldp, h->Ftmp + ns, ldf, nd, ns, 1.0, s, B, ms, fs))) return rc;
         } else {
             const double* Pm = l_panel(h, i - 1);
-            GemmDesc d[2];
-            d[0] = gdesc(Pm + ns, ldp, Pm + ns, ldp, P, ldp, ns, ns, ns, -1.0, 1.0, G_CLOWER);
-            d[0].Cin = A; d[0].ldcin = ldp;
-            batch2(d[0], B, ms, ms, ms);
-            int ndesc = 1;
-            if (nd > 0) {
-                d[1] = gdesc(Pm + 2 * ns, ldp, Pm + ns, ldp, h->Ftmp + ns, ldf, nd, ns, ns, -1.0, 1.0, 0);
-                d[1].Cin = A + 2 * ns; d[1].ldcin = ldp;
-                batch2(d[1], B, ms, ms, fs);
-                d[1].sCin2 = ms;
-                ndesc = 2;
+            GemmDesc d = gdesc(Pm + ns, ldp, Pm + ns, ldp, P, ldp, ns, ns, ns, -1.0, 1.0, G_CLOWER);
+            d.Cin = A; d.ldcin = ldp;
+            batch2(d, B, ms, ms, ms);
+            // arrow rows, A_di - D_{i-1} B_{i-1}^T (nd x ns, k = ns): as a skinny product
+            // in transposed form, Ftmp_d^T = A_d^T - B_{i-1} D_{i-1}^T (16 arrow rows per
+            // launch); a GEMM launch would give it only ns/64 tiles with a long k loop
+            for (int c0 = 0; c0 < nd; c0 += 16) {
+                const int nc = std::min(16, nd - c0);
+                rc = skinny_f64s(Op::N, 0, ns, ns, nc, -1.0, Pm + ns, ldp, Pm + 2 * ns + c0, ldp, 1, 1.0,
+                                 A + 2 * ns + c0, ldp, 1, h->Ftmp + ns + c0, ldf, 1, h->work, h->work_elems, s, B,
+                                 ms, ms, ms, fs);
+                if (rc) return rc;
             }
-            if ((rc = prod(h, Op::N, Op::T, d, ndesc))) return rc;
+            if ((rc = prod(h, Op::N, Op::T, &d, 1))) return rc;
         }
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, false, s)))
@@ -434,7 +437,9 @@ int do_factorize(rgf_handle_s* h)
     if (!h->Ap && h->factorized) return RGF_OK;   // in place: A is gone, the factor is current
     if (h->broken) return set_err(h, RGF_ESTATE, "factorization failed earlier and the matrix was not kept");
     hipSetDevice(h->device);
-    int rc = factor_sweep(h);
+    int rc;
+    if (h->nd > 0 && h->nt > 1 && (rc = ensure_solve_buffers(h))) return rc;   // arrow products use `work`
+    rc = factor_sweep(h);
     if (rc) return dev_status(h, rc, "factorize");
     std::vector<int> info(h->count);
     hipError_t e = hipMemcpyAsync(info.data(), h->dinfo, sizeof(int) * h->count, hipMemcpyDeviceToHost, h->stream);

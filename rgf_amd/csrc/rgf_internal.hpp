@@ -122,6 +122,13 @@ size_t skinny_work_elems(int m, int k, int nr);
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
                double* work, hipStream_t s);
+// Same with element strides: R(kk, c) = R[kk * rsk + c * rsj], Y(i, c) = Y[i * ysi + c * ysj]
+// (Y0 likewise), e.g. row-major operands as transposed views, over `count` matrices at
+// strides sA, sR, sY0, sY in one launch; work holds work_elems doubles (>= count * 16 * m).
+int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda, const double* R,
+                long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
+                double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s,
+                int count = 1, long long sA = 0, long long sR = 0, long long sY0 = 0, long long sY = 0);
 
 // ---------------------------------------------------------------- generator / ingestion
 struct GenParams {

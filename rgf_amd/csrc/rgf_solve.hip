@@ -4,18 +4,21 @@
 // residual (RGF.H:631-663).  The factor stores L_ii^-1, so every triangular
 // solve is a product here and the whole solve is a stream over the factor.
 //
-// N: one lane per row of op(A) (coalesced column reads), the R slab staged in
-//    LDS; k split into slabs across workgroups for occupancy, partials reduced
-//    in a fixed order (deterministic).
-// T: one wave per output column (a column of A read contiguously by 64 lanes),
-//    wave-level reduction; k split into slabs only when there are few columns.
+// N: one lane per row of op(A) (coalesced column reads, 8 columns in flight), the
+//    R chunk staged in LDS; k split into slabs across workgroups for memory-level
+//    parallelism, partials reduced in a fixed order (deterministic).
+// T: one wave per output row (a column of A read contiguously by 64 lanes, 4 loads
+//    in flight), wave-level reduction; k split into slabs the same way.
 #include "rgf_internal.hpp"
 #include <algorithm>
 
 namespace rgf {
 
 constexpr int MAXR = 16;
-constexpr int KCH = 64;   // k-chunk staged in LDS
+constexpr int KCH = 128;   // k-chunk of R staged in LDS (N kernel)
+constexpr int KU = 8;      // A loads in flight per lane
+constexpr int KCT = 256;   // k-chunk of the T kernel (4 per lane x 64 lanes)
+constexpr int TARGET_WG = 2048;
 
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
@@ -23,22 +26,14 @@ struct SkinnyPlan {
     int slabs, kslab;
 };
 
-static SkinnyPlan plan_n(int m, int k)
+// k is split into slabs (partials reduced afterwards) until the launch has ~2048
+// workgroups: the products stream A once from HBM and need many loads in flight.
+// The partials of all `count` matrices must fit the caller's buffer (cap elements).
+static SkinnyPlan plan(int wgm, int k, int chunk, int nrt, int m, int count, size_t cap)
 {
-    const int wgm = cdiv(m, 256);
-    int slabs = std::max(1, std::min(cdiv(k, KCH), cdiv(768, wgm)));
-    slabs = std::min(slabs, 64);
-    int kslab = cdiv(cdiv(k, slabs), KCH) * KCH;
-    slabs = cdiv(k, kslab);
-    return {std::max(slabs, 1), kslab};
-}
-
-static SkinnyPlan plan_t(int m, int k)
-{
-    const int wgm = cdiv(m, 4);
-    int slabs = std::max(1, std::min(cdiv(k, 256), cdiv(1024, wgm)));
-    slabs = std::min(slabs, 64);
-    int kslab = cdiv(cdiv(k, slabs), 64) * 64;
+    int slabs = std::max(1, std::min(cdiv(k, chunk), cdiv(TARGET_WG, (long long)wgm * count)));
+    slabs = (int)std::min<size_t>((size_t)slabs, cap / ((size_t)count * nrt * m));
+    int kslab = cdiv(cdiv(k, slabs), chunk) * chunk;
     slabs = cdiv(k, kslab);
     return {std::max(slabs, 1), kslab};
 }
@@ -50,18 +45,37 @@ size_t skinny_work_elems(int m, int k, int nr)
     return (size_t)64 * (size_t)m * (size_t)MAXR;
 }
 
-// partial layout: work[(s * nr + c) * m + i]
+// Operands by element strides: R(kk, c) = R[kk * rsk + c * rsj], Y(i, c) = Y[i * ysi + c * ysj]
+// (column-major: rsk = ysi = 1), so a row-major R or Y is a transposed view.
+struct SkinnyIO {
+    const double* R;
+    long long rsk, rsj;
+    const double* Y0;
+    long long y0si, y0sj;
+    double* Y;
+    long long ysi, ysj;
+    long long sA, sR, sY0, sY;   // strides between the matrices of a batch (blockIdx.z)
+};
+
+// partial layout: work[((b * slabs + s) * NR + c) * m + i] for matrix b = blockIdx.z
+// N: one lane per row of op(A) = A (column reads coalesced), KU columns of A in flight
 template <int NR>
 __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
-                                                       const double* __restrict__ A, int lda,
-                                                       const double* __restrict__ R, int ldr, double beta,
-                                                       const double* Y0, int ldy0, double* Y, int ldy,
-                                                       double* work, int direct)
+                                                       const double* __restrict__ A, int lda, SkinnyIO io,
+                                                       double beta, double* work, int direct)
 {
     __shared__ double Rs[KCH][NR];
     const int tid = threadIdx.x;
     const int i = blockIdx.x * 256 + tid;
     const int s = blockIdx.y;
+    {
+        const long long b = blockIdx.z;
+        A += b * io.sA;
+        io.R += b * io.sR;
+        io.Y0 += b * io.sY0;
+        io.Y += b * io.sY;
+        work += b * (long long)gridDim.y * NR * m;
+    }
     const int kb = s * kslab, ke = min(k, kb + kslab);
     double acc[NR];
 #pragma unroll
@@ -75,26 +89,20 @@ __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int
         const int kc = min(KCH, kend - k0);
         __syncthreads();
         for (int e = tid; e < KCH * NR; e += 256) {
-            int kk = e % KCH, c = e / KCH;
-            Rs[kk][c] = (kk < kc && c < nr) ? R[(long long)c * ldr + k0 + kk] : 0.0;
+            const int kk = e % KCH, c = e / KCH;
+            Rs[kk][c] = (kk < kc && c < nr) ? io.R[(long long)(k0 + kk) * io.rsk + (long long)c * io.rsj] : 0.0;
         }
         __syncthreads();
         if (i < m) {
             const double* Ap = A + (long long)k0 * lda + i;
-            int kk = 0;
-            for (; kk + 4 <= kc; kk += 4) {
-                double a0 = Ap[(long long)(kk + 0) * lda];
-                double a1 = Ap[(long long)(kk + 1) * lda];
-                double a2 = Ap[(long long)(kk + 2) * lda];
-                double a3 = Ap[(long long)(kk + 3) * lda];
+            for (int kk = 0; kk < kc; kk += KU) {
+                double a[KU];
 #pragma unroll
-                for (int c = 0; c < NR; c++)
-                    acc[c] += a0 * Rs[kk][c] + a1 * Rs[kk + 1][c] + a2 * Rs[kk + 2][c] + a3 * Rs[kk + 3][c];
-            }
-            for (; kk < kc; kk++) {
-                double a0 = Ap[(long long)kk * lda];
+                for (int u = 0; u < KU; u++) a[u] = kk + u < kc ? Ap[(long long)(kk + u) * lda] : 0.0;
 #pragma unroll
-                for (int c = 0; c < NR; c++) acc[c] += a0 * Rs[kk][c];
+                for (int u = 0; u < KU; u++)
+#pragma unroll
+                    for (int c = 0; c < NR; c++) acc[c] = fma(a[u], Rs[min(kk + u, KCH - 1)][c], acc[c]);
             }
         }
     }
@@ -104,8 +112,8 @@ __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int
         for (int c = 0; c < NR; c++) {
             if (c >= nr) break;
             double v = alpha * acc[c];
-            if (beta != 0.0) v += beta * Y0[(long long)c * ldy0 + i];
-            Y[(long long)c * ldy + i] = v;
+            if (beta != 0.0) v += beta * io.Y0[(long long)i * io.y0si + (long long)c * io.y0sj];
+            io.Y[(long long)i * io.ysi + (long long)c * io.ysj] = v;
         }
     } else {
 #pragma unroll
@@ -113,37 +121,50 @@ __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int
     }
 }
 
+// T: one wave per output row j of op(A) = A^T (column j of A read contiguously),
+// lanes over k with 4 loads in flight, wave-level reduction
 template <int NR>
 __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
-                                                       const double* __restrict__ A, int lda,
-                                                       const double* __restrict__ R, int ldr, double beta,
-                                                       const double* Y0, int ldy0, double* Y, int ldy,
-                                                       double* work, int direct)
+                                                       const double* __restrict__ A, int lda, SkinnyIO io,
+                                                       double beta, double* work, int direct)
 {
-    __shared__ double Rs[NR][KCH];
+    __shared__ double Rs[NR][KCT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * 4 + wave;   // output row (column of A)
     const int s = blockIdx.y;
+    {
+        const long long b = blockIdx.z;
+        A += b * io.sA;
+        io.R += b * io.sR;
+        io.Y0 += b * io.sY0;
+        io.Y += b * io.sY;
+        work += b * (long long)gridDim.y * NR * m;
+    }
     const int kb = s * kslab, ke = min(k, kb + kslab);
     double acc[NR];
 #pragma unroll
     for (int c = 0; c < NR; c++) acc[c] = 0.0;
     int kbeg = kb, kend = ke;
     const int jmin = blockIdx.x * 4, jmax = min(m, jmin + 4) - 1;
-    if (tri == 2) kbeg = max(kbeg, (jmin / KCH) * KCH);   // op(A) upper: A^T[j][kk] == 0 for kk < j
+    if (tri == 2) kbeg = max(kbeg, (jmin / KCT) * KCT);   // op(A) upper: A^T[j][kk] == 0 for kk < j
     if (tri == 1) kend = min(kend, jmax + 1);             // op(A) lower
-    for (int k0 = kbeg; k0 < kend; k0 += KCH) {
-        const int kc = min(KCH, kend - k0);
+    const double* Aj = A + (long long)min(j, m - 1) * lda;
+    for (int k0 = kbeg; k0 < kend; k0 += KCT) {
+        const int kc = min(KCT, kend - k0);
         __syncthreads();
-        for (int e = tid; e < KCH * NR; e += 256) {
-            int kk = e % KCH, c = e / KCH;
-            Rs[c][kk] = (kk < kc && c < nr) ? R[(long long)c * ldr + k0 + kk] : 0.0;
+        for (int e = tid; e < KCT * NR; e += 256) {
+            const int kk = e % KCT, c = e / KCT;
+            Rs[c][kk] = (kk < kc && c < nr) ? io.R[(long long)(k0 + kk) * io.rsk + (long long)c * io.rsj] : 0.0;
         }
         __syncthreads();
-        if (j < m && lane < kc) {
-            double a = A[(long long)j * lda + k0 + lane];
+        if (j < m) {
+            double a[KCT / 64];
 #pragma unroll
-            for (int c = 0; c < NR; c++) acc[c] += a * Rs[c][lane];
+            for (int u = 0; u < KCT / 64; u++) a[u] = lane + 64 * u < kc ? Aj[k0 + lane + 64 * u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < KCT / 64; u++)
+#pragma unroll
+                for (int c = 0; c < NR; c++) acc[c] = fma(a[u], Rs[c][lane + 64 * u], acc[c]);
         }
     }
 #pragma unroll
@@ -158,8 +179,8 @@ __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int
         for (int c = 0; c < NR; c++) {
             if (c >= nr) break;
             double v = alpha * acc[c];
-            if (beta != 0.0) v += beta * Y0[(long long)c * ldy0 + j];
-            Y[(long long)c * ldy + j] = v;
+            if (beta != 0.0) v += beta * io.Y0[(long long)j * io.y0si + (long long)c * io.y0sj];
+            io.Y[(long long)j * io.ysi + (long long)c * io.ysj] = v;
         }
     } else {
 #pragma unroll
@@ -167,63 +188,76 @@ __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int
     }
 }
 
-__global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, double alpha, const double* work,
-                                     double beta, const double* Y0, int ldy0, double* Y, int ldy)
+__global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, int count, double alpha,
+                                     const double* work, double beta, SkinnyIO io)
 {
-    const long long total = (long long)m * nr;
+    const long long per = (long long)m * nr, total = per * count;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % m), c = (int)(e / m);
+        const long long b = e / per, r = e % per;
+        const int i = (int)(r % m), c = (int)(r / m);
+        const double* w = work + b * (long long)slabs * wstride * m;
         double v = 0.0;
-        for (int s = 0; s < slabs; s++) v += work[((long long)s * wstride + c) * m + i];
+        for (int s = 0; s < slabs; s++) v += w[((long long)s * wstride + c) * m + i];
         v *= alpha;
-        if (beta != 0.0) v += beta * Y0[(long long)c * ldy0 + i];
-        Y[(long long)c * ldy + i] = v;
+        if (beta != 0.0) v += beta * io.Y0[b * io.sY0 + (long long)i * io.y0si + (long long)c * io.y0sj];
+        io.Y[b * io.sY + (long long)i * io.ysi + (long long)c * io.ysj] = v;
     }
 }
 
 template <int NR>
 static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
-                         const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y,
-                         int ldy, double* work, hipStream_t s)
+                         const SkinnyIO& io, double beta, double* work, size_t work_elems, int count, hipStream_t s)
 {
-    SkinnyPlan p = ta == Op::N ? plan_n(m, k) : plan_t(m, k);
+    const SkinnyPlan p = ta == Op::N ? plan(cdiv(m, 256), k, KCH, NR, m, count, work_elems)
+                                     : plan(cdiv(m, 4), k, KCT, NR, m, count, work_elems);
     const int direct = p.slabs == 1;
     if (ta == Op::N) {
-        dim3 grid(cdiv(m, 256), p.slabs);
-        hipLaunchKernelGGL((skinny_n_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda,
-                           R, ldr, beta, Y0, ldy0, Y, ldy, work, direct);
+        dim3 grid(cdiv(m, 256), p.slabs, count);
+        hipLaunchKernelGGL((skinny_n_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
+                           beta, work, direct);
     } else {
-        dim3 grid(cdiv(m, 4), p.slabs);
-        hipLaunchKernelGGL((skinny_t_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda,
-                           R, ldr, beta, Y0, ldy0, Y, ldy, work, direct);
+        dim3 grid(cdiv(m, 4), p.slabs, count);
+        hipLaunchKernelGGL((skinny_t_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
+                           beta, work, direct);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
     if (!direct) {
-        long long total = (long long)m * nr;
+        long long total = (long long)m * nr * count;
         int grid = (int)std::min((total + 255) / 256, (long long)4096);
-        hipLaunchKernelGGL(skinny_reduce_kernel, dim3(grid), dim3(256), 0, s, m, nr, NR, p.slabs, alpha, work,
-                           beta, Y0, ldy0, Y, ldy);
+        hipLaunchKernelGGL(skinny_reduce_kernel, dim3(grid), dim3(256), 0, s, m, nr, NR, p.slabs, count, alpha, work,
+                           beta, io);
         e = hipGetLastError();
     }
     return (int)e;
+}
+
+int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda, const double* R,
+                long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
+                double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s, int count,
+                long long sA, long long sR, long long sY0, long long sY)
+{
+    if (m <= 0 || nr <= 0 || count <= 0) return 0;
+    if (nr > MAXR || (size_t)count * MAXR * m > work_elems) return (int)hipErrorInvalidValue;
+    const SkinnyIO io{R, rsk, rsj, Y0, y0si, y0sj, Y, ysi, ysj, sA, sR, sY0, sY};
+    if (k <= 0) {   // Y = beta*Y0
+        if (count == 1 && ysi == 1 && y0si == 1) return copy2d(Y0, (int)y0sj, Y, (int)ysj, m, nr, beta, s);
+        return (int)hipErrorInvalidValue;
+    }
+    if (nr == 1) return launch_skinny<1>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
+    if (nr <= 2) return launch_skinny<2>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
+    if (nr <= 4) return launch_skinny<4>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
+    if (nr <= 8) return launch_skinny<8>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
+    return launch_skinny<16>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
 }
 
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
                double* work, hipStream_t s)
 {
-    if (m <= 0 || nr <= 0) return 0;
-    if (k <= 0) {   // Y = beta*Y0
-        return copy2d(Y0, ldy0, Y, ldy, m, nr, beta, s);
-    }
-    if (nr == 1) return launch_skinny<1>(ta, tri, m, k, nr, alpha, A, lda, R, ldr, beta, Y0, ldy0, Y, ldy, work, s);
-    if (nr <= 2) return launch_skinny<2>(ta, tri, m, k, nr, alpha, A, lda, R, ldr, beta, Y0, ldy0, Y, ldy, work, s);
-    if (nr <= 4) return launch_skinny<4>(ta, tri, m, k, nr, alpha, A, lda, R, ldr, beta, Y0, ldy0, Y, ldy, work, s);
-    if (nr <= 8) return launch_skinny<8>(ta, tri, m, k, nr, alpha, A, lda, R, ldr, beta, Y0, ldy0, Y, ldy, work, s);
-    if (nr <= MAXR) return launch_skinny<16>(ta, tri, m, k, nr, alpha, A, lda, R, ldr, beta, Y0, ldy0, Y, ldy, work, s);
-    return (int)hipErrorInvalidValue;
+    return skinny_f64s(ta, tri, m, k, nr, alpha, A, lda, R, 1, ldr, beta, Y0, 1, ldy0, Y, 1, ldy, work,
+                       skinny_work_elems(m, k, nr), s);
 }
 
 }  // namespace rgf
