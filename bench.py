@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--requests", type=int, default=2000)
     p.add_argument("--rate", type=float, default=20.0, help="requests per second (Poisson)")
     p.add_argument("--slots", type=int, default=8)
+    p.add_argument("--budget-gb", type=float, default=160.0,
+                   help="serving: HBM budget of live handles (the device block cache takes up to 40%% more)")
     p.add_argument("--c3", action="store_true", help="c3: 64 matrices (1500, 60, 5), per-instance theta")
     p.add_argument("--only-batched", choices=["fp64", "mixed"], default=None,
                    help="run only the batched measurement of one precision (for profiling)")
@@ -171,11 +173,25 @@ def run_serve(args):
            "workload": f"c4: {args.requests} requests, Poisson rate {args.rate}/s, ns~U{{300..3000}}, "
                        f"nt~U{{10..120}}, nd~U{{0..8}}, 10% follow-up solves, {args.slots} slots",
            "data": "synthetic (device generator, per-request seed)"}
+    import ctypes as C
+    lib = rgf_amd._lib.load()
+
+    def cache_stats():
+        h, m, f, g = C.c_longlong(), C.c_longlong(), C.c_longlong(), C.c_double()
+        lib.rgf_debug_cache_stats(C.byref(h), C.byref(m), C.byref(f), C.byref(g))
+        return h.value, m.value, f.value, g.value
+
     for name, prec in (("fp64", rgf_amd.RGF_FP64), ("mixed_bf16x3", rgf_amd.RGF_MIXED_BF16X3)):
         reqs = serve.poisson_workload(args.requests, args.rate, seed=args.seed, followup_frac=0.1)
-        srv = serve.Server(serve.GPUExecutor(prec), slots=args.slots, budget_bytes=200 << 30)
+        srv = serve.Server(serve.GPUExecutor(prec), slots=args.slots, budget_bytes=int(args.budget_gb * (1 << 30)),
+                           progress=lambda d, t, nm=name: print(f"serve {nm}: {d}/{t} done", file=sys.stderr,
+                                                                flush=True))
+        c0 = cache_stats()
         stats = srv.run(reqs)
+        c1 = cache_stats()
         out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in stats.items()}
+        out[name]["device_cache"] = {"reuses": c1[0] - c0[0], "hipMalloc": c1[1] - c0[1], "hipFree": c1[2] - c0[2],
+                                     "held_GiB": round(c1[3], 1)}
         print(f"serve {name}: {out[name]}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 

@@ -30,7 +30,8 @@ EXPORTS = [
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats",
 ]
 # include/rgf_testing.h
-TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf"]
+TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
+                "rgf_debug_cache_stats"]
 
 
 class Options(C.Structure):
@@ -111,6 +112,9 @@ def load():
         lib.rgf_debug_bench_gemm.restype = C.c_int
         lib.rgf_debug_bench_potrf.argtypes = [C.c_int, C.c_int, dp, dp]
         lib.rgf_debug_bench_potrf.restype = C.c_int
+        lp = C.POINTER(C.c_longlong)
+        lib.rgf_debug_cache_stats.argtypes = [lp, lp, lp, dp]
+        lib.rgf_debug_cache_stats.restype = C.c_int
         for name in EXPORTS:
             getattr(lib, name).restype = getattr(lib, name).restype if name in (
                 "rgf_last_error", "rgf_default_options", "rgf_default_gen_params") else C.c_int

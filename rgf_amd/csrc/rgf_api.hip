@@ -11,6 +11,7 @@
 // leading dimension, kept resident for the handle's lifetime instead of being
 // streamed to host memory after every block (RGF.H:263, 290, 321, 704-728).
 #include "rgf_internal.hpp"
+#include <cstdlib>
 #include "../../include/rgf.h"
 
 #include <algorithm>
@@ -124,10 +125,28 @@ int dev_status(rgf_handle_s* h, int hip_rc, const char* where)
 // Device memory comes from a per-device block cache: destroying a handle returns its
 // blocks to the cache (after its stream has drained) instead of calling hipFree, and a
 // new handle reuses a cached block of a fitting size before calling hipMalloc.  The
-// cache is bounded (kCacheCap bytes; beyond that the largest blocks are freed), and all
-// access is serialized by one mutex, so concurrent requests of the serving loop can
-// create and destroy handles without device-wide synchronization on every free.
-constexpr size_t kCacheCap = 48ull << 30;
+// cache is bounded (40% of the device's memory, or RGF_DEVICE_CACHE_GB; beyond that the
+// largest blocks are freed, and an allocation that fails empties the cache and retries),
+// and its bookkeeping is serialized by one mutex, so concurrent requests of the serving
+// loop can create and destroy handles without device-wide synchronization on every free
+// (hipFree waits for the whole device; it runs outside the mutex).
+size_t cache_cap()
+{
+    static const size_t cap = [] {
+        const char* v = getenv("RGF_DEVICE_CACHE_GB");
+        if (v) return (size_t)(std::max(0.0, atof(v)) * (double)(1ull << 30));
+        size_t freeb = 0, total = 0;
+        if (hipMemGetInfo(&freeb, &total) != hipSuccess || total == 0) {
+            (void)hipGetLastError();
+            return (size_t)48 << 30;
+        }
+        return total / 10 * 4;
+    }();
+    return cap;
+}
+struct CacheStats {
+    long long mallocs = 0, frees = 0, hits = 0;
+} g_cache_stats;
 constexpr size_t kGranule = 2ull << 20;
 std::mutex g_cache_mu;
 std::multimap<size_t, void*> g_cache[64];
@@ -144,17 +163,24 @@ hipError_t cache_alloc(int dev, size_t bytes, void** out, size_t* got)
             *got = it->first;
             g_cached[dev] -= it->first;
             g_cache[dev].erase(it);
+            g_cache_stats.hits++;
             return hipSuccess;
         }
+        g_cache_stats.mallocs++;
     }
     *got = want;
     hipError_t e = hipMalloc(out, want);
     if (e == hipErrorOutOfMemory && dev >= 0 && dev < 64) {   // give the cache back and retry once
         (void)hipGetLastError();
-        std::lock_guard<std::mutex> lk(g_cache_mu);
-        for (auto& kv : g_cache[dev]) hipFree(kv.second);
-        g_cache[dev].clear();
-        g_cached[dev] = 0;
+        std::vector<void*> drop;
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            for (auto& kv : g_cache[dev]) drop.push_back(kv.second);
+            g_cache_stats.frees += (long long)drop.size();
+            g_cache[dev].clear();
+            g_cached[dev] = 0;
+        }
+        for (void* p : drop) hipFree(p);
         e = hipMalloc(out, want);
     }
     return e;
@@ -165,15 +191,21 @@ void cache_free(int dev, void* p, size_t bytes)
 {
     if (!p) return;
     if (dev < 0 || dev >= 64) { hipFree(p); return; }
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    g_cache[dev].emplace(bytes, p);
-    g_cached[dev] += bytes;
-    while (g_cached[dev] > kCacheCap && !g_cache[dev].empty()) {
-        auto it = std::prev(g_cache[dev].end());
-        hipFree(it->second);
-        g_cached[dev] -= it->first;
-        g_cache[dev].erase(it);
+    const size_t cap = cache_cap();
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache[dev].emplace(bytes, p);
+        g_cached[dev] += bytes;
+        while (g_cached[dev] > cap && !g_cache[dev].empty()) {
+            auto it = std::prev(g_cache[dev].end());
+            g_cache_stats.frees++;
+            drop.push_back(it->second);
+            g_cached[dev] -= it->first;
+            g_cache[dev].erase(it);
+        }
     }
+    for (void* q : drop) hipFree(q);
 }
 
 template <class T>
@@ -1261,6 +1293,20 @@ extern "C" int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int fla
     hipEventDestroy(e1);
     if (rc) return dev_status(nullptr, rc, "bench_gemm");
     *ms_per_launch = ms / reps;
+    return RGF_OK;
+}
+
+extern "C" int rgf_debug_cache_stats(long long* hits, long long* mallocs, long long* frees, double* cached_gb)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if (hits) *hits = g_cache_stats.hits;
+    if (mallocs) *mallocs = g_cache_stats.mallocs;
+    if (frees) *frees = g_cache_stats.frees;
+    if (cached_gb) {
+        size_t t = 0;
+        for (size_t b : g_cached) t += b;
+        *cached_gb = (double)t / (double)(1ull << 30);
+    }
     return RGF_OK;
 }
 

@@ -119,8 +119,12 @@ class GPUExecutor(Executor):
 class Server:
     """Continuous-batching scheduler over `slots` worker threads and an HBM budget."""
 
-    def __init__(self, executor: Executor, slots: int = 8, budget_bytes: int = 200 << 30,
-                 clock: Callable[[], float] = time.perf_counter):
+    def __init__(self, executor: Executor, slots: int = 8, budget_bytes: int = 160 << 30,
+                 clock: Callable[[], float] = time.perf_counter,
+                 progress: Optional[Callable[[int, int], None]] = None, progress_every: int = 100):
+        self.progress = progress              # called as progress(done, total) every progress_every
+        self.progress_every = progress_every
+        self.total = 0
         self.ex = executor
         self.slots = slots
         self.budget = budget_bytes
@@ -201,11 +205,14 @@ class Server:
                     if handle is not None:
                         self.ex.release(handle)
                 self.done.append(req)
+                if self.progress and len(self.done) % self.progress_every == 0:
+                    self.progress(len(self.done), self.total)
                 self.lock.notify_all()
 
     def run(self, requests: list[Request]) -> dict:
         """Feed requests at their arrival times; returns throughput and latency stats."""
         self.t0 = self.clock()
+        self.total = len(requests)
         workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.slots)]
         for w in workers:
             w.start()
