@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--cpu-blocks", type=int, default=16, help="blocks of the CPU baseline sample")
     p.add_argument("--no-kernel-timing", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
-    p.add_argument("--batch", type=int, default=4, help="matrices per batched launch (0/1: skip)")
+    p.add_argument("--batch", type=int, default=8, help="matrices per batched launch (0/1: skip)")
     p.add_argument("--serve", action="store_true", help="c4: Poisson request stream through the serving loop")
     p.add_argument("--requests", type=int, default=2000)
     p.add_argument("--rate", type=float, default=20.0, help="requests per second (Poisson)")
