@@ -145,7 +145,7 @@ def time_batch(rgf_amd, ns, nt, nd, count, precision, seed, steps, warmup):
     """Batched handle of `count` c2-shaped matrices (in place, regenerated before every
     step outside the timed region); returns (s per step, logdets, diag rows)."""
     import numpy as np
-    opts = rgf_amd.default_options(precision=precision, keep_matrix=0)
+    opts = rgf_amd.default_options(precision=precision, keep_matrix=0, graphs=1 if warmup > 0 else 0)
     h = rgf_amd.RGF.generated(ns, nt, nd, seed=seed, count=count, options=opts)
     diag = rgf_amd.DeviceBuffer(h.n * count)
     times = []
@@ -206,7 +206,7 @@ def run_c3(args):
     lds = {}
     diags = {}
     for name, prec in (("fp64", rgf_amd.RGF_FP64), ("mixed_bf16x3", rgf_amd.RGF_MIXED_BF16X3)):
-        opts = rgf_amd.default_options(precision=prec, keep_matrix=0)
+        opts = rgf_amd.default_options(precision=prec, keep_matrix=0, graphs=1 if args.warmup > 0 else 0)
         h = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, count=count, theta_mode=1, options=opts)
         dg = rgf_amd.DeviceBuffer(h.n * count)
         times = []
@@ -258,12 +258,18 @@ def main():
         print(json.dumps({"only_batched": args.only_batched, "count": args.batch, "s_per_matrix": tb / args.batch,
                           "TFLOPs_canonical": args.batch * (F + S) / tb / 1e12}), flush=True)
         return
+    # The timed FP64 steps launch their kernels one by one: the roofline's HIP events
+    # around each launch of the NN class cannot be read back from a graph replay.  The
+    # same steps replayed from HIP graphs are timed separately (graph_replay), and the
+    # mixed and batched measurements replay graphs (captured in their untimed warm-up).
+    graphs = 1 if args.warmup > 0 else 0
+    main_graphs = graphs if (args.no_kernel_timing and not args.profile_step) else 0
     t_gen = time.perf_counter()
     if args.profile_step and args.profile_precision == "mixed":
         s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
                                   options=rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3))
     else:
-        s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed)
+        s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=rgf_amd.default_options(graphs=main_graphs))
     s.synchronize()
     t_gen = time.perf_counter() - t_gen
     diag = rgf_amd.DeviceBuffer(s.n)
@@ -288,11 +294,14 @@ def main():
         s.close()
         return
 
+    # NN class (W = F X, Z = Sb W, trtri merges) timed with HIP events around each launch;
+    # enabled before the warm-up so the captured sweep records the events (a replay
+    # re-records them: the figures are then those of the last timed step)
+    if not args.no_kernel_timing:
+        rgf_amd.kernel_timing_enable(0)
     for _ in range(args.warmup):
         step()
     s.synchronize()
-    if not args.no_kernel_timing:
-        rgf_amd.kernel_timing_enable(0)   # NN class: trmm W = F X, Z = Sb W, trtri merges
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ld = step()
@@ -300,6 +309,9 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     kms, kfl, kn = rgf_amd.kernel_timing_read() if not args.no_kernel_timing else (0.0, 0.0, 0)
     rgf_amd.kernel_timing_enable(-1)
+    if main_graphs:   # the timing change invalidates the captured sweeps: capture them again
+        step()
+        s.synchronize()
 
     # per-phase times (one extra pass, outside the timed region)
     ph = {}
@@ -323,10 +335,27 @@ def main():
     s.close()
     diag.free()
 
+    # the same FP64 steps replayed from HIP graphs
+    graph_replay = None
+    if graphs and not main_graphs:
+        g = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=rgf_amd.default_options(graphs=1))
+        gd = rgf_amd.DeviceBuffer(g.n)
+        for _ in range(args.warmup):
+            g.factorize(); g.logDet(); g.selinv_diag_device(gd)
+        g.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.factorize(); g.logDet(); g.selinv_diag_device(gd)
+        g.synchronize()
+        dtg = (time.perf_counter() - t0) / args.steps
+        graph_replay = {"s_per_matrix": round(dtg, 4), "TFLOPs_canonical": round((F + S) / dtg / 1e12, 4),
+                        "speedup_vs_launches": round(dt / dtg, 3)}
+        g.close(); gd.free()
+
     # mixed precision, same matrix: split-BF16 products, FP64 potrf / logdet / refinement
     mixed = None
     if not args.no_mixed:
-        opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3, refine_tol=1e-13)
+        opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3, refine_tol=1e-13, graphs=graphs)
         m = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=opts)
         md = rgf_amd.DeviceBuffer(m.n)
         for _ in range(args.warmup):
@@ -403,6 +432,7 @@ def main():
                    "canonical_flops": F + S, "precision": "fp64", "selinv": "exact",
                    "logdet": ld, "device_bytes": info.device_bytes, "generate_s": round(t_gen, 3)},
         "phases": ph,
+        "graph_replay": graph_replay,
         "mixed_bf16x3": mixed,
         "batched": batched,
         "roofline": roofline,

@@ -48,6 +48,8 @@ typedef struct {
     int device;         /* HIP device ordinal */
     int refine_iters;   /* max FP64 iterative-refinement steps of a mixed-precision solve */
     double refine_tol;  /* stop refining when ||b - A x|| / ||b|| <= refine_tol */
+    int graphs;         /* 1: run the factorize and selected-inversion sweeps as HIP graphs,
+                           captured on first use and replayed (repeated sweeps of one handle) */
 } rgf_options;
 
 typedef struct {

@@ -52,6 +52,8 @@ struct rgf_handle_s {
     rgf_options opt{};
     int device = 0;
     hipStream_t stream = nullptr;
+    hipGraphExec_t gx_factor = nullptr, gx_selinv = nullptr;   // captured sweeps (opt.graphs)
+    long long gx_epoch_factor = -1, gx_epoch_selinv = -1;
     int ldp = 0;                // panel leading dimension
     long long pstride = 0;      // elements between panels of one matrix
     long long mstride = 0;      // elements between matrices (nt panels)
@@ -464,6 +466,40 @@ int factor_sweep(rgf_handle_s* h)
     return 0;
 }
 
+// Run a sweep on h->stream, through a HIP graph when the handle asks for it: captured
+// and instantiated on the first call (and again after the kernel-timing configuration
+// changed, since the graph records its events), replayed afterwards.  Every device
+// pointer a sweep uses belongs to the handle, so a replay does the same work.
+template <class Fn>
+int run_captured(rgf_handle_s* h, hipGraphExec_t& gx, long long& epoch, Fn&& fn)
+{
+    if (!h->opt.graphs) return fn();
+    const long long ep = timing_epoch();
+    if (gx && epoch != ep) {
+        hipGraphExecDestroy(gx);
+        gx = nullptr;
+    }
+    if (!gx) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return (int)e;
+        const int rc = fn();
+        e = hipStreamEndCapture(h->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : (int)e;
+        }
+        e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            gx = nullptr;
+            return (int)e;
+        }
+        epoch = ep;
+    }
+    return (int)hipGraphLaunch(gx, h->stream);
+}
+
 int do_factorize(rgf_handle_s* h)
 {
     if (!h->Ap && h->factorized) return RGF_OK;   // in place: A is gone, the factor is current
@@ -471,7 +507,7 @@ int do_factorize(rgf_handle_s* h)
     hipSetDevice(h->device);
     int rc;
     if (h->nd > 0 && h->nt > 1 && (rc = ensure_solve_buffers(h))) return rc;   // arrow products use `work`
-    rc = factor_sweep(h);
+    rc = run_captured(h, h->gx_factor, h->gx_epoch_factor, [h] { return factor_sweep(h); });
     if (rc) return dev_status(h, rc, "factorize");
     std::vector<int> info(h->count);
     hipError_t e = hipMemcpyAsync(info.data(), h->dinfo, sizeof(int) * h->count, hipMemcpyDeviceToHost, h->stream);
@@ -698,7 +734,14 @@ int do_selinv_device(rgf_handle_s* h, double* diag_dev)
     int rc = ensure_factorized(h);
     if (rc) return rc;
     if ((rc = ensure_selinv_buffers(h))) return rc;
-    rc = selinv_sweep(h, diag_dev);
+    if (!h->opt.graphs) {
+        rc = selinv_sweep(h, diag_dev);
+    } else {   // the captured sweep writes the handle's buffer
+        rc = run_captured(h, h->gx_selinv, h->gx_epoch_selinv, [h] { return selinv_sweep(h, h->diag); });
+        if (!rc && diag_dev != h->diag)
+            rc = (int)hipMemcpyAsync(diag_dev, h->diag, sizeof(double) * h->n * h->count, hipMemcpyDeviceToDevice,
+                                     h->stream);
+    }
     if (rc) return dev_status(h, rc, "selinv");
     return RGF_OK;
 }
@@ -868,6 +911,7 @@ void rgf_default_options(rgf_options* o)
     o->device = 0;
     o->refine_iters = 10;
     o->refine_tol = 1e-14;
+    o->graphs = 0;
 }
 
 void rgf_default_gen_params(rgf_gen_params* p)
@@ -973,6 +1017,8 @@ int rgf_destroy(rgf_handle_t h)
     if (!h) return RGF_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->gx_factor) hipGraphExecDestroy(h->gx_factor);
+    if (h->gx_selinv) hipGraphExecDestroy(h->gx_selinv);
     for (auto& a : h->allocs) cache_free(h->device, a.first, a.second);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;

@@ -205,6 +205,7 @@ struct TimedLaunch {
 };
 int g_timing_class = -1;
 std::vector<TimedLaunch> g_timed;
+long long g_timing_epoch = 0;   // changes whenever the timed events are destroyed
 
 double canonical_flops(const GemmDesc& d)
 {
@@ -288,6 +289,8 @@ void timing_end(void* token, hipStream_t s)
     delete tl;
 }
 
+long long timing_epoch() { return g_timing_epoch; }
+
 }  // namespace rgf
 
 extern "C" int rgf_kernel_timing_enable(int op_class)
@@ -295,6 +298,7 @@ extern "C" int rgf_kernel_timing_enable(int op_class)
     for (auto& t : rgf::g_timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     rgf::g_timed.clear();
     rgf::g_timing_class = op_class;
+    rgf::g_timing_epoch++;
     return 0;
 }
 

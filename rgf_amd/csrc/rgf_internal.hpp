@@ -74,6 +74,9 @@ int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d, __bf16* scratch, size_t scratch
 // 4..7 the same transposes on split-BF16).
 void* timing_begin(int op_class, const GemmDesc* d, int nd, hipStream_t s);
 void timing_end(void* token, hipStream_t s);
+// Changes whenever kernel timing is (re)configured: a HIP graph captured with the
+// timing events of an earlier epoch must be captured again.
+long long timing_epoch();
 
 // Convenience for a single, non-batched product.
 inline GemmDesc gdesc(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int m,

@@ -32,6 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_defaults():
     o = rgf_amd.default_options()
     assert (o.precision, o.selinv_mode, o.keep_matrix) == (rgf_amd.RGF_FP64, rgf_amd.RGF_SELINV_EXACT, 1)
+    assert (o.refine_iters, o.refine_tol, o.graphs) == (10, 1e-14, 0)
     g = rgf_amd.default_gen_params()
     assert (g.sigma_c, g.sigma_a, g.theta_d, g.theta_c) == (0.1, 0.05, 1.0, 1.0)
 

@@ -455,3 +455,46 @@ def test_mixed_batch_matches_single(gpu):
         with rgf_amd.RGF.generated(150, 5, 4, seed=8, matrix_id=b, theta_d=td, theta_c=tc, options=opts) as h1:
             assert h1.logDet() == lds[b]
             assert np.array_equal(h1.RGFdiag(), diags[b])
+
+
+# ------------------------------------------------------------------ HIP graph replays
+@pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
+def test_graph_replays_match_eager(gpu, prec):
+    """Sweeps captured in HIP graphs (options.graphs = 1): every replay -- with eager work
+    (regeneration, solves, host copies) and synchronizations between the replays, and a
+    re-capture after the kernel-timing configuration changed -- gives bitwise the eager
+    handle's logdet, diag(A^-1) and solution."""
+    say(f"graph replays prec={prec}")
+    ns, nt, nd, count = 150, 5, 4, 2
+    base = dict(precision=prec, keep_matrix=1)
+    eager = rgf_amd.RGF.generated(ns, nt, nd, seed=21, count=count, theta_mode=1,
+                                  options=rgf_amd.default_options(**base))
+    graph = rgf_amd.RGF.generated(ns, nt, nd, seed=21, count=count, theta_mode=1,
+                                  options=rgf_amd.default_options(graphs=1, **base))
+    n = eager.n
+    b = np.random.default_rng(3).standard_normal(n)
+    buf = rgf_amd.DeviceBuffer(n * count)
+    try:
+        for rnd in range(5):
+            if rnd == 2:
+                rgf_amd.kernel_timing_enable(0)    # the next capture records timing events
+            if rnd == 4:
+                rgf_amd.kernel_timing_enable(-1)   # and the one after that drops them again
+            for h in (eager, graph):
+                if rnd == 3:
+                    h.regenerate()
+                h.factorize()
+            assert np.array_equal(graph.logdet_batch(), eager.logdet_batch())
+            assert np.array_equal(graph.selinv_diag_batch(), eager.selinv_diag_batch())
+            graph.selinv_diag_device(buf)   # asynchronous on the handle's stream
+            graph.synchronize()
+            assert np.array_equal(buf.download().reshape(count, n), eager.selinv_diag_batch())
+            for m in range(count):
+                eager.select(m)
+                graph.select(m)
+                assert np.array_equal(graph.solve(None, b, 1), eager.solve(None, b, 1))
+    finally:
+        rgf_amd.kernel_timing_enable(-1)
+        buf.free()
+        eager.close()
+        graph.close()
