@@ -412,8 +412,14 @@ int prod(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d, int nd)
 //   X_i  = L_ii^-1     (in place)
 //   [B_i; D_i] = [A_{i+1,i}; A_di] X_i^T                             (one GEMM launch)
 //   S_dd = A_dd - sum_i D_i D_i^T ; L_dd = chol(S_dd) ; X_dd = L_dd^-1
+int prod_cb(void* ctx, Op ta, Op tb, GemmDesc* d, int nd, hipStream_t)
+{
+    return prod((rgf_handle_s*)ctx, ta, tb, d, nd);
+}
+
 int factor_sweep(rgf_handle_s* h)
 {
+    const ProdFn pf{prod_cb, h};   // trtri merges: split-BF16 products on the mixed path
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, ldf = h->ldf, B = h->count;
     const long long ms = h->mstride, fs = h->fstride;
     hipStream_t s = h->stream;
@@ -446,7 +452,9 @@ int factor_sweep(rgf_handle_s* h)
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, false, s)))
             return rc;
-        if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s))) return rc;
+        if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s,
+                                    h->opt.precision == RGF_MIXED_BF16X3 ? &pf : nullptr)))
+            return rc;
         const bool has_next = i < nt - 1;
         if (has_next && (rc = copy2d(A + ns, ldp, h->Ftmp, ldf, ns, ns, 1.0, s, B, ms, fs))) return rc;
         const int mF = (has_next ? ns : 0) + nd;

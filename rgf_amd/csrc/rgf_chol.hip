@@ -415,8 +415,11 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, lon
 }
 
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
-                      const double* xdiag, long long sX, hipStream_t s)
+                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod)
 {
+    auto run = [&](GemmDesc* d, int nd) {
+        return prod ? prod->fn(prod->ctx, Op::N, Op::N, d, nd, s) : gemm_f64(Op::N, Op::N, d, nd, s);
+    };
     const int T = (n + NB - 1) / NB;
     hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, xdiag, sX);
     hipError_t e = hipGetLastError();
@@ -452,9 +455,9 @@ int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double
             batch2(b, count, sA, sT, sA);
             d1[nd1] = a; d2[nd1] = b; nd1++;
         }
-        int rc = gemm_f64(Op::N, Op::N, d1, nd1, s);
+        int rc = run(d1, nd1);
         if (rc) return rc;
-        rc = gemm_f64(Op::N, Op::N, d2, nd1, s);
+        rc = run(d2, nd1);
         if (rc) return rc;
     }
     return 0;

@@ -103,9 +103,15 @@ inline void batch2(GemmDesc& d, int count, long long sA, long long sB, long long
 // them by their inverses from xdiag and skips the store).
 int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
               int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s);
-// In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.
+// A product launcher supplied by the caller (precision policy of a handle).
+struct ProdFn {
+    int (*fn)(void* ctx, Op ta, Op tb, GemmDesc* d, int nd, hipStream_t s);
+    void* ctx;
+};
+// In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.  The
+// merge products run through `prod` (null: FP64 GEMM).
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
-                      const double* xdiag, long long sX, hipStream_t s);
+                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod = nullptr);
 int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
            hipStream_t s, int count = 1, long long sS = 0, long long sD = 0);
