@@ -51,9 +51,34 @@ __device__ __forceinline__ double rsqrt_nr(double d)
     return fma(y0, e * fma(e, 0.375, 0.5), y0);
 }
 
+// 64 x 64 MFMA product on LDS operands, accumulated into the caller's C quarter:
+// wave (wm, wn) adds sgn * X Y^T to rows 32 wm.., columns 32 wn.. (X, Y row-major
+// 64 x 64 with pitch LP).  acc[a][b] item t is row 16 a + l16, column 16 b + 4 t + lq.
+__device__ __forceinline__ void mfma_nt64(v4d (&acc)[2][2], const double (*X)[LP], const double (*Y)[LP], double sgn,
+                                          int wm, int wn, int l16, int lq)
+{
+#pragma unroll
+    for (int ks = 0; ks < NB / 4; ks++) {
+        double af[2], bf[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) af[a] = sgn * X[32 * wm + 16 * a + l16][4 * ks + lq];
+#pragma unroll
+        for (int b = 0; b < 2; b++) bf[b] = Y[32 * wn + 16 * b + l16][4 * ks + lq];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+    }
+}
+
+// Column block k0 of the blocked Cholesky, one launch: tile factor and inverse, then the
+// panel rows.  With prev_update the update of this column block by the previous panel
+// (columns k0-64..k0-1), which the trailing update left out, is applied first: to the
+// tile (A_kk -= P_k P_k^T) and to each row block (A_Rk -= P_R P_k^T).
 __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, long long sA, int k0, int kb,
-                                                         int below, double* xout, long long sX, double* logdet_slot,
-                                                         long long sP, int* info, int info_base, bool store_l)
+                                                         int below, bool prev_update, double* xout, long long sX,
+                                                         double* logdet_slot, long long sP, int* info, int info_base,
+                                                         bool store_l)
 {
     {
         const long long b = blockIdx.y;   // batch entry
@@ -66,6 +91,8 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
     __shared__ __attribute__((aligned(16))) double Ls[NB][LP];
     __shared__ __attribute__((aligned(16))) double Xs[NB][LP];
     __shared__ __attribute__((aligned(16))) double Ts[2][NB / 2][NB / 2 + 2];
+    __shared__ __attribute__((aligned(16))) double Pk[NB][LP];   // previous panel, rows of this tile
+    __shared__ __attribute__((aligned(16))) double Pr[NB][LP];   // previous panel, rows of a row block
     __shared__ double colb[NB];
     __shared__ double rinv[NB];
     __shared__ double dgl[NB];
@@ -81,14 +108,44 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
             const bool in = r < kb && c < kb;
             v[q] = A[in ? (long long)(k0 + c) * lda + (k0 + r) : (long long)k0 * lda + k0];
         }
+        double w[NB * NB / 256];
+        if (prev_update) {
+            const double* P = A + (long long)(k0 - NB) * lda;
+#pragma unroll
+            for (int q = 0; q < NB * NB / 256; q++) {
+                const int r = lane, c = 4 * q + wave;
+                w[q] = P[r < kb ? (long long)c * lda + (k0 + r) : (long long)k0];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < NB * NB / 256; q++) {
             const int r = lane, c = 4 * q + wave;
             Ls[r][c] = (r < kb && c < kb) ? (c <= r ? v[q] : 0.0) : (r == c ? 1.0 : 0.0);
             Xs[r][c] = 0.0;
+            if (prev_update) Pk[r][c] = r < kb ? w[q] : 0.0;
         }
     }
     __syncthreads();
+    if (prev_update) {   // A_kk -= P_k P_k^T (lower quarters; rows >= kb of P_k are zero)
+        const int wm = wave & 1, wn = wave >> 1;
+        if (wn <= wm) {
+            v4d acc[2][2];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) acc[a][b][t] = Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
+            mfma_nt64(acc, Pk, Pk, -1.0, wm, wn, l16, lq);
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
+        }
+        __syncthreads();
+    }
 
     // ---- factor
 #pragma unroll
@@ -276,37 +333,44 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
         const int r0 = k0 + kb + rb * NB, rows = min(NB, below - rb * NB);
         __syncthreads();   // Ls is free (inverse, stores or the previous block done)
         {
-            double v[NB * NB / 256];
+            double v[NB * NB / 256], w[NB * NB / 256];
 #pragma unroll
             for (int q = 0; q < NB * NB / 256; q++) {
                 const int r = lane, c = 4 * q + wave;
                 const bool in = r < rows && c < kb;
                 v[q] = A[in ? (long long)(k0 + c) * lda + (r0 + r) : (long long)k0 * lda + r0];
+                if (prev_update) w[q] = A[r < rows ? (long long)(k0 - NB + c) * lda + (r0 + r) : (long long)k0 * lda + r0];
             }
 #pragma unroll
             for (int q = 0; q < NB * NB / 256; q++) {
                 const int r = lane, c = 4 * q + wave;
                 Ls[r][c] = (r < rows && c < kb) ? v[q] : 0.0;
+                if (prev_update) Pr[r][c] = r < rows ? w[q] : 0.0;
             }
         }
         __syncthreads();
         v4d acc[2][2];
+        if (prev_update) {   // A_Rk -= P_R P_k^T, in place in Ls (each lane rewrites its own entries)
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) acc[a][b][t] = Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
+            mfma_nt64(acc, Pr, Pk, -1.0, wm, wn, l16, lq);
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
+            __syncthreads();
+        }
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < NB / 4; ks++) {
-            double af[2], bf[2];
-#pragma unroll
-            for (int a = 0; a < 2; a++) af[a] = Ls[32 * wm + 16 * a + l16][4 * ks + lq];
-#pragma unroll
-            for (int b = 0; b < 2; b++) bf[b] = Xs[32 * wn + 16 * b + l16][4 * ks + lq];   // X^T[k][j] = X[j][k]
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
-        }
+        mfma_nt64(acc, Ls, Xs, 1.0, wm, wn, l16, lq);   // R X^T (X^T[k][j] = X[j][k])
 #pragma unroll
         for (int a = 0; a < 2; a++) {
             const int row = 32 * wm + 16 * a + l16;
@@ -321,29 +385,40 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
     }
 }
 
-// Blocked right-looking Cholesky of the n x n block at A (lower): per 64-wide panel
-// one launch of the tile kernel (factor + inverse of the diagonal tile, then the rows
-// below as an in-place product with X_kk^T, spread over the launch's workgroups) and
-// the trailing lower update on MFMA.  xdiag receives the T inverted diagonal tiles.
+// Blocked right-looking Cholesky of the n x n block at A (lower), 64-wide column blocks
+// with the trailing matrix updated every second block by a rank-128 GEMM: column block
+// p is one launch of the tile kernel (for odd p first the update of the block by panel
+// p-1, then factor + inverse of the diagonal tile, then the rows below as an in-place
+// product with X_pp^T spread over the launch's workgroups); after an odd block the
+// columns beyond it are updated by panels p-1 and p together (K = 128: the same traffic
+// on the trailing matrix as one rank-64 update, half the launches).  xdiag receives the
+// T inverted diagonal tiles.
 int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
               int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s)
 {
-    for (int k0 = 0; k0 < n; k0 += NB) {
+    int p = 0;
+    for (int k0 = 0; k0 < n; k0 += NB, p++) {
         const int kb = min(NB, n - k0);
         const int below = n - k0 - kb;
-        double* xt = xdiag + (long long)(k0 / NB) * NB * NB;
+        const bool odd = p & 1;
+        double* xt = xdiag + (long long)p * NB * NB;
         // every workgroup of a matrix factors the tile (in LDS) and then produces its
-        // share of the 64-row blocks of the panel below: no separate panel launch
+        // share of the 64-row blocks of the panel below
         const int nrb = (below + NB - 1) / NB;
         const int nf = std::max(1, std::min(nrb, std::max(1, 256 / count)));
-        hipLaunchKernelGGL(potrf_tile_kernel, dim3(nf, count), dim3(256), 0, s, A, lda, sA, k0, kb, below, xt, sX,
-                           logdet_parts + k0 / NB, sP, info, info_base, store_diag_l);
+        hipLaunchKernelGGL(potrf_tile_kernel, dim3(nf, count), dim3(256), 0, s, A, lda, sA, k0, kb, below, odd, xt,
+                           sX, logdet_parts + p, sP, info, info_base, store_diag_l);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
-        if (below > 0) {
-            double* Pn = A + (long long)k0 * lda + (k0 + kb);
-            double* Cn = A + (long long)(k0 + kb) * lda + (k0 + kb);
-            GemmDesc d = gdesc(Pn, lda, Pn, lda, Cn, lda, below, below, kb, -1.0, 1.0, G_CLOWER);
+        // after an odd block: rank-128 update by panels p-1, p; after an even block the
+        // next (odd) block applies panel p to itself
+        const bool update = below > 0 && odd;
+        if (update) {
+            const int c0 = odd ? k0 - NB : k0, kk = (odd ? NB : 0) + kb;   // panels [c0, c0 + kk)
+            const int t0 = k0 + kb;
+            const double* Pn = A + (long long)c0 * lda + t0;
+            double* Cn = A + (long long)t0 * lda + t0;
+            GemmDesc d = gdesc(Pn, lda, Pn, lda, Cn, lda, n - t0, n - t0, kk, -1.0, 1.0, G_CLOWER);
             batch2(d, count, sA, sA, sA);
             int rc = gemm_f64(Op::N, Op::T, &d, 1, s);
             if (rc) return rc;
@@ -355,8 +430,8 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
 // one launch of the tile kernel on the leading tile (tuning only)
 int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s)
 {
-    hipLaunchKernelGGL(potrf_tile_kernel, dim3(1, 1), dim3(256), 0, s, A, lda, 0LL, 0, NB, 0, xdiag, 0LL, logdet_slot,
-                       0LL, info, 0, false);
+    hipLaunchKernelGGL(potrf_tile_kernel, dim3(1, 1), dim3(256), 0, s, A, lda, 0LL, 0, NB, 0, false, xdiag, 0LL,
+                       logdet_slot, 0LL, info, 0, false);
     return (int)hipGetLastError();
 }
 
