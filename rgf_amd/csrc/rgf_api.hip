@@ -95,6 +95,7 @@ struct rgf_handle_s {
     double* scal = nullptr;     // small device scalars
     __bf16* split = nullptr;    // bf16 hi/lo operand copies of the split-BF16 products
     size_t split_elems = 0;
+    __bf16* split_x = nullptr;  // selected inversion: split copy of X_i kept for three products
     bool factorized = false;
     bool broken = false;        // in-place factorization failed: A is gone
     bool generated = false;
@@ -250,6 +251,7 @@ size_t split_need(int ns, int nd, int count)
 {
     return split_scratch_elems(ns + nd, ns + nd, ns + nd, count);
 }
+size_t split_x_need(int ns, int count) { return (size_t)count * split_entry_elems(ns, ns); }
 
 size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision)
 {
@@ -266,7 +268,7 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
     size_t once = 3 * ((size_t)ns * nt + nd) * 16 + 64 * (size_t)std::max<long long>((long long)ns * nt + nd, 2LL * ns + nd) * 16;
     size_t bytes = (per * count + once) * sizeof(double);
-    if (precision == RGF_MIXED_BF16X3) bytes += split_need(ns, nd, count) * 2;
+    if (precision == RGF_MIXED_BF16X3) bytes += (split_need(ns, nd, count) + split_x_need(ns, count)) * 2;
     return bytes;
 }
 
@@ -337,6 +339,7 @@ int ensure_selinv_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->W, (size_t)h->fstride * B))) return rc;
     if ((rc = dalloc(h, &h->Z, (size_t)h->fstride * B))) return rc;
     if ((rc = dalloc(h, &h->diag, (size_t)h->n * B))) return rc;
+    if (h->opt.precision == RGF_MIXED_BF16X3 && (rc = dalloc(h, &h->split_x, split_x_need(h->ns, B)))) return rc;
     return RGF_OK;
 }
 
@@ -404,6 +407,32 @@ int prod(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d, int nd)
     return nrest ? gemm_bf16x3(ta, tb, rest, nrest, h->stream) : 0;
 }
 
+bool presplit_ok(rgf_handle_s* h, const GemmDesc& d)
+{
+    return h->opt.precision == RGF_MIXED_BF16X3 && d.m >= 128 && d.n >= 128;
+}
+
+// split op(X) of a product's operand (as gemm_bf16x3_split would) into dst
+int split_a(rgf_handle_s* h, Op ta, const GemmDesc& d, __bf16* dst)
+{
+    return split_operand(d.A, d.lda, d.m, d.k, ta == Op::T, d.count, d.sA, d.count2, d.sA2, dst, h->stream);
+}
+int split_b(rgf_handle_s* h, Op tb, const GemmDesc& d, __bf16* dst)
+{
+    return split_operand(d.B, d.ldb, d.n, d.k, tb == Op::N, d.count, d.sB, d.count2, d.sB2, dst, h->stream);
+}
+
+// A product whose two operands have the same split copy (op(A) = op(B)^T, e.g. the
+// Schur update B B^T): split once
+int prod_same(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d)
+{
+    const bool same = d->A == d->B && d->lda == d->ldb && d->m == d->n && (ta == Op::T) == (tb == Op::N) &&
+                      d->sA == d->sB && d->sA2 == d->sB2;
+    if (!same || !presplit_ok(h, *d)) return prod(h, ta, tb, d, 1);
+    int rc = split_a(h, ta, *d, h->split);
+    return rc ? rc : gemm_bf16x3_presplit(ta, tb, d, h->split, h->split, h->stream);
+}
+
 // ------------------------------------------------------------------ factorize
 // FirstStageFactor (RGF.H:233-340) with every block resident, over all matrices
 // of the handle at once (each launch covers the whole batch):
@@ -447,7 +476,7 @@ int factor_sweep(rgf_handle_s* h)
                                  ms, ms, ms, fs);
                 if (rc) return rc;
             }
-            if ((rc = prod(h, Op::N, Op::T, &d, 1))) return rc;
+            if ((rc = prod_same(h, Op::N, Op::T, &d))) return rc;
         }
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, false, s)))
@@ -702,6 +731,11 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out)
         if ((rc = extract_diag(h->Sb[0] + (long long)ns * lds + ns, lds, nd, diag_out + (long long)nt * ns, s, B, ss, n)))
             return rc;
     }
+    // Split-BF16 path: X_i is split once (k-contiguous columns) for W = F X, X^T X as
+    // both operands; W once for Z = Sb W and W^T Z.  Regions: h->split_x holds X, the
+    // second half of h->split holds W, the first half the operand used once.
+    __bf16* const rT = h->split;
+    __bf16* const rW = h->split ? h->split + (size_t)B * split_entry_elems(m, m) : nullptr;
     int cur = 0;
     for (int i = nt - 1; i >= 0; i--) {
         const double* P = l_panel(h, i);
@@ -711,23 +745,46 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out)
         const int r0 = last ? ns : 0;
         double* Sc = h->Sb[cur];
         double* Sn = h->Sb[cur ^ 1];
+        GemmDesc dx = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
+        batch2(dx, B, ms, ms, ss);
+        const bool px = presplit_ok(h, dx);
+        if (px && (rc = split_a(h, Op::T, dx, h->split_x))) return rc;
         if (mF > 0) {
             GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
             batch2(d, B, ms, ms, fs);
-            if ((rc = prod(h, Op::N, Op::N, &d, 1))) return rc;
+            const bool pw = px && presplit_ok(h, d);
+            if (pw) {   // op(B) = X: the split of X^T X's op(A)
+                if ((rc = split_a(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, h->split_x, s)))
+                    return rc;
+            } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
+                return rc;
+            }
             d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
             batch2(d, B, ss, fs, fs);
-            if ((rc = prod(h, Op::N, Op::N, &d, 1))) return rc;
-        }
-        {
-            GemmDesc d = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
-            batch2(d, B, ms, ms, ss);
-            if ((rc = prod(h, Op::T, Op::N, &d, 1))) return rc;
-        }
-        if (mF > 0) {
-            GemmDesc d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 1.0, G_CLOWER);
+            if (pw) {
+                if ((rc = split_b(h, Op::N, d, rW)) || (rc = split_a(h, Op::N, d, rT)) ||
+                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, rW, s)))
+                    return rc;
+            } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
+                return rc;
+            }
+            if (px) {
+                if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &dx, h->split_x, h->split_x, s))) return rc;
+            } else if ((rc = prod(h, Op::T, Op::N, &dx, 1))) {
+                return rc;
+            }
+            d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 1.0, G_CLOWER);
             batch2(d, B, fs, fs, ss);
-            if ((rc = prod(h, Op::T, Op::N, &d, 1))) return rc;
+            if (pw) {   // op(A) = W^T: the split of Z = Sb W's op(B)
+                if ((rc = split_b(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::T, Op::N, &d, rW, rT, s)))
+                    return rc;
+            } else if ((rc = prod(h, Op::T, Op::N, &d, 1))) {
+                return rc;
+            }
+        } else if (px) {
+            if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &dx, h->split_x, h->split_x, s))) return rc;
+        } else if ((rc = prod(h, Op::T, Op::N, &dx, 1))) {
+            return rc;
         }
         if ((rc = selinv_finish(Sn, lds, ss, ns, nd, h->Z, lds, fs, mF - nd, exact && mF > 0,
                                 diag_out + (long long)i * ns, n, B, s)))

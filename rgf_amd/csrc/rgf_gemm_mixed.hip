@@ -278,40 +278,53 @@ namespace rgf {
 
 constexpr int SPLIT_PITCH = 64;
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
 // dst[e][r*ldk + k] = split(op(X)[r][k]) for r < R, k < ldk (0 beyond K); hi at dst,
 // lo at dst + lo_off.  op(X)[r][k] = X[r + k*ld] (kcontig = false) or X[k + r*ld].
-// grid: (ceil(ldk/32), ceil(R/32), entries); 32x32 tiles through LDS for the transpose.
+// grid: (ldk/64, ceil(R/32), entries).  A 32 x 64 tile is read coalesced along its
+// contiguous edge into LDS (all eight loads of a lane in flight: out-of-range lanes read
+// element 0 and keep zero); each lane then converts 4 consecutive k of a row and stores
+// them as 8-byte hi and lo groups.
 __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld, int R, int K, bool kcontig,
                                                          __bf16* dst, int ldk, long long lo_off, int count,
                                                          long long sX1, long long sX2, long long sD)
 {
-    __shared__ double t[32][33];
+    __shared__ double t[32][65];
     const int e = blockIdx.z;
     const long long b1 = e % count, b2 = e / count;
     X += b1 * sX1 + b2 * sX2;
     dst += (long long)e * sD;
-    const int k0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
-    for (int q = ty; q < 32; q += 8) {
-        double v = 0.0;
-        if (kcontig) {   // row r0+q, k k0+tx contiguous in k
-            const int r = r0 + q, k = k0 + tx;
-            if (r < R && k < K) v = X[(long long)r * ld + k];
-            t[q][tx] = v;
-        } else {         // element (r0+tx, k0+q): contiguous in r
-            const int r = r0 + tx, k = k0 + q;
-            if (r < R && k < K) v = X[(long long)k * ld + r];
-            t[tx][q] = v;
-        }
+    const int k0 = blockIdx.x * 64, r0 = blockIdx.y * 32;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int idx = q * 256 + tid;
+        const int r = kcontig ? idx >> 6 : idx & 31, k = kcontig ? idx & 63 : idx >> 5;
+        const int gr = r0 + r, gk = k0 + k;
+        const bool ok = gr < R && gk < K;
+        const long long at = kcontig ? (long long)gr * ld + gk : (long long)gk * ld + gr;
+        const double v = X[ok ? at : 0];
+        t[r][k] = ok ? v : 0.0;
     }
     __syncthreads();
-    for (int q = ty; q < 32; q += 8) {
-        const int r = r0 + q, k = k0 + tx;
-        if (r < R && k < ldk) {
-            __bf16 h, l;
-            split_bf16(t[q][tx], h, l);
-            dst[(long long)r * ldk + k] = h;
-            dst[lo_off + (long long)r * ldk + k] = l;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int g = q * 256 + tid;
+        const int r = g >> 4, kq = (g & 15) * 4;
+        const int gr = r0 + r;
+        if (gr < R) {
+            bf16x4 vh, vl;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                __bf16 h, l;
+                split_bf16(t[r][kq + j], h, l);
+                vh[j] = h;
+                vl[j] = l;
+            }
+            const long long o = (long long)gr * ldk + k0 + kq;
+            *(bf16x4*)(dst + o) = vh;
+            *(bf16x4*)(dst + lo_off + o) = vl;
         }
     }
 }
@@ -461,16 +474,31 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     }
 }
 
-size_t split_scratch_elems(int m, int n, int k, int entries)
+size_t split_entry_elems(int rows, int k)
 {
     const size_t ldk = (size_t)((k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH);
-    const size_t mp = (size_t)((m + 127) / 128 * 128), np = (size_t)((n + 127) / 128 * 128);
-    return (size_t)entries * 2 * (mp + np) * ldk;
+    return 2 * (size_t)((rows + 127) / 128 * 128) * ldk;
 }
 
-// One product on split-BF16 MFMA with operands split first into `scratch` (bf16
-// elements, >= split_scratch_elems of the launch).  One descriptor per call.
-int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d0, __bf16* scratch, size_t scratch_elems, hipStream_t s)
+size_t split_scratch_elems(int m, int n, int k, int entries)
+{
+    return (size_t)entries * (split_entry_elems(m, k) + split_entry_elems(n, k));
+}
+
+int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int count, long long s1, int count2,
+                  long long s2, __bf16* dst, hipStream_t s)
+{
+    if (rows <= 0 || k <= 0 || count <= 0) return 0;
+    if (count2 < 1) count2 = 1;
+    const int ldk = (k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+    const long long rp = (rows + 127) / 128 * 128;
+    dim3 grid(ldk / 64, (rows + 31) / 32, count * count2);
+    hipLaunchKernelGGL(split_bf16_kernel, grid, dim3(256), 0, s, X, ld, rows, k, kcontig, dst, ldk, rp * ldk, count,
+                       s1, s2, 2 * rp * ldk);
+    return (int)hipGetLastError();
+}
+
+int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, const __bf16* Bs, hipStream_t s)
 {
     GemmDesc d = *d0;
     if (d.count2 < 1) d.count2 = 1;
@@ -478,20 +506,6 @@ int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d0, __bf16* scratch, size_t scratc
     const int entries = d.count * d.count2;
     const int ldk = (d.k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
     const long long mp = (d.m + 127) / 128 * 128, np = (d.n + 127) / 128 * 128;
-    const long long sEA = 2 * mp * ldk, sEB = 2 * np * ldk;
-    if ((size_t)entries * (sEA + sEB) > scratch_elems) return (int)hipErrorInvalidValue;
-    __bf16* As = scratch;
-    __bf16* Bs = scratch + (size_t)entries * sEA;
-    if (d.k > 0) {
-        dim3 ga((ldk + 31) / 32, (d.m + 31) / 32, entries), gb((ldk + 31) / 32, (d.n + 31) / 32, entries);
-        // op(A)[i][k]: k contiguous iff TA; op(B)^T[j][k] = op(B)[k][j]: k contiguous iff !TB
-        hipLaunchKernelGGL(split_bf16_kernel, ga, dim3(256), 0, s, d.A, d.lda, d.m, d.k, ta == Op::T, As, ldk,
-                           mp * ldk, d.count, d.sA, d.sA2, sEA);
-        hipLaunchKernelGGL(split_bf16_kernel, gb, dim3(256), 0, s, d.B, d.ldb, d.n, d.k, tb == Op::N, Bs, ldk,
-                           np * ldk, d.count, d.sB, d.sB2, sEB);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return (int)e;
-    }
     long long work128 = (long long)((d.m + 127) / 128) * ((d.n + 127) / 128) * entries;
     const int BT = work128 >= 768 ? 128 : 64;
     GemmArgs args{};
@@ -505,14 +519,32 @@ int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d0, __bf16* scratch, size_t scratc
     dim3 grid(nt, entries);
     void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
     if (BT == 128)
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128>), grid, dim3(256), 0, s, args, As, Bs, sEA, sEB, mp * ldk,
-                           np * ldk, ldk, ldk);
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128>), grid, dim3(256), 0, s, args, As, Bs, 2 * mp * ldk,
+                           2 * np * ldk, mp * ldk, np * ldk, ldk, ldk);
     else
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64>), grid, dim3(256), 0, s, args, As, Bs, sEA, sEB, mp * ldk,
-                           np * ldk, ldk, ldk);
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64>), grid, dim3(256), 0, s, args, As, Bs, 2 * mp * ldk,
+                           2 * np * ldk, mp * ldk, np * ldk, ldk, ldk);
     hipError_t e = hipGetLastError();
     timing_end(tok, s);
     return (int)e;
+}
+
+// One product on split-BF16 MFMA with operands split first into `scratch` (bf16
+// elements, >= split_scratch_elems of the launch).  One descriptor per call.
+int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d0, __bf16* scratch, size_t scratch_elems, hipStream_t s)
+{
+    GemmDesc d = *d0;
+    if (d.count2 < 1) d.count2 = 1;
+    if (d.m <= 0 || d.n <= 0 || d.count <= 0) return 0;
+    const int entries = d.count * d.count2;
+    if (split_scratch_elems(d.m, d.n, d.k, entries) > scratch_elems) return (int)hipErrorInvalidValue;
+    __bf16* As = scratch;
+    __bf16* Bs = scratch + (size_t)entries * split_entry_elems(d.m, d.k);
+    int rc;
+    // op(A)[i][k]: k contiguous iff TA; op(B)^T[j][k] = op(B)[k][j]: k contiguous iff !TB
+    if ((rc = split_operand(d.A, d.lda, d.m, d.k, ta == Op::T, d.count, d.sA, d.count2, d.sA2, As, s))) return rc;
+    if ((rc = split_operand(d.B, d.ldb, d.n, d.k, tb == Op::N, d.count, d.sB, d.count2, d.sB2, Bs, s))) return rc;
+    return gemm_bf16x3_presplit(ta, tb, &d, As, Bs, s);
 }
 
 }  // namespace rgf

@@ -69,6 +69,15 @@ int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
 // bf16 values.  Same result as gemm_bf16x3 to FP32 accumulation order.
 size_t split_scratch_elems(int m, int n, int k, int entries);
 int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d, __bf16* scratch, size_t scratch_elems, hipStream_t s);
+// The two halves of gemm_bf16x3_split, for operands used by several products: split
+// op(X) (rows x k; op(X)[r][k] = X[r + k*ld], or X[k + r*ld] when kcontig) of
+// count x count2 matrices (element strides s1, s2) into dst, split_entry_elems(rows, k)
+// bf16 values per matrix; then a product on split copies: As holds op(A) (m x k, kcontig
+// = TA), Bs holds op(B)^T (n x k, kcontig = !TB), both split with the product's k.
+size_t split_entry_elems(int rows, int k);
+int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int count, long long s1, int count2,
+                  long long s2, __bf16* dst, hipStream_t s);
+int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s);
 
 // Kernel-timing hooks shared by the GEMM launchers (op class: 0..3 FP64 NN/NT/TN/TT,
 // 4..7 the same transposes on split-BF16).
