@@ -25,15 +25,21 @@ int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const doub
  * *logdet and RGF_ENOTSPD for a non-positive pivot. */
 int rgf_debug_potrf(int n, double *A, int lda, int invert, double *logdet);
 
-/* Tuning loops: average device time (ms) of one GEMM launch of the given shape
- * (operands allocated and filled once; flags as rgf_debug_gemm), and of the blocked
- * Cholesky of an n x n block and of its 64 x 64 tile kernel alone. */
 /* Device block cache counters since load: reuses, hipMalloc calls, hipFree calls (cache
  * over its cap) and the bytes held now (GiB). */
 int rgf_debug_cache_stats(long long *hits, long long *mallocs, long long *frees, double *cached_gb);
 
+/* Tuning loops: average device time (ms) of one GEMM launch of the given shape
+ * (operands allocated and filled once; flags as rgf_debug_gemm), and of the blocked
+ * Cholesky of an n x n block and of its 64 x 64 tile kernel alone. */
 int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int flags, int reps, double *ms_per_launch);
 int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
+
+/* Process-wide kernel variant switches, for measurements of whole sweeps: key 1 = k per
+ * LDS stage of the pre-split split-BF16 product (32 or 64).  Returns the previous value,
+ * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
+ * running handles. */
+int rgf_debug_set_tuning(int key, int value);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ EXPORTS = [
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
-                "rgf_debug_cache_stats"]
+                "rgf_debug_cache_stats", "rgf_debug_set_tuning"]
 
 
 class Options(C.Structure):
@@ -116,6 +116,8 @@ def load():
         lp = C.POINTER(C.c_longlong)
         lib.rgf_debug_cache_stats.argtypes = [lp, lp, lp, dp]
         lib.rgf_debug_cache_stats.restype = C.c_int
+        lib.rgf_debug_set_tuning.argtypes = [C.c_int, C.c_int]
+        lib.rgf_debug_set_tuning.restype = C.c_int
         for name in EXPORTS:
             getattr(lib, name).restype = getattr(lib, name).restype if name in (
                 "rgf_last_error", "rgf_default_options", "rgf_default_gen_params") else C.c_int

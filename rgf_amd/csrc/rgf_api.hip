@@ -1407,6 +1407,16 @@ extern "C" int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int fla
     return RGF_OK;
 }
 
+extern "C" int rgf_debug_set_tuning(int key, int value)
+{
+    if (key == 1 && (value == 32 || value == 64)) {
+        const int old = g_presplit_kb;
+        g_presplit_kb = value;
+        return old;
+    }
+    return -1;
+}
+
 extern "C" int rgf_debug_cache_stats(long long* hits, long long* mallocs, long long* frees, double* cached_gb)
 {
     std::lock_guard<std::mutex> lk(g_cache_mu);

@@ -329,7 +329,8 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
     }
 }
 
-template <int BT>
+// KB: k per LDS stage (32 or 64; the split copies are padded to a multiple of 64)
+template <int BT, int KB = 32>
 __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
                                                                               const __bf16* Bs, long long sEA,
                                                                               long long sEB, long long loA,
@@ -337,11 +338,13 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 {
     constexpr int WT = BT / 2;
     constexpr int NF = WT / 16;
-    constexpr int CH = BT * MBK / 8 / 256;   // 16-byte chunks per thread per array per k-step
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[BT][LROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[BT][LROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[BT][LROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[BT][LROW];
+    constexpr int CH = BT * KB / 8 / 256;    // 16-byte chunks per thread per array per k-step
+    constexpr int KR = KB + KPAD;            // LDS row (bf16)
+    constexpr int CPR = KB / 8;              // 16-byte chunks per row
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[BT][KR];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[BT][KR];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[BT][KR];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[BT][KR];
 
     const int z = blockIdx.y;
     GemmDesc D;
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     if (flags & G_AUPPER) klo = max(klo, i0);
     if (flags & G_BLOWER) klo = max(klo, j0);
     if (flags & G_BUPPER) khi = min(khi, j0 + BT);
-    klo = (klo / MBK) * MBK;
+    klo = (klo / KB) * KB;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -396,12 +399,12 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 #pragma unroll
         for (int b = 0; b < NF; b++) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-    // chunk c (0..BT*4-1) of a [BT][32] tile: row c/4, k offset 8*(c%4)
+    // chunk c (0..BT*CPR-1) of a [BT][KB] tile: row c/CPR, k offset 8*(c%CPR)
     bf16x8 rah[CH], ral[CH], rbh[CH], rbl[CH];
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < CH; q++) {
-            const int c = tid + 256 * q, r = c >> 2, kk = 8 * (c & 3);
+            const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
             const int ga = i0 + r, gb = j0 + r;
             const long long oa = (long long)ga * ldka + k0 + kk, ob = (long long)gb * ldkb + k0 + kk;
             const bf16x8 zero = {};
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     auto store = [&]() {
 #pragma unroll
         for (int q = 0; q < CH; q++) {
-            const int c = tid + 256 * q, r = c >> 2, kk = 8 * (c & 3);
+            const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
             *(bf16x8*)&Ah[r][kk] = rah[q];
             *(bf16x8*)&Al[r][kk] = ral[q];
             *(bf16x8*)&Bh[r][kk] = rbh[q];
@@ -424,33 +427,36 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     // the split copies are zero beyond K up to their 64-aligned pitch, so a k-step may
     // read past khi (the values there are zero or belong to skipped triangles: both exact)
     if (klo < khi) load(klo);
-    for (int k0 = klo; k0 < khi; k0 += MBK) {
+    for (int k0 = klo; k0 < khi; k0 += KB) {
         store();
         __syncthreads();
-        if (k0 + MBK < khi) load(k0 + MBK);
-        bf16x8 ah[NF], al[NF], bh[NF], bl[NF];
+        if (k0 + KB < khi) load(k0 + KB);
 #pragma unroll
-        for (int a = 0; a < NF; a++) {
-            const int r = wm * WT + a * 16 + l16;
-            ah[a] = *(const bf16x8*)&Ah[r][8 * lq];
-            al[a] = *(const bf16x8*)&Al[r][8 * lq];
-        }
+        for (int ks = 0; ks < KB / 32; ks++) {
+            bf16x8 ah[NF], al[NF], bh[NF], bl[NF];
 #pragma unroll
-        for (int b = 0; b < NF; b++) {
-            const int c = wn * WT + b * 16 + l16;
-            bh[b] = *(const bf16x8*)&Bh[c][8 * lq];
-            bl[b] = *(const bf16x8*)&Bl[c][8 * lq];
-        }
-#pragma unroll
-        for (int a = 0; a < NF; a++)
+            for (int a = 0; a < NF; a++) {
+                const int r = wm * WT + a * 16 + l16;
+                ah[a] = *(const bf16x8*)&Ah[r][32 * ks + 8 * lq];
+                al[a] = *(const bf16x8*)&Al[r][32 * ks + 8 * lq];
+            }
 #pragma unroll
             for (int b = 0; b < NF; b++) {
-                v4f t = acc[a][b];
-                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[b], ah[a], t, 0, 0, 0);
-                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], al[a], t, 0, 0, 0);
-                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], ah[a], t, 0, 0, 0);
-                acc[a][b] = t;
+                const int c = wn * WT + b * 16 + l16;
+                bh[b] = *(const bf16x8*)&Bh[c][32 * ks + 8 * lq];
+                bl[b] = *(const bf16x8*)&Bl[c][32 * ks + 8 * lq];
             }
+#pragma unroll
+            for (int a = 0; a < NF; a++)
+#pragma unroll
+                for (int b = 0; b < NF; b++) {
+                    v4f t = acc[a][b];
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[b], ah[a], t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], al[a], t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], ah[a], t, 0, 0, 0);
+                    acc[a][b] = t;
+                }
+        }
         __syncthreads();
     }
 
@@ -473,6 +479,12 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
         }
     }
 }
+
+// k per LDS stage of the pre-split product: 64 (half the barriers per MFMA of 32, 72 KB
+// of LDS for 128-wide tiles, still two workgroups per CU) measured 5-6 % faster over the
+// whole mixed sweep at ns = 2865 (single matrix 0.314 -> 0.295 s, batch of 8 0.133 ->
+// 0.126 s per matrix), same bits.  32 stays selectable for measurements.
+int g_presplit_kb = 64;
 
 size_t split_entry_elems(int rows, int k)
 {
@@ -518,12 +530,16 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     args.nd = 1;
     dim3 grid(nt, entries);
     void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
-    if (BT == 128)
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128>), grid, dim3(256), 0, s, args, As, Bs, 2 * mp * ldk,
-                           2 * np * ldk, mp * ldk, np * ldk, ldk, ldk);
+    const long long a1 = 2 * mp * ldk, b1 = 2 * np * ldk, a2 = mp * ldk, b2 = np * ldk;
+    const bool k64 = g_presplit_kb == 64;
+    if (BT == 128 && k64)
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128, 64>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
+    else if (BT == 128)
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
+    else if (k64)
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64, 64>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
     else
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64>), grid, dim3(256), 0, s, args, As, Bs, 2 * mp * ldk,
-                           2 * np * ldk, mp * ldk, np * ldk, ldk, ldk);
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
     hipError_t e = hipGetLastError();
     timing_end(tok, s);
     return (int)e;

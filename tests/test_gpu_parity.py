@@ -397,11 +397,22 @@ def test_gemm_bf16x3_kernel(gpu, ta, tb, m, n, k, kind):
 @pytest.mark.parametrize("ta,tb,flags", [(0, 1, 1), (0, 0, 8), (0, 1, 16), (1, 0, 1), (1, 0, 1 | 4 | 8),
                                          (0, 0, 2), (1, 1, 0)])
 @pytest.mark.parametrize("m,n,k", [(200, 200, 200), (333, 333, 333), (70, 70, 70)])
-def test_gemm_bf16x3_presplit_matches_staged(gpu, ta, tb, flags, m, n, k):
+@pytest.mark.parametrize("kb", [32, 64])
+def test_gemm_bf16x3_presplit_matches_staged(gpu, ta, tb, flags, m, n, k, kb):
     """The pre-split product runs the same MFMA sequence on the same hi/lo values as the
     staged one, over the same k ranges (triangle flags included), so both give the
-    same bits."""
-    say(f"gemm bf16x3 presplit vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k}")
+    same bits -- with 32 or 64 k per LDS stage of the pre-split kernel."""
+    say(f"gemm bf16x3 presplit vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k} kb={kb}")
+    lib = _lib.load()
+    old = lib.rgf_debug_set_tuning(1, kb)
+    assert old in (32, 64)
+    try:
+        _presplit_vs_staged(ta, tb, flags, m, n, k)
+    finally:
+        lib.rgf_debug_set_tuning(1, old)
+
+
+def _presplit_vs_staged(ta, tb, flags, m, n, k):
     rng = np.random.default_rng(flags + m)
     lda, ldb, ldc = (k if ta else m), (n if tb else k), m + 1
     A = rng.standard_normal(lda * (m if ta else k))
