@@ -36,7 +36,8 @@ int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int flags, int rep
 int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
 
 /* Process-wide kernel variant switches, for measurements of whole sweeps: key 1 = k per
- * LDS stage of the pre-split split-BF16 product (32 or 64).  Returns the previous value,
+ * LDS stage of the pre-split split-BF16 product (32 or 64); key 3 = its LDS row layout
+ * (0 padded rows, 1 unpadded rows with XOR-swizzled 16-byte chunks).  Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */
 int rgf_debug_set_tuning(int key, int value);

@@ -1414,6 +1414,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         g_presplit_kb = value;
         return old;
     }
+    if (key == 3 && (value == 0 || value == 1)) {
+        const int old = g_presplit_swizzle;
+        g_presplit_swizzle = value;
+        return old;
+    }
     return -1;
 }
 

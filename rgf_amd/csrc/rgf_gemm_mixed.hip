@@ -329,8 +329,12 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
     }
 }
 
-// KB: k per LDS stage (32 or 64; the split copies are padded to a multiple of 64)
-template <int BT, int KB = 32>
+// KB: k per LDS stage (32 or 64; the split copies are padded to a multiple of 64).
+// SW: LDS rows unpadded with the 16-byte chunk index XOR-swizzled by the row
+// (chunk ^ (row / (16 / CPR)) mod CPR): any 16 lanes that store 16 chunks of 16 / CPR
+// consecutive rows, or read one chunk of 16 consecutive rows, touch 16 distinct
+// 16-byte bank groups.  Without SW rows are padded by 16 bytes.
+template <int BT, int KB = 32, bool SW = false>
 __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
                                                                               const __bf16* Bs, long long sEA,
                                                                               long long sEB, long long loA,
@@ -339,8 +343,9 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     constexpr int WT = BT / 2;
     constexpr int NF = WT / 16;
     constexpr int CH = BT * KB / 8 / 256;    // 16-byte chunks per thread per array per k-step
-    constexpr int KR = KB + KPAD;            // LDS row (bf16)
+    constexpr int KR = SW ? KB : KB + KPAD;  // LDS row (bf16)
     constexpr int CPR = KB / 8;              // 16-byte chunks per row
+    constexpr int RPG = 16 / CPR;            // rows sharing one swizzle value
     __shared__ __attribute__((aligned(16))) __bf16 Ah[BT][KR];
     __shared__ __attribute__((aligned(16))) __bf16 Al[BT][KR];
     __shared__ __attribute__((aligned(16))) __bf16 Bh[BT][KR];
@@ -401,6 +406,8 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 
     // chunk c (0..BT*CPR-1) of a [BT][KB] tile: row c/CPR, k offset 8*(c%CPR)
     bf16x8 rah[CH], ral[CH], rbh[CH], rbl[CH];
+    // bf16 offset in an LDS row of logical 16-byte chunk j of row r
+    auto lcol = [](int r, int j) { return 8 * (SW ? (j ^ ((r / RPG) % CPR)) : j); };
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < CH; q++) {
@@ -418,10 +425,11 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 #pragma unroll
         for (int q = 0; q < CH; q++) {
             const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
-            *(bf16x8*)&Ah[r][kk] = rah[q];
-            *(bf16x8*)&Al[r][kk] = ral[q];
-            *(bf16x8*)&Bh[r][kk] = rbh[q];
-            *(bf16x8*)&Bl[r][kk] = rbl[q];
+            const int col = lcol(r, kk / 8);
+            *(bf16x8*)&Ah[r][col] = rah[q];
+            *(bf16x8*)&Al[r][col] = ral[q];
+            *(bf16x8*)&Bh[r][col] = rbh[q];
+            *(bf16x8*)&Bl[r][col] = rbl[q];
         }
     };
     // the split copies are zero beyond K up to their 64-aligned pitch, so a k-step may
@@ -437,14 +445,14 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 #pragma unroll
             for (int a = 0; a < NF; a++) {
                 const int r = wm * WT + a * 16 + l16;
-                ah[a] = *(const bf16x8*)&Ah[r][32 * ks + 8 * lq];
-                al[a] = *(const bf16x8*)&Al[r][32 * ks + 8 * lq];
+                ah[a] = *(const bf16x8*)&Ah[r][lcol(r, 4 * ks + lq)];
+                al[a] = *(const bf16x8*)&Al[r][lcol(r, 4 * ks + lq)];
             }
 #pragma unroll
             for (int b = 0; b < NF; b++) {
                 const int c = wn * WT + b * 16 + l16;
-                bh[b] = *(const bf16x8*)&Bh[c][32 * ks + 8 * lq];
-                bl[b] = *(const bf16x8*)&Bl[c][32 * ks + 8 * lq];
+                bh[b] = *(const bf16x8*)&Bh[c][lcol(c, 4 * ks + lq)];
+                bl[b] = *(const bf16x8*)&Bl[c][lcol(c, 4 * ks + lq)];
             }
 #pragma unroll
             for (int a = 0; a < NF; a++)
@@ -485,6 +493,10 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 // whole mixed sweep at ns = 2865 (single matrix 0.314 -> 0.295 s, batch of 8 0.133 ->
 // 0.126 s per matrix), same bits.  32 stays selectable for measurements.
 int g_presplit_kb = 64;
+// unpadded XOR-swizzled LDS rows: 2 % faster over the mixed sweep at ns = 2865 (single
+// 0.293 -> 0.287 s, batch of 8 0.1235 -> 0.1219 s per matrix), same bits; padded rows
+// stay selectable for measurements
+int g_presplit_swizzle = 1;
 
 size_t split_entry_elems(int rows, int k)
 {
@@ -531,15 +543,17 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     dim3 grid(nt, entries);
     void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
     const long long a1 = 2 * mp * ldk, b1 = 2 * np * ldk, a2 = mp * ldk, b2 = np * ldk;
-    const bool k64 = g_presplit_kb == 64;
-    if (BT == 128 && k64)
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128, 64>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
-    else if (BT == 128)
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<128>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
-    else if (k64)
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64, 64>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
-    else
-        hipLaunchKernelGGL((gemm_bf16x3s_kernel<64>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk);
+    const bool k64 = g_presplit_kb == 64, sw = g_presplit_swizzle != 0;
+#define RGF_PRESPLIT(bt, kb, swz) \
+    hipLaunchKernelGGL((gemm_bf16x3s_kernel<bt, kb, swz>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk)
+    if (BT == 128) {
+        if (k64) { if (sw) RGF_PRESPLIT(128, 64, true); else RGF_PRESPLIT(128, 64, false); }
+        else     { if (sw) RGF_PRESPLIT(128, 32, true); else RGF_PRESPLIT(128, 32, false); }
+    } else {
+        if (k64) { if (sw) RGF_PRESPLIT(64, 64, true); else RGF_PRESPLIT(64, 64, false); }
+        else     { if (sw) RGF_PRESPLIT(64, 32, true); else RGF_PRESPLIT(64, 32, false); }
+    }
+#undef RGF_PRESPLIT
     hipError_t e = hipGetLastError();
     timing_end(tok, s);
     return (int)e;

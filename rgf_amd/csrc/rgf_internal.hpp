@@ -79,6 +79,7 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
                   long long s2, __bf16* dst, hipStream_t s);
 int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s);
 extern int g_presplit_kb;   // k per LDS stage of gemm_bf16x3_presplit (32 or 64)
+extern int g_presplit_swizzle;   // 1: its LDS rows unpadded with XOR-swizzled chunks
 
 // Kernel-timing hooks shared by the GEMM launchers (op class: 0..3 FP64 NN/NT/TN/TT,
 // 4..7 the same transposes on split-BF16).

@@ -398,18 +398,22 @@ def test_gemm_bf16x3_kernel(gpu, ta, tb, m, n, k, kind):
                                          (0, 0, 2), (1, 1, 0)])
 @pytest.mark.parametrize("m,n,k", [(200, 200, 200), (333, 333, 333), (70, 70, 70)])
 @pytest.mark.parametrize("kb", [32, 64])
-def test_gemm_bf16x3_presplit_matches_staged(gpu, ta, tb, flags, m, n, k, kb):
+@pytest.mark.parametrize("swz", [0, 1])
+def test_gemm_bf16x3_presplit_matches_staged(gpu, ta, tb, flags, m, n, k, kb, swz):
     """The pre-split product runs the same MFMA sequence on the same hi/lo values as the
     staged one, over the same k ranges (triangle flags included), so both give the
-    same bits -- with 32 or 64 k per LDS stage of the pre-split kernel."""
-    say(f"gemm bf16x3 presplit vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k} kb={kb}")
+    same bits -- with 32 or 64 k per LDS stage of the pre-split kernel, padded or
+    XOR-swizzled LDS rows."""
+    say(f"gemm bf16x3 presplit vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k} kb={kb} swz={swz}")
     lib = _lib.load()
     old = lib.rgf_debug_set_tuning(1, kb)
-    assert old in (32, 64)
+    old_sw = lib.rgf_debug_set_tuning(3, swz)
+    assert old in (32, 64) and old_sw in (0, 1)
     try:
         _presplit_vs_staged(ta, tb, flags, m, n, k)
     finally:
         lib.rgf_debug_set_tuning(1, old)
+        lib.rgf_debug_set_tuning(3, old_sw)
 
 
 def _presplit_vs_staged(ta, tb, flags, m, n, k):
