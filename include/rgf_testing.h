@@ -25,8 +25,9 @@ int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const doub
  * *logdet and RGF_ENOTSPD for a non-positive pivot. */
 int rgf_debug_potrf(int n, double *A, int lda, int invert, double *logdet);
 
-/* Device block cache counters since load: reuses, hipMalloc calls, hipFree calls (cache
- * over its cap) and the bytes held now (GiB). */
+/* Device arena counters since load: allocations served from existing slabs, hipMalloc
+ * calls (slabs added), hipFree calls (wholly free slabs released over the cap) and the
+ * free bytes held now (GiB). */
 int rgf_debug_cache_stats(long long *hits, long long *mallocs, long long *frees, double *cached_gb);
 
 /* Tuning loops: average device time (ms) of one GEMM launch of the given shape

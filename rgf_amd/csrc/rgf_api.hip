@@ -11,6 +11,7 @@
 // leading dimension, kept resident for the handle's lifetime instead of being
 // streamed to host memory after every block (RGF.H:263, 290, 321, 704-728).
 #include "rgf_internal.hpp"
+#include "rgf_arena.hpp"
 #include <cstdlib>
 #include "../../include/rgf.h"
 
@@ -125,14 +126,12 @@ int dev_status(rgf_handle_s* h, int hip_rc, const char* where)
                    where, hip_rc, hipGetErrorString((hipError_t)hip_rc));
 }
 
-// Device memory comes from a per-device block cache: destroying a handle returns its
-// blocks to the cache (after its stream has drained) instead of calling hipFree, and a
-// new handle reuses a cached block of a fitting size before calling hipMalloc.  The
-// cache is bounded (40% of the device's memory, or RGF_DEVICE_CACHE_GB; beyond that the
-// largest blocks are freed, and an allocation that fails empties the cache and retries),
-// and its bookkeeping is serialized by one mutex, so concurrent requests of the serving
-// loop can create and destroy handles without device-wide synchronization on every free
-// (hipFree waits for the whole device; it runs outside the mutex).
+// Device memory comes from a per-device arena (rgf_arena.hpp): slabs of at least 1 GiB
+// from hipMalloc, sub-allocated best-fit and coalesced on free, so creating and
+// destroying handles calls neither hipMalloc nor hipFree once the arena has grown to the
+// working set -- hipFree waits for the whole device, which would stall the other requests
+// of the serving loop.  Free bytes are kept up to a cap (40% of the device's memory, or
+// RGF_DEVICE_CACHE_GB); beyond it wholly free slabs are released.
 size_t cache_cap()
 {
     static const size_t cap = [] {
@@ -147,68 +146,33 @@ size_t cache_cap()
     }();
     return cap;
 }
-struct CacheStats {
-    long long mallocs = 0, frees = 0, hits = 0;
-} g_cache_stats;
-constexpr size_t kGranule = 2ull << 20;
-std::mutex g_cache_mu;
-std::multimap<size_t, void*> g_cache[64];
-size_t g_cached[64] = {};
+struct HipSlabs {
+    static int grow(void** p, size_t n)
+    {
+        const hipError_t e = hipMalloc(p, n);
+        if (e != hipSuccess) (void)hipGetLastError();
+        return (int)e;
+    }
+    static bool oom(int rc) { return rc == (int)hipErrorOutOfMemory; }
+    static void release(void* p) { hipFree(p); }
+};
+Arena<HipSlabs> g_arena[64];
 
 hipError_t cache_alloc(int dev, size_t bytes, void** out, size_t* got)
 {
-    const size_t want = (std::max<size_t>(bytes, 1) + kGranule - 1) / kGranule * kGranule;
-    if (dev >= 0 && dev < 64) {
-        std::lock_guard<std::mutex> lk(g_cache_mu);
-        auto it = g_cache[dev].lower_bound(want);
-        if (it != g_cache[dev].end() && it->first <= want + want / 4) {
-            *out = it->second;
-            *got = it->first;
-            g_cached[dev] -= it->first;
-            g_cache[dev].erase(it);
-            g_cache_stats.hits++;
-            return hipSuccess;
-        }
-        g_cache_stats.mallocs++;
+    if (dev < 0 || dev >= 64) {
+        *got = bytes;
+        return hipMalloc(out, bytes);
     }
-    *got = want;
-    hipError_t e = hipMalloc(out, want);
-    if (e == hipErrorOutOfMemory && dev >= 0 && dev < 64) {   // give the cache back and retry once
-        (void)hipGetLastError();
-        std::vector<void*> drop;
-        {
-            std::lock_guard<std::mutex> lk(g_cache_mu);
-            for (auto& kv : g_cache[dev]) drop.push_back(kv.second);
-            g_cache_stats.frees += (long long)drop.size();
-            g_cache[dev].clear();
-            g_cached[dev] = 0;
-        }
-        for (void* p : drop) hipFree(p);
-        e = hipMalloc(out, want);
-    }
-    return e;
+    return (hipError_t)g_arena[dev].alloc(bytes, out, got);
 }
 
 // p must no longer be in use by any stream.
 void cache_free(int dev, void* p, size_t bytes)
 {
+    (void)bytes;
     if (!p) return;
-    if (dev < 0 || dev >= 64) { hipFree(p); return; }
-    const size_t cap = cache_cap();
-    std::vector<void*> drop;
-    {
-        std::lock_guard<std::mutex> lk(g_cache_mu);
-        g_cache[dev].emplace(bytes, p);
-        g_cached[dev] += bytes;
-        while (g_cached[dev] > cap && !g_cache[dev].empty()) {
-            auto it = std::prev(g_cache[dev].end());
-            g_cache_stats.frees++;
-            drop.push_back(it->second);
-            g_cached[dev] -= it->first;
-            g_cache[dev].erase(it);
-        }
-    }
-    for (void* q : drop) hipFree(q);
+    if (dev < 0 || dev >= 64 || !g_arena[dev].free(p, cache_cap())) hipFree(p);
 }
 
 template <class T>
@@ -1424,15 +1388,19 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
 
 extern "C" int rgf_debug_cache_stats(long long* hits, long long* mallocs, long long* frees, double* cached_gb)
 {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    if (hits) *hits = g_cache_stats.hits;
-    if (mallocs) *mallocs = g_cache_stats.mallocs;
-    if (frees) *frees = g_cache_stats.frees;
-    if (cached_gb) {
-        size_t t = 0;
-        for (size_t b : g_cached) t += b;
-        *cached_gb = (double)t / (double)(1ull << 30);
+    long long h = 0, m = 0, f = 0;
+    size_t t = 0;
+    for (const auto& A : g_arena) {
+        const auto st = A.stats();
+        h += st.hits;
+        m += st.grows;
+        f += st.releases;
+        t += A.free_bytes();
     }
+    if (hits) *hits = h;
+    if (mallocs) *mallocs = m;
+    if (frees) *frees = f;
+    if (cached_gb) *cached_gb = (double)t / (double)(1ull << 30);
     return RGF_OK;
 }
 
