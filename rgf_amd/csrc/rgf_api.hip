@@ -55,6 +55,14 @@ struct rgf_handle_s {
     hipStream_t stream = nullptr;
     hipGraphExec_t gx_factor = nullptr, gx_selinv = nullptr;   // captured sweeps (opt.graphs)
     long long gx_epoch_factor = -1, gx_epoch_selinv = -1;
+    struct PieceGraph {   // a captured piece of a solve, keyed by what it reads and writes
+        int kind, mb, nr;
+        const void* x;
+        const void* b;
+        hipGraphExec_t gx;
+        long long epoch;
+    };
+    std::vector<PieceGraph> gx_pieces;   // most recently used last, at most kMaxPieceGraphs
     int ldp = 0;                // panel leading dimension
     long long pstride = 0;      // elements between panels of one matrix
     long long mstride = 0;      // elements between matrices (nt panels)
@@ -501,6 +509,30 @@ int run_captured(rgf_handle_s* h, hipGraphExec_t& gx, long long& epoch, Fn&& fn)
     return (int)hipGraphLaunch(gx, h->stream);
 }
 
+// The same for a piece of a solve, keyed by (kind, matrix, RHS count, the buffers it
+// reads and writes): repeated solves on the same buffers replay their graph.
+constexpr size_t kMaxPieceGraphs = 8;
+template <class Fn>
+int run_piece(rgf_handle_s* h, int kind, int mb, int nr, const void* x, const void* b, Fn&& fn)
+{
+    if (!h->opt.graphs) return fn();
+    auto& v = h->gx_pieces;
+    size_t i = 0;
+    while (i < v.size() && !(v[i].kind == kind && v[i].mb == mb && v[i].nr == nr && v[i].x == x && v[i].b == b)) i++;
+    if (i == v.size()) {
+        if (v.size() >= kMaxPieceGraphs) {   // drop the least recently used
+            if (v.front().gx) hipGraphExecDestroy(v.front().gx);
+            v.erase(v.begin());
+        }
+        v.push_back({kind, mb, nr, x, b, nullptr, -1});
+        i = v.size() - 1;
+    } else if (i + 1 != v.size()) {
+        std::rotate(v.begin() + i, v.begin() + i + 1, v.end());
+        i = v.size() - 1;
+    }
+    return run_captured(h, v[i].gx, v[i].epoch, std::forward<Fn>(fn));
+}
+
 int do_factorize(rgf_handle_s* h)
 {
     if (!h->Ap && h->factorized) return RGF_OK;   // in place: A is gone, the factor is current
@@ -618,15 +650,20 @@ int solve_refined(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
 {
     const long long n = h->n;
     hipStream_t s = h->stream;
-    int rc = solve_block(h, mb, x, b, nr);
+    int rc = run_piece(h, 0, mb, nr, x, b, [&] { return solve_block(h, mb, x, b, nr); });
     if (rc) return rc;
     h->last_refine_iters = 0;
     h->last_refine_res = 0.0;
     if (!h->Ap || h->opt.refine_iters <= 0) return 0;
     hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, b, n, h->scal + 32);
     for (int it = 0; it <= h->opt.refine_iters; it++) {
-        if ((rc = block_matvec_residual(h, mb, x, b, h->R, nr))) return rc;
-        hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, h->R, n, h->scal + 16);
+        rc = run_piece(h, 1, mb, nr, x, b, [&] {
+            const int r = block_matvec_residual(h, mb, x, b, h->R, nr);
+            if (r) return r;
+            hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, h->R, n, h->scal + 16);
+            return (int)hipGetLastError();
+        });
+        if (rc) return rc;
         double v[32];
         hipError_t e = hipMemcpyAsync(v, h->scal + 16, 32 * sizeof(double), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -635,10 +672,15 @@ int solve_refined(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
         for (int c = 0; c < nr; c++) worst = std::max(worst, v[16 + c] > 0 ? v[c] / v[16 + c] : v[c]);
         h->last_refine_res = worst;
         if (worst <= h->opt.refine_tol || it == h->opt.refine_iters) break;
-        if ((rc = solve_block(h, mb, h->Dc, h->R, nr))) return rc;
-        const long long total = n * nr;
-        hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)), dim3(256), 0, s,
-                           x, h->Dc, total);
+        rc = run_piece(h, 2, mb, nr, x, nullptr, [&] {
+            const int r = solve_block(h, mb, h->Dc, h->R, nr);
+            if (r) return r;
+            const long long total = n * nr;
+            hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)), dim3(256), 0,
+                               s, x, h->Dc, total);
+            return (int)hipGetLastError();
+        });
+        if (rc) return rc;
         h->last_refine_iters = it + 1;
     }
     return 0;
@@ -661,7 +703,9 @@ int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
             }
             rc = solve_refined(h, h->cur, x + c0 * n, bb, nr);
         } else {
-            rc = solve_block(h, h->cur, x + c0 * n, b + c0 * n, nr);
+            double* xc = x + c0 * n;
+            const double* bc = b + c0 * n;
+            rc = run_piece(h, 0, h->cur, nr, xc, bc, [&] { return solve_block(h, h->cur, xc, bc, nr); });
         }
         if (rc) return dev_status(h, rc, "solve");
     }
@@ -1048,6 +1092,8 @@ int rgf_destroy(rgf_handle_t h)
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->gx_factor) hipGraphExecDestroy(h->gx_factor);
     if (h->gx_selinv) hipGraphExecDestroy(h->gx_selinv);
+    for (auto& g : h->gx_pieces)
+        if (g.gx) hipGraphExecDestroy(g.gx);
     for (auto& a : h->allocs) cache_free(h->device, a.first, a.second);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -1124,8 +1170,9 @@ int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
         const size_t bytes = (size_t)n * nr * sizeof(double);
         hipError_t e = hipMemcpyAsync(h->Bv, b + c0 * n, bytes, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) return dev_status(h, (int)e, "solve upload");
-        rc = h->opt.precision != RGF_FP64 ? solve_refined(h, h->cur, h->X, h->Bv, nr)
-                                          : solve_block(h, h->cur, h->X, h->Bv, nr);
+        rc = h->opt.precision != RGF_FP64
+                 ? solve_refined(h, h->cur, h->X, h->Bv, nr)
+                 : run_piece(h, 0, h->cur, nr, h->X, h->Bv, [&] { return solve_block(h, h->cur, h->X, h->Bv, nr); });
         if (rc) return dev_status(h, rc, "solve");
         e = hipMemcpyAsync(x + c0 * n, h->X, bytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

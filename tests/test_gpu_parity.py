@@ -475,10 +475,11 @@ def test_mixed_batch_matches_single(gpu):
 # ------------------------------------------------------------------ HIP graph replays
 @pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
 def test_graph_replays_match_eager(gpu, prec):
-    """Sweeps captured in HIP graphs (options.graphs = 1): every replay -- with eager work
-    (regeneration, solves, host copies) and synchronizations between the replays, and a
-    re-capture after the kernel-timing configuration changed -- gives bitwise the eager
-    handle's logdet, diag(A^-1) and solution."""
+    """Sweeps and solve pieces captured in HIP graphs (options.graphs = 1): every replay --
+    with eager work (regeneration, solves, host copies) and synchronizations between the
+    replays, and a re-capture after the kernel-timing configuration changed -- gives
+    bitwise the eager handle's logdet, diag(A^-1), solutions (1 and 16 RHS, host and
+    device buffers) and refinement statistics."""
     say(f"graph replays prec={prec}")
     ns, nt, nd, count = 150, 5, 4, 2
     base = dict(precision=prec, keep_matrix=1)
@@ -488,7 +489,11 @@ def test_graph_replays_match_eager(gpu, prec):
                                   options=rgf_amd.default_options(graphs=1, **base))
     n = eager.n
     b = np.random.default_rng(3).standard_normal(n)
+    b16 = np.random.default_rng(4).standard_normal(n * 16)
     buf = rgf_amd.DeviceBuffer(n * count)
+    bdev = rgf_amd.DeviceBuffer(n * 16)
+    xdev = rgf_amd.DeviceBuffer(n * 16)
+    bdev.upload(b16)
     try:
         for rnd in range(5):
             if rnd == 2:
@@ -508,8 +513,17 @@ def test_graph_replays_match_eager(gpu, prec):
                 eager.select(m)
                 graph.select(m)
                 assert np.array_equal(graph.solve(None, b, 1), eager.solve(None, b, 1))
+                x16 = eager.solve(None, b16, 16)
+                assert np.array_equal(graph.solve(None, b16, 16), x16)
+                for rep in range(2):   # device buffers: the captured solve pieces replay
+                    graph.solve_device(xdev, bdev, 16)
+                    graph.synchronize()
+                    assert np.array_equal(xdev.download(), x16)
+                    assert graph.refine_stats() == eager.refine_stats()
     finally:
         rgf_amd.kernel_timing_enable(-1)
+        bdev.free()
+        xdev.free()
         buf.free()
         eager.close()
         graph.close()
