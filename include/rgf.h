@@ -137,6 +137,17 @@ const char *rgf_last_error(rgf_handle_t h);
 int rgf_solve_device(rgf_handle_t h, double *x_dev, const double *b_dev, size_t nrhs);
 /* diag(A^-1) of every matrix of the handle into device memory (count x n). */
 int rgf_selinv_diag_device(rgf_handle_t h, double *diag_dev);
+/* Extension (gradients of INLA-type objectives need more than the diagonal; the reference
+ * forms only diag(A^-1), RGF.H:582-612): the blocks of A^-1 on the block pattern of A for the matrix
+ * chosen by rgf_select, into device buffers (any may be NULL to skip it), column-major:
+ *   diag_blocks   nt blocks Sigma_ii      (ns x ns each, block i at i*ns*ns)
+ *   sub_blocks    nt-1 blocks Sigma_{i+1,i} (ns x ns each)
+ *   arrow_blocks  nt blocks Sigma_{d,i}   (nd x ns each, block i at i*nd*ns)
+ *   corner        Sigma_dd                (nd x nd)
+ * Needs selinv_mode = RGF_SELINV_EXACT.  Asynchronous on the handle's stream (see
+ * rgf_synchronize); also refreshes what rgf_selinv_diag returns. */
+int rgf_selinv_blocks_device(rgf_handle_t h, double *diag_blocks, double *sub_blocks, double *arrow_blocks,
+                             double *corner);
 int rgf_synchronize(rgf_handle_t h);
 /* Refinement of the last mixed-precision solve: steps taken and the final
  * max_c ||b_c - A x_c|| / ||b_c||. */

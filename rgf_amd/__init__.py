@@ -195,6 +195,31 @@ class RGF:
     def selinv_diag_device(self, diag: DeviceBuffer) -> None:
         _check(_lib.load().rgf_selinv_diag_device(self._h, diag.ptr), self._h)
 
+    def selinv_blocks(self) -> dict:
+        """Blocks of A^-1 on the block pattern of A for the selected matrix (exact selected
+        inversion): {"diag": (nt, ns, ns) Sigma_ii, "sub": (nt-1, ns, ns) Sigma_{i+1,i},
+        "arrow": (nt, nd, ns) Sigma_{d,i}, "corner": (nd, nd) Sigma_dd}."""
+        ns, nt, nd = self.ns, self.nt, self.nd
+        sizes = {"diag": nt * ns * ns, "sub": (nt - 1) * ns * ns, "arrow": nt * nd * ns, "corner": nd * nd}
+        bufs = {k: DeviceBuffer(v) if v > 0 else None for k, v in sizes.items()}
+        try:
+            ptr = [bufs[k].ptr if bufs[k] is not None else None for k in ("diag", "sub", "arrow", "corner")]
+            _check(_lib.load().rgf_selinv_blocks_device(self._h, *ptr), self._h)
+            self.synchronize()
+            out = {}
+            shapes = {"diag": (nt, ns, ns), "sub": (nt - 1, ns, ns), "arrow": (nt, nd, ns), "corner": (1, nd, nd)}
+            for k, (nb, r, c) in shapes.items():
+                if bufs[k] is None:
+                    out[k] = np.zeros((nb, r, c))
+                else:   # column-major blocks
+                    out[k] = bufs[k].download().reshape(nb, c, r).transpose(0, 2, 1).copy()
+            out["corner"] = out["corner"][0]
+            return out
+        finally:
+            for b in bufs.values():
+                if b is not None:
+                    b.free()
+
     def synchronize(self) -> None:
         _check(_lib.load().rgf_synchronize(self._h), self._h)
 

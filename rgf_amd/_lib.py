@@ -24,7 +24,7 @@ EXPORTS = [
     "rgf_default_options", "rgf_default_gen_params", "rgf_create_csc", "rgf_create_generated",
     "rgf_destroy", "rgf_factorize", "rgf_solve", "rgf_logdet", "rgf_selinv_diag",
     "rgf_residual_norm", "rgf_residual_norm_normalized", "rgf_last_error", "rgf_solve_device",
-    "rgf_selinv_diag_device", "rgf_synchronize", "rgf_get_info", "rgf_export_matrix_mf",
+    "rgf_selinv_diag_device", "rgf_selinv_blocks_device", "rgf_synchronize", "rgf_get_info", "rgf_export_matrix_mf",
     "rgf_export_factor_mf", "rgf_kernel_timing_enable", "rgf_kernel_timing_read",
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats",
@@ -89,6 +89,7 @@ def load():
         lib.rgf_last_error.restype = C.c_char_p
         lib.rgf_solve_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_size_t]
         lib.rgf_selinv_diag_device.argtypes = [H, C.c_void_p]
+        lib.rgf_selinv_blocks_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.rgf_synchronize.argtypes = [H]
         lib.rgf_get_info.argtypes = [H, C.POINTER(Info)]
         lib.rgf_export_matrix_mf.argtypes = [H, dp]

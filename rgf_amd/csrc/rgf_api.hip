@@ -719,7 +719,19 @@ int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
 //   S_ii = X_i^T X_i + W^T Z                 S_{d,i} = -Z[D rows]
 // exact = 0 keeps Sb block-diagonal, which is the reference's formula.
 // diag_out: count x n (matrix b at b*n).
-int selinv_sweep(rgf_handle_s* h, double* diag_out)
+// Blocks of A^-1 on the pattern of A for matrix mb, written as the sweep forms them
+// (device buffers, any may be null): Sigma_ii (nt x ns x ns), Sigma_{i+1,i} = -Z[0:ns]
+// ((nt-1) x ns x ns), Sigma_{d,i} (nt x nd x ns, formed by the exact recursion) and
+// Sigma_dd (nd x nd); column-major blocks.
+struct SigmaOut {
+    double* diag;
+    double* sub;
+    double* arrow;
+    double* corner;
+    int mb;
+};
+
+int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, lds = h->ldf, B = h->count;
     const long long ms = h->mstride, ss = h->sstride, fs = h->fstride, n = h->n;
@@ -737,6 +749,9 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out)
             if ((rc = gemm_f64(Op::T, Op::N, &d, 1, s))) return rc;
         }
         if ((rc = extract_diag(h->Sb[0] + (long long)ns * lds + ns, lds, nd, diag_out + (long long)nt * ns, s, B, ss, n)))
+            return rc;
+        if (so && so->corner &&
+            (rc = copy2d(h->Sb[0] + so->mb * ss + (long long)ns * lds + ns, lds, so->corner, nd, nd, nd, 1.0, s)))
             return rc;
     }
     // Split-BF16 path: X_i is split once (k-contiguous columns) for W = F X, X^T X as
@@ -797,6 +812,16 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out)
         if ((rc = selinv_finish(Sn, lds, ss, ns, nd, h->Z, lds, fs, mF - nd, exact && mF > 0,
                                 diag_out + (long long)i * ns, n, B, s)))
             return rc;
+        if (so) {
+            const double* Sm = Sn + so->mb * ss;
+            const long long bs = (long long)ns * ns;
+            if (so->diag && (rc = copy2d(Sm, lds, so->diag + i * bs, ns, ns, ns, 1.0, s))) return rc;
+            if (so->sub && !last && (rc = copy2d(h->Z + so->mb * fs, lds, so->sub + i * bs, ns, ns, ns, -1.0, s)))
+                return rc;
+            if (so->arrow && nd > 0 &&
+                (rc = copy2d(Sm + ns, lds, so->arrow + (long long)i * nd * ns, nd, nd, ns, 1.0, s)))
+                return rc;
+        }
         cur ^= 1;
     }
     return 0;
@@ -1186,6 +1211,22 @@ int rgf_selinv_diag_device(rgf_handle_t h, double* diag_dev)
     if (!h || !diag_dev) return RGF_EINVAL;
     hipSetDevice(h->device);
     return do_selinv_device(h, diag_dev);
+}
+
+int rgf_selinv_blocks_device(rgf_handle_t h, double* diag_blocks, double* sub_blocks, double* arrow_blocks,
+                             double* corner)
+{
+    if (!h) return RGF_EINVAL;
+    if (h->opt.selinv_mode != RGF_SELINV_EXACT)
+        return set_err(h, RGF_EINVAL, "the blocks of A^-1 need the exact selected inversion (selinv_mode 0)");
+    hipSetDevice(h->device);
+    int rc = ensure_factorized(h);
+    if (rc) return rc;
+    if ((rc = ensure_selinv_buffers(h))) return rc;
+    const SigmaOut so{diag_blocks, sub_blocks, arrow_blocks, corner, h->cur};
+    rc = selinv_sweep(h, h->diag, &so);   // eager: the output buffers vary between calls
+    if (rc) return dev_status(h, rc, "selinv blocks");
+    return RGF_OK;
 }
 
 int rgf_selinv_diag_batch(rgf_handle_t h, double* diag)

@@ -472,6 +472,43 @@ def test_mixed_batch_matches_single(gpu):
             assert np.array_equal(h1.RGFdiag(), diags[b])
 
 
+# ------------------------------------------------------------------ selected-inverse blocks
+@pytest.mark.parametrize("prec,tol", [(rgf_amd.RGF_FP64, 1e-10), (rgf_amd.RGF_MIXED_BF16X3, 1e-4)])
+@pytest.mark.parametrize("ns,nt,nd", [(40, 4, 3), (64, 3, 0), (33, 1, 5), (130, 3, 7)])
+def test_selinv_blocks_vs_dense_inverse(gpu, prec, tol, ns, nt, nd):
+    """Every block of A^-1 on the pattern of A (Sigma_ii, Sigma_{i+1,i}, Sigma_{d,i},
+    Sigma_dd) against numpy's dense inverse of the handle's own matrix; the diagonal of
+    the blocks equals diag(A^-1) from RGFdiag."""
+    say(f"selinv blocks prec={prec} ns={ns} nt={nt} nd={nd}")
+    opts = rgf_amd.default_options(precision=prec, keep_matrix=1)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=17, options=opts) as h:
+        A = oracle.mf_to_dense(h.export_matrix_mf(), ns, nt, nd)
+        S = np.linalg.inv(A)
+        blk = h.selinv_blocks()
+        d = h.RGFdiag()
+    scale = np.abs(S).max()
+    for i in range(nt):
+        r = slice(i * ns, (i + 1) * ns)
+        assert np.abs(blk["diag"][i] - S[r, r]).max() <= tol * scale, ("diag", i)
+        if i < nt - 1:
+            r1 = slice((i + 1) * ns, (i + 2) * ns)
+            assert np.abs(blk["sub"][i] - S[r1, r]).max() <= tol * scale, ("sub", i)
+        if nd > 0:
+            assert np.abs(blk["arrow"][i] - S[nt * ns:, r]).max() <= tol * scale, ("arrow", i)
+    if nd > 0:
+        assert np.abs(blk["corner"] - S[nt * ns:, nt * ns:]).max() <= tol * scale
+    diag_blocks = np.concatenate([np.diag(blk["diag"][i]) for i in range(nt)] +
+                                 ([np.diag(blk["corner"])] if nd > 0 else []))
+    assert np.array_equal(diag_blocks, d)
+
+
+def test_selinv_blocks_need_exact_mode(gpu):
+    opts = rgf_amd.default_options(selinv_mode=rgf_amd.RGF_SELINV_REFERENCE)
+    with rgf_amd.RGF.generated(40, 3, 2, seed=1, options=opts) as h:
+        with pytest.raises(rgf_amd.RGFError):
+            h.selinv_blocks()
+
+
 # ------------------------------------------------------------------ HIP graph replays
 @pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
 def test_graph_replays_match_eager(gpu, prec):
