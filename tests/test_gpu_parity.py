@@ -294,6 +294,70 @@ def test_larger_block_properties(gpu):
     assert np.abs(d / oracle.selinv(L, ns, nt, nd, exact=True) - 1).max() <= 1e-11
 
 
+def test_c1_full_size_vs_oracle(gpu):
+    """The INLA-like single-matrix workload at its full size (ns 1000, nt 50, nd 4,
+    n = 50 004): logdet, a solve and diag(A^-1) against the oracle on the same matrix."""
+    ns, nt, nd = 1000, 50, 4
+    say(f"c1 full size ns={ns} nt={nt} nd={nd}")
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=2024) as s:
+        A_mf = s.export_matrix_mf()
+        b = np.random.default_rng(9).standard_normal(s.n)
+        x = s.solve(None, b, 1)
+        ld = s.logDet()
+        d = s.RGFdiag()
+    L = oracle.factor(A_mf, ns, nt, nd)
+    assert abs(ld - oracle.logdet(L, ns, nt, nd)) <= 1e-11 * abs(ld)
+    x0 = oracle.solve(L, ns, nt, nd, b)
+    assert np.abs(x - x0).max() <= 1e-10 * np.abs(x0).max()
+    assert np.abs(d / oracle.selinv(L, ns, nt, nd, exact=True) - 1).max() <= 1e-10
+
+
+def test_c2_full_size_properties(gpu):
+    """The c2 workload at full size (ns 2865, nt 100, nd 6, n = 286 506), where a dense
+    check is out of reach, through properties that do not depend on the size: a solve's
+    residual against the resident matrix; diag(A^-1)_j equals x_j of A x = e_j (and the
+    selected-inverse block entries equal the same solve's neighbours) for sampled j in
+    the first, a middle and the last block and in the arrow; the mixed path against
+    FP64 (logdet 1e-8 relative, diag 1e-4)."""
+    ns, nt, nd = 2865, 100, 6
+    say(f"c2 full size ns={ns} nt={nt} nd={nd}")
+    n = ns * nt + nd
+    js = [0, 17, ns - 1, 50 * ns + 1000, (nt - 1) * ns + 5, n - 3, n - 1]
+    E = np.zeros((len(js), n))
+    for c, j in enumerate(js):
+        E[c, j] = 1.0
+    rhs = E.reshape(-1)   # column-major n x len(js)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=2024) as s:
+        b = np.random.default_rng(11).standard_normal(n)
+        x = s.solve(None, b, 1)
+        assert s.residualNormNormalized(x, b) < 1e-14
+        ld = s.logDet()
+        d = s.RGFdiag()
+        X = s.solve(None, rhs, len(js)).reshape(len(js), n)
+        # only the sub-diagonal and arrow blocks (the diagonal blocks alone are 6.6 GB)
+        sub = rgf_amd.DeviceBuffer((nt - 1) * ns * ns)
+        arrow = rgf_amd.DeviceBuffer(nt * nd * ns)
+        try:
+            assert _lib.load().rgf_selinv_blocks_device(s._h, None, sub.ptr, arrow.ptr, None) == 0
+            s.synchronize()
+            sub0 = sub.download(ns * ns).reshape(ns, ns, order="F")
+            arr = arrow.download().reshape(nt, ns, nd).transpose(0, 2, 1)
+        finally:
+            sub.free()
+            arrow.free()
+    for c, j in enumerate(js):
+        assert abs(X[c, j] - d[j]) <= 1e-10 * abs(d[j]), j
+    # Sigma_{1,0}[:, 17] is column 17 of A^-1 in rows ns..2ns-1
+    assert np.abs(sub0[:, 17] - X[1, ns:2 * ns]).max() <= 1e-10 * np.abs(X[1]).max()
+    # Sigma_{d,99}[:, 5] is column (nt-1)*ns+5 of A^-1 in the arrow rows
+    assert np.abs(arr[nt - 1][:, 5] - X[4, nt * ns:]).max() <= 1e-10 * np.abs(X[4]).max()
+    assert (d > 0).all()
+    opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=2024, options=opts) as m:
+        assert abs(m.logDet() - ld) <= 1e-8 * abs(ld)
+        assert np.abs(m.RGFdiag() / d - 1).max() <= 1e-4
+
+
 def test_cli_matches_reference_driver_format(gpu, tmp_path):
     say("RGFSolver CLI")
     ia, ja, a = oracle.closed_form_csc(3, 3, 2)
