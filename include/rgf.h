@@ -163,7 +163,8 @@ int rgf_export_factor_mf(rgf_handle_t h, double *mf);
 
 /* Per-process GEMM kernel timing with HIP events on the launching stream:
  * op_class 0 = NN, 1 = NT, 2 = TN, 3 = TT, -1 = off.  read() synchronizes and
- * returns the summed kernel time (ms), the summed canonical flops and the launch count. */
+ * returns the summed kernel time (ms), the summed canonical flops and the launch count.
+ * A measurement aid: enable it only while one thread drives the library. */
 int rgf_kernel_timing_enable(int op_class);
 int rgf_kernel_timing_read(double *ms, double *flops, long long *launches);
 

@@ -11,6 +11,9 @@ for follow-up solves and are evicted when the budget is needed.
 
 The scheduler (admission, ordering, eviction, latency accounting) is independent
 of the executor; tests drive it with a CPU executor, the benchmark with the GPU one.
+Give the process at least one hardware queue per slot (GPU_MAX_HW_QUEUES, set before the
+HIP runtime starts; HIP's default is 4): streams that share a hardware queue run their
+kernels one after another.
 """
 from __future__ import annotations
 
