@@ -88,14 +88,20 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
         info += b;
     }
     const bool lead = blockIdx.x == 0;   // writes X_kk, the log-determinant part and info
-    __shared__ __attribute__((aligned(16))) double Ls[NB][LP];
-    __shared__ __attribute__((aligned(16))) double Xs[NB][LP];
-    __shared__ __attribute__((aligned(16))) double Ts[2][NB / 2][NB / 2 + 2];
-    __shared__ __attribute__((aligned(16))) double Pk[NB][LP];   // previous panel, rows of this tile
-    __shared__ __attribute__((aligned(16))) double Pr[NB][LP];   // previous panel, rows of a row block
-    __shared__ double colb[NB];
-    __shared__ double rinv[NB];
-    __shared__ double dgl[NB];
+    // One LDS pool, the per-step broadcast arrays at its start: below 64 KB their
+    // addresses fit the instructions' immediate offsets (no address register set up per
+    // access in the pivot chain).
+    constexpr int TSZ = 2 * (NB / 2) * (NB / 2 + 2);
+    __shared__ __attribute__((aligned(16))) double pool[3 * NB + 4 * NB * LP + TSZ];
+    double* const colb = pool;
+    double* const rinv = pool + NB;
+    double* const dgl = pool + 2 * NB;
+    double (*const Ls)[LP] = reinterpret_cast<double (*)[LP]>(pool + 3 * NB);
+    double (*const Xs)[LP] = Ls + NB;
+    double (*const Pk)[LP] = Ls + 2 * NB;   // previous panel, rows of this tile
+    double (*const Pr)[LP] = Ls + 3 * NB;   // previous panel, rows of a row block
+    double (*const Ts)[NB / 2][NB / 2 + 2] =
+        reinterpret_cast<double (*)[NB / 2][NB / 2 + 2]>(pool + 3 * NB + 4 * NB * LP);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     int bad = INT_MAX;   // first non-positive pivot (wave 0, uniform)
