@@ -174,6 +174,7 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
             bool ok = d > 0.0;
             double y = rsqrt_nr(ok ? d : 1.0);
             double lprev = 0.0;
+            double my_rinv = 1.0, my_dgl = 1.0;   // 1/sqrt and sqrt of this lane's row's pivot
 #pragma unroll
             for (int jj = 0; jj < 16; jj++) {
                 const int j = o + jj;
@@ -192,15 +193,14 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
 #pragma unroll
                     for (int k = jj + 2; k < 16; k++) p[k] = fma(-lprev, cbv[k], p[k]);
                 }
-                const double sd = ok ? d * y : 1.0;
                 if (!ok && j < kb) bad = min(bad, j);
-                if (r == 0) {
-                    rinv[j] = ok ? y : 1.0;
-                    dgl[j] = sd;
-                }
                 // lane j holds p[jj] == d, so its entry is sqrt(d) = d y
                 const double l = r >= j ? p[jj] * y : 0.0;
                 p[jj] = l;
+                if (r == j) {   // kept in registers, stored once after the panel
+                    my_rinv = ok ? y : 1.0;
+                    my_dgl = ok ? l : 1.0;
+                }
                 if (jj < 15) {
                     p[jj + 1] = fma(-l, s1, p[jj + 1]);
                     if (jj < 14) {
@@ -228,6 +228,10 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
             if (r >= o) {
 #pragma unroll
                 for (int c = 0; c < 16; c++) Ls[r][o + c] = p[c];
+            }
+            if (r >= o && r < o + 16) {
+                rinv[r] = my_rinv;
+                dgl[r] = my_dgl;
             }
         }
         __syncthreads();
