@@ -51,25 +51,31 @@ __device__ __forceinline__ double rsqrt_nr(double d)
     return fma(y0, e * fma(e, 0.375, 0.5), y0);
 }
 
-// 64 x 64 MFMA product on LDS operands, accumulated into the caller's C quarter:
-// wave (wm, wn) adds sgn * X Y^T to rows 32 wm.., columns 32 wn.. (X, Y row-major
-// 64 x 64 with pitch LP).  acc[a][b] item t is row 16 a + l16, column 16 b + 4 t + lq.
-__device__ __forceinline__ void mfma_nt64(v4d (&acc)[2][2], const double (*X)[LP], const double (*Y)[LP], double sgn,
-                                          int wm, int wn, int l16, int lq)
+// (16 RA) x 64 x 64 MFMA product per wave on LDS operands, accumulated into the caller's
+// C block: wave (wm, wn) adds sgn * X Y^T to rows 16 RA wm.., columns 32 wn.. (X, Y
+// row-major with pitch LP, 64 columns).  acc[a][b] item t is row 16 RA wm + 16 a + l16,
+// column 32 wn + 16 b + 4 t + lq.  RA = 2: a 64 x 64 C over the four waves; RA = 1: 32 x 64.
+template <int RA>
+__device__ __forceinline__ void mfma_nt(v4d (&acc)[RA][2], const double (*X)[LP], const double (*Y)[LP], double sgn,
+                                        int wm, int wn, int l16, int lq)
 {
 #pragma unroll
     for (int ks = 0; ks < NB / 4; ks++) {
-        double af[2], bf[2];
+        double af[RA], bf[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++) af[a] = sgn * X[32 * wm + 16 * a + l16][4 * ks + lq];
+        for (int a = 0; a < RA; a++) af[a] = sgn * X[16 * RA * wm + 16 * a + l16][4 * ks + lq];
 #pragma unroll
         for (int b = 0; b < 2; b++) bf[b] = Y[32 * wn + 16 * b + l16][4 * ks + lq];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 0; a < RA; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
     }
 }
+
+// rows of the panel below the tile handled per workgroup pass (half a tile: the MFMA work
+// of a pass, on the launch's critical path, is half that of a 64-row block)
+constexpr int RB = 32;
 
 // Column block k0 of the blocked Cholesky, one launch: tile factor and inverse, then the
 // panel rows.  With prev_update the update of this column block by the previous panel
@@ -142,7 +148,7 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
                 for (int b = 0; b < 2; b++)
 #pragma unroll
                     for (int t = 0; t < 4; t++) acc[a][b][t] = Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
-            mfma_nt64(acc, Pk, Pk, -1.0, wm, wn, l16, lq);
+            mfma_nt<2>(acc, Pk, Pk, -1.0, wm, wn, l16, lq);
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -334,64 +340,57 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
     }
 
     // ---- the panel below the tile, in place: L[R][k] = A[R][k] X_kk^T for row blocks R
-    // of 64 (block rb by workgroup rb mod gridDim.x).  Each wave computes a 32 x 32
-    // quarter on FP64 MFMA; item t of lane (l16, lq) is row l16, column 4 t + lq of
+    // of RB = 32 rows (block rb by workgroup rb mod gridDim.x).  Each wave computes a
+    // 16 x 32 part on FP64 MFMA; item t of lane (l16, lq) is row l16, column 4 t + lq of
     // each 16 x 16 block.
-    const int nrb = (below + NB - 1) / NB;
+    const int nrb = (below + RB - 1) / RB;
     const int wm = wave & 1, wn = wave >> 1;
     for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
-        const int r0 = k0 + kb + rb * NB, rows = min(NB, below - rb * NB);
+        const int r0 = k0 + kb + rb * RB, rows = min(RB, below - rb * RB);
         __syncthreads();   // Ls is free (inverse, stores or the previous block done)
         {
-            double v[NB * NB / 256], w[NB * NB / 256];
+            // element e = q * 256 + tid: row e % RB, column e / RB (32 rows of a column in
+            // a row of lanes: coalesced)
+            double v[RB * NB / 256], w[RB * NB / 256];
 #pragma unroll
-            for (int q = 0; q < NB * NB / 256; q++) {
-                const int r = lane, c = 4 * q + wave;
+            for (int q = 0; q < RB * NB / 256; q++) {
+                const int e = q * 256 + tid, r = e % RB, c = e / RB;
                 const bool in = r < rows && c < kb;
                 v[q] = A[in ? (long long)(k0 + c) * lda + (r0 + r) : (long long)k0 * lda + r0];
                 if (prev_update) w[q] = A[r < rows ? (long long)(k0 - NB + c) * lda + (r0 + r) : (long long)k0 * lda + r0];
             }
 #pragma unroll
-            for (int q = 0; q < NB * NB / 256; q++) {
-                const int r = lane, c = 4 * q + wave;
+            for (int q = 0; q < RB * NB / 256; q++) {
+                const int e = q * 256 + tid, r = e % RB, c = e / RB;
                 Ls[r][c] = (r < rows && c < kb) ? v[q] : 0.0;
                 if (prev_update) Pr[r][c] = r < rows ? w[q] : 0.0;
             }
         }
         __syncthreads();
-        v4d acc[2][2];
+        v4d acc[1][2];
         if (prev_update) {   // A_Rk -= P_R P_k^T, in place in Ls (each lane rewrites its own entries)
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-#pragma unroll
-                    for (int t = 0; t < 4; t++) acc[a][b][t] = Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
-            mfma_nt64(acc, Pr, Pk, -1.0, wm, wn, l16, lq);
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-#pragma unroll
-                    for (int t = 0; t < 4; t++) Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
-            __syncthreads();
-        }
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-        mfma_nt64(acc, Ls, Xs, 1.0, wm, wn, l16, lq);   // R X^T (X^T[k][j] = X[j][k])
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-            const int row = 32 * wm + 16 * a + l16;
 #pragma unroll
             for (int b = 0; b < 2; b++)
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int col = 32 * wn + 16 * b + 4 * t + lq;
-                    if (row < rows && col < kb) A[(long long)(k0 + col) * lda + (r0 + row)] = acc[a][b][t];
-                }
+                for (int t = 0; t < 4; t++) acc[0][b][t] = Ls[16 * wm + l16][32 * wn + 16 * b + 4 * t + lq];
+            mfma_nt<1>(acc, Pr, Pk, -1.0, wm, wn, l16, lq);
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) Ls[16 * wm + l16][32 * wn + 16 * b + 4 * t + lq] = acc[0][b][t];
+            __syncthreads();
         }
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[0][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+        mfma_nt<1>(acc, Ls, Xs, 1.0, wm, wn, l16, lq);   // R X^T (X^T[k][j] = X[j][k])
+        const int row = 16 * wm + l16;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int col = 32 * wn + 16 * b + 4 * t + lq;
+                if (row < rows && col < kb) A[(long long)(k0 + col) * lda + (r0 + row)] = acc[0][b][t];
+            }
     }
 }
 
@@ -413,8 +412,8 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
         const bool odd = p & 1;
         double* xt = xdiag + (long long)p * NB * NB;
         // every workgroup of a matrix factors the tile (in LDS) and then produces its
-        // share of the 64-row blocks of the panel below
-        const int nrb = (below + NB - 1) / NB;
+        // share of the RB-row blocks of the panel below
+        const int nrb = (below + RB - 1) / RB;
         const int nf = std::max(1, std::min(nrb, std::max(1, 256 / count)));
         hipLaunchKernelGGL(potrf_tile_kernel, dim3(nf, count), dim3(256), 0, s, A, lda, sA, k0, kb, below, odd, xt,
                            sX, logdet_parts + p, sP, info, info_base, store_diag_l);
