@@ -56,10 +56,13 @@ def test_gemm_kernel(gpu, ta, tb, m, n, k, tile):
 
 @pytest.mark.parametrize("flags", [1, 2, 8, 16, 1 | 4 | 8])
 @pytest.mark.parametrize("tile", [0, 2, 3])
-def test_gemm_kernel_triangular_flags(gpu, flags, tile):
-    say(f"gemm tile={tile} flags={flags}")
+@pytest.mark.parametrize("size", [150, 100])
+def test_gemm_kernel_triangular_flags(gpu, flags, tile, size):
+    """Triangle flags (k-range skipping, lower-stored C) against numpy, at a size that is
+    and one that is not a multiple of the 4-wide MFMA k step."""
+    say(f"gemm tile={tile} flags={flags} size={size}")
     rng = np.random.default_rng(flags)
-    m = n = k = 150
+    m = n = k = size
     L = np.tril(rng.standard_normal((k, k)))
     G0 = rng.standard_normal((k, k))
     # choose operands so that the flagged operand really is triangular
