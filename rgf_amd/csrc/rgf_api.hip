@@ -1471,6 +1471,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         g_presplit_swizzle = value;
         return old;
     }
+    if (key == 4 && (value == 0 || value == 1)) {
+        const int old = g_gemm_fast_loads;
+        g_gemm_fast_loads = value;
+        return old;
+    }
     return -1;
 }
 

@@ -148,11 +148,48 @@ __global__ __launch_bounds__(256, gemm_occupancy(TM, TN)) void gemm_f64_kernel(G
         }
     };
 
-    if (klo < khi) load_tiles(klo);
+    // Full k-steps: each address is the tile origin moved to the k-step (uniform) plus a
+    // loop-invariant 32-bit lane offset, with no per-load address arithmetic or bounds
+    // branch.  Lanes on rows of op(A) beyond M (columns of op(B) beyond N) load the last
+    // valid row (column) instead: they feed only C entries that are never stored.  The
+    // k-step that crosses khi keeps the per-element masks (k beyond khi must read zero).
+    const bool fast = args.fast_loads && (long long)(BK + TM) * lda < (1LL << 28) &&
+                      (long long)(BK + TN) * ldb < (1LL << 28);
+    unsigned aoff[LA], boff[LB];   // bytes (below 2^31 when fast)
+#pragma unroll
+    for (int q = 0; q < LA; q++) {
+        int r, kk;
+        a_rk(q, r, kk);
+        const unsigned rr = (unsigned)min(r, M - 1 - i0);
+        aoff[q] = 8u * (TA ? rr * (unsigned)lda + kk : (unsigned)kk * (unsigned)lda + rr);
+    }
+#pragma unroll
+    for (int q = 0; q < LB; q++) {
+        int c, kk;
+        b_ck(q, c, kk);
+        const unsigned cc = (unsigned)min(c, N - 1 - j0);
+        boff[q] = 8u * (TB ? (unsigned)kk * (unsigned)ldb + cc : cc * (unsigned)ldb + kk);
+    }
+    const double* const At = A + (TA ? (long long)i0 * lda : (long long)i0);
+    const double* const Bt = B + (TB ? (long long)j0 : (long long)j0 * ldb);
+    auto load = [&](int k0) {
+        if (fast && k0 + BK <= khi) {
+            const char* pa = (const char*)(At + (TA ? (long long)k0 : (long long)k0 * lda));
+            const char* pb = (const char*)(Bt + (TB ? (long long)k0 * ldb : (long long)k0));
+#pragma unroll
+            for (int q = 0; q < LA; q++) ra[q] = *(const double*)(pa + aoff[q]);
+#pragma unroll
+            for (int q = 0; q < LB; q++) rb[q] = *(const double*)(pb + boff[q]);
+        } else {
+            load_tiles(k0);
+        }
+    };
+
+    if (klo < khi) load(klo);
     for (int k0 = klo; k0 < khi; k0 += BK) {
         store_tiles();
         __syncthreads();
-        if (k0 + BK < khi) load_tiles(k0 + BK);
+        if (k0 + BK < khi) load(k0 + BK);
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ks++) {
             const int kr = ks * 4 + lq;
@@ -230,6 +267,8 @@ static void launch_f64(Op ta, Op tb, const GemmArgs& args, dim3 grid, hipStream_
     else hipLaunchKernelGGL((gemm_f64_kernel<true, true, TM, TN>), grid, block, 0, s, args);
 }
 
+int g_gemm_fast_loads = 1;
+
 // Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, -1 = automatic.
 // Measured on MI355X at the sweep's shapes (ns = 2865, single and 4-matrix batches):
 // 64x64 at 5 workgroups per CU is the fastest or within a few percent everywhere
@@ -257,6 +296,7 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
     }
     if (nd == 0) return 0;
     args.nd = nd;
+    args.fast_loads = g_gemm_fast_loads;
     dim3 grid(maxtiles, z);
     const int cls = (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0);
     void* tok = timing_begin(cls, args.d, nd, s);

@@ -408,17 +408,29 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     bf16x8 rah[CH], ral[CH], rbh[CH], rbl[CH];
     // bf16 offset in an LDS row of logical 16-byte chunk j of row r
     auto lcol = [](int r, int j) { return 8 * (SW ? (j ^ ((r / RPG) % CPR)) : j); };
+    // Loads without masks: the split copies hold roundup(rows, 128) rows of a 64-aligned
+    // pitch, so every row and k of a tile is inside its entry.  Rows beyond M (N) hold
+    // unwritten values that reach only C entries that are never stored; k beyond K is
+    // zero.  Lane offsets from the tile origin (bytes) are loop-invariant 32-bit values.
+    unsigned lofa[CH], lofb[CH];
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+        const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
+        lofa[q] = 2u * ((unsigned)r * (unsigned)ldka + kk);
+        lofb[q] = 2u * ((unsigned)r * (unsigned)ldkb + kk);
+    }
+    const char* const Ah0 = (const char*)(Ahg + (long long)i0 * ldka);
+    const char* const Al0 = (const char*)(Ahg + loA + (long long)i0 * ldka);
+    const char* const Bh0 = (const char*)(Bhg + (long long)j0 * ldkb);
+    const char* const Bl0 = (const char*)(Bhg + loB + (long long)j0 * ldkb);
     auto load = [&](int k0) {
+        const long long kb = 2LL * k0;
 #pragma unroll
         for (int q = 0; q < CH; q++) {
-            const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
-            const int ga = i0 + r, gb = j0 + r;
-            const long long oa = (long long)ga * ldka + k0 + kk, ob = (long long)gb * ldkb + k0 + kk;
-            const bf16x8 zero = {};
-            rah[q] = ga < M ? *(const bf16x8*)(Ahg + oa) : zero;
-            ral[q] = ga < M ? *(const bf16x8*)(Ahg + loA + oa) : zero;
-            rbh[q] = gb < N ? *(const bf16x8*)(Bhg + ob) : zero;
-            rbl[q] = gb < N ? *(const bf16x8*)(Bhg + loB + ob) : zero;
+            rah[q] = *(const bf16x8*)(Ah0 + kb + lofa[q]);
+            ral[q] = *(const bf16x8*)(Al0 + kb + lofa[q]);
+            rbh[q] = *(const bf16x8*)(Bh0 + kb + lofb[q]);
+            rbl[q] = *(const bf16x8*)(Bl0 + kb + lofb[q]);
         }
     };
     auto store = [&]() {

@@ -48,7 +48,8 @@ struct GemmDesc {
 constexpr int GEMM_MAXD = 4;
 struct GemmArgs {
     int nd;
-    int pad_[3];
+    int fast_loads;   // FP64 kernel: full k-steps loaded through loop-invariant lane offsets
+    int pad_[2];
     GemmDesc d[GEMM_MAXD];
 };
 
@@ -58,6 +59,7 @@ enum class Op { N, T };
 // transposes (all entries are independent).  Returns hipError_t as int.
 // cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128; -1 automatic).
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
+extern int g_gemm_fast_loads;   // 1: GemmArgs::fast_loads set on FP64 launches
 
 // Same interface on split-BF16 MFMA (three bf16 products per FP64 product, FP32
 // accumulation inside a call, FP64 alpha/beta epilogue).

@@ -54,6 +54,44 @@ def test_gemm_kernel(gpu, ta, tb, m, n, k, tile):
     assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * k
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("tile", [1, 2, 3])
+def test_gemm_fast_loads_match_masked_loads(gpu, ta, tb, tile):
+    """The FP64 kernel's full-k-step loads (loop-invariant lane offsets, edge rows clamped
+    to the last valid row) give the same bits as per-element masked loads, on ragged
+    shapes whose padding rows hold NaN (a read of them would reach the output)."""
+    say(f"gemm fast loads tile={tile} ta={ta} tb={tb}")
+    lib = _lib.load()
+    rng = np.random.default_rng(11 + 4 * ta + 2 * tb + tile)
+    m, n, k = 197, 131, 75
+    lda, ldb, ldc = (k if ta else m) + 5, (n if tb else k) + 3, m
+    A = np.full((lda, m if ta else k), np.nan, order="F")
+    B = np.full((ldb, k if tb else n), np.nan, order="F")
+    if ta:
+        A[:k, :m] = rng.standard_normal((k, m))
+    else:
+        A[:m, :k] = rng.standard_normal((m, k))
+    if tb:
+        B[:n, :k] = rng.standard_normal((n, k))
+    else:
+        B[:k, :n] = rng.standard_normal((k, n))
+    ref = (A[:k, :m].T if ta else A[:m, :k]) @ (B[:n, :k].T if tb else B[:k, :n])
+    outs = []
+    for fast in (1, 0):
+        old = lib.rgf_debug_set_tuning(4, fast)
+        try:
+            Cf = np.zeros(ldc * n)
+            rc = lib.rgf_debug_gemm(ta, tb, m, n, k, 1.0, dptr(A.reshape(-1, order="F").copy()), lda,
+                                    dptr(B.reshape(-1, order="F").copy()), ldb, 0.0, dptr(Cf), ldc, tile << 12)
+        finally:
+            lib.rgf_debug_set_tuning(4, old)
+        assert rc == 0
+        outs.append(Cf.reshape(ldc, n, order="F"))
+    assert np.isfinite(outs[0]).all()
+    assert np.array_equal(outs[0], outs[1])
+    assert np.abs(outs[0] - ref).max() <= 1e-13 * np.abs(ref).max() * k
+
+
 @pytest.mark.parametrize("flags", [1, 2, 8, 16, 1 | 4 | 8])
 @pytest.mark.parametrize("tile", [0, 2, 3])
 @pytest.mark.parametrize("size", [150, 100])
