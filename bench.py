@@ -243,6 +243,11 @@ def run_c3(args):
     print(json.dumps(res), flush=True)
 
 
+def note(msg):
+    """Progress of a long run on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -317,6 +322,7 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     kms, kfl, kn = rgf_amd.kernel_timing_read() if not args.no_kernel_timing else (0.0, 0.0, 0)
     rgf_amd.kernel_timing_enable(-1)
+    note(f"fp64 steps: {dt:.4f} s per step")
     if main_graphs:   # the timing change invalidates the captured sweeps: capture them again
         step()
         s.synchronize()
@@ -367,6 +373,7 @@ def main():
                         "solve_1rhs_ms": round(gs[1] * 1e3, 3), "solve_16rhs_ms": round(gs[16] * 1e3, 3),
                         "solve_1rhs_GBps": round(info.bytes_solve_per_rhs / gs[1] / 1e9, 1)}
         g.close(); gd.free()
+        note(f"graph replay: {dtg:.4f} s per step")
 
     # mixed precision, same matrix: split-BF16 products, FP64 potrf / logdet / refinement
     mixed = None
@@ -400,6 +407,7 @@ def main():
             "x_16rhs_max_rel_err_vs_fp64": float(np.max(np.abs(xmh - x64)) / np.max(np.abs(x64))),
         }
         m.close(); md.free(); xm.free()
+        note(f"mixed steps: {dtm:.4f} s per step")
     b.free(); x.free()
 
     # batched: B matrices of the c2 shape per launch, FP64 and mixed
@@ -408,6 +416,7 @@ def main():
         tb64, ld64, d64 = time_batch(rgf_amd, ns, nt, nd, args.batch, rgf_amd.RGF_FP64, args.seed, 2, 1)
         batched = {"count": args.batch, "fp64_s_per_matrix": round(tb64 / args.batch, 4),
                    "fp64_TFLOPs": round(args.batch * (F + S) / tb64 / 1e12, 4)}
+        note(f"fp64 batch of {args.batch}: {tb64 / args.batch:.4f} s per matrix")
         if not args.no_mixed:
             tbm, ldm_b, dm_b = time_batch(rgf_amd, ns, nt, nd, args.batch, rgf_amd.RGF_MIXED_BF16X3, args.seed, 2, 1)
             batched.update({
@@ -431,7 +440,11 @@ def main():
         "launches": kn,
         "avg_launch_ms": round(kms / kn, 4) if kn else None,
     }
+    if batched and "mixed_s_per_matrix" in batched:
+        note(f"mixed batch of {args.batch}: {batched['mixed_s_per_matrix']:.4f} s per matrix")
     cpu = None if args.no_cpu_baseline else cpu_baseline(ns, nt, nd, args.seed, args.cpu_blocks)
+    if cpu:
+        note(f"cpu baseline: {cpu}")
     out = {
         "metric": "factorize+logdet+selinv s/matrix & TFLOP/s; batched req/s + p99, mixed vs FP64",
         "value": round((F + S) / dt / 1e12, 4),
