@@ -78,3 +78,18 @@ def test_footprint_bytes_host_only():
     assert rgf_amd.footprint_bytes(500, 10, 4, 3, keep) > 2 * f
     assert rgf_amd.footprint_bytes(0, 10, 4, 1, keep) == 0
     assert rgf_amd.footprint_bytes(5, 10, 6, 1, keep) == 0     # nd > ns
+
+
+def test_tuning_switches_validate_without_device():
+    """rgf_debug_set_tuning returns the previous value for a known key and value and -1
+    (changing nothing) otherwise; it touches no device."""
+    lib = _lib.load()
+    for key, good, bad in ((1, (32, 64), (16, 48)), (3, (0, 1), (2, -1)), (4, (0, 1), (2, 7))):
+        cur = lib.rgf_debug_set_tuning(key, good[0])
+        assert cur in good
+        for v in bad:
+            assert lib.rgf_debug_set_tuning(key, v) == -1
+        assert lib.rgf_debug_set_tuning(key, good[1]) == good[0]
+        assert lib.rgf_debug_set_tuning(key, cur) == good[1]
+    for key in (0, 2, 5, 99):
+        assert lib.rgf_debug_set_tuning(key, 1) == -1
