@@ -1476,6 +1476,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         g_gemm_fast_loads = value;
         return old;
     }
+    if (key == 8 && (value == 0 || value == 1)) {
+        const int old = g_tri_fold;
+        g_tri_fold = value;
+        return old;
+    }
     return -1;
 }
 

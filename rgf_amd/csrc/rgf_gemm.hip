@@ -53,17 +53,13 @@ __global__ __launch_bounds__(256, gemm_occupancy(TM, TN)) void gemm_f64_kernel(G
     // remap the launch index to give each XCD a contiguous range of tiles (their
     // A row-panels and B column-panels then meet in that XCD's L2).
     int g = blockIdx.x;
-    {
+    if (D.tri_tiles != 2) {   // (2: tiles of unequal k-work stay dealt round-robin)
         const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
         g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
     }
     int ti, tj;
-    if (D.tri_tiles) {   // lower-triangular C: enumerate only the tiles on or below the diagonal
-        if (g >= D.tiles_m * (D.tiles_m + 1) / 2) return;
-        ti = (int)((sqrt(8.0 * g + 1.0) - 1.0) * 0.5);
-        while (ti * (ti + 1) / 2 > g) ti--;
-        while ((ti + 1) * (ti + 2) / 2 <= g) ti++;
-        tj = g - ti * (ti + 1) / 2;
+    if (D.tri_tiles) {   // lower-triangular C: only the tiles on or below the diagonal
+        if (!tri_tile(g, D.tiles_m, ti, tj)) return;
     } else {             // bands of 8 tile-rows, column-major inside a band
         if (g >= D.tiles_m * D.tiles_n) return;
         const int band = g / (8 * D.tiles_n);
@@ -268,6 +264,7 @@ static void launch_f64(Op ta, Op tb, const GemmArgs& args, dim3 grid, hipStream_
 }
 
 int g_gemm_fast_loads = 1;
+int g_tri_fold = 1;
 
 // Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, -1 = automatic.
 // Measured on MI355X at the sweep's shapes (ns = 2865, single and 4-matrix batches):
@@ -287,7 +284,7 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
         d.tiles_m = (d.m + TM - 1) / TM;
         d.tiles_n = (d.n + TN - 1) / TN;
-        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n && TM == TN;
+        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n && TM == TN ? tri_mode(d) : 0;
         d.zbase = z;
         z += d.count * d.count2;
         const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;

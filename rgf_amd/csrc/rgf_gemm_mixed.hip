@@ -59,17 +59,13 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3_kernel(G
     select_desc_m(args, z, D);
     const int flags = D.flags;
     int g = blockIdx.x;
-    {
+    if (D.tri_tiles != 2) {   // (2: tiles of unequal k-work stay dealt round-robin)
         const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
         g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
     }
     int ti, tj;
-    if (D.tri_tiles) {
-        if (g >= D.tiles_m * (D.tiles_m + 1) / 2) return;
-        ti = (int)((sqrt(8.0 * g + 1.0) - 1.0) * 0.5);
-        while (ti * (ti + 1) / 2 > g) ti--;
-        while ((ti + 1) * (ti + 2) / 2 <= g) ti++;
-        tj = g - ti * (ti + 1) / 2;
+    if (D.tri_tiles) {   // lower-triangular C: only the tiles on or below the diagonal
+        if (!tri_tile(g, D.tiles_m, ti, tj)) return;
     } else {
         if (g >= D.tiles_m * D.tiles_n) return;
         const int band = g / (8 * D.tiles_n);
@@ -251,7 +247,7 @@ int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
         d.tiles_m = (d.m + BT - 1) / BT;
         d.tiles_n = (d.n + BT - 1) / BT;
-        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n;
+        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n ? tri_mode(d) : 0;
         d.zbase = z;
         z += d.count * d.count2;
         const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
@@ -356,17 +352,13 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     select_desc_m(args, z, D);
     const int flags = D.flags;
     int g = blockIdx.x;
-    {
+    if (D.tri_tiles != 2) {   // (2: tiles of unequal k-work stay dealt round-robin)
         const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
         g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
     }
     int ti, tj;
-    if (D.tri_tiles) {
-        if (g >= D.tiles_m * (D.tiles_m + 1) / 2) return;
-        ti = (int)((sqrt(8.0 * g + 1.0) - 1.0) * 0.5);
-        while (ti * (ti + 1) / 2 > g) ti--;
-        while ((ti + 1) * (ti + 2) / 2 <= g) ti++;
-        tj = g - ti * (ti + 1) / 2;
+    if (D.tri_tiles) {   // lower-triangular C: only the tiles on or below the diagonal
+        if (!tri_tile(g, D.tiles_m, ti, tj)) return;
     } else {
         if (g >= D.tiles_m * D.tiles_n) return;
         const int band = g / (8 * D.tiles_n);
@@ -547,7 +539,7 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     GemmArgs args{};
     d.tiles_m = (d.m + BT - 1) / BT;
     d.tiles_n = (d.n + BT - 1) / BT;
-    d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n;
+    d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n ? tri_mode(d) : 0;
     d.zbase = 0;
     const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
     args.d[0] = d;

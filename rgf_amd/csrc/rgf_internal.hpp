@@ -45,6 +45,22 @@ struct GemmDesc {
     int tiles_m, tiles_n, zbase, tri_tiles;
 };
 
+// Lower-triangular tile enumeration (GemmDesc::tri_tiles != 0): tile g of the launch is
+// (ti, tj), row by row.  tri_tiles == 2 marks launches whose k-range starts at the tile's
+// row or column (triangular operands): a row's work then varies ~(ti+1)(T-ti), and the
+// kernels keep the hardware's round-robin dealing of workgroups over the XCDs instead of
+// giving each XCD a contiguous tile range (which would hand one XCD all the long rows:
+// 2.3x the mean work at T = 45; round-robin: 1.01x).
+__device__ __forceinline__ bool tri_tile(int g, int T, int& ti, int& tj)
+{
+    if (g >= T * (T + 1) / 2) return false;
+    ti = (int)((sqrt(8.0 * g + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > g) ti--;
+    while ((ti + 1) * (ti + 2) / 2 <= g) ti++;
+    tj = g - ti * (ti + 1) / 2;
+    return true;
+}
+
 constexpr int GEMM_MAXD = 4;
 struct GemmArgs {
     int nd;
@@ -60,6 +76,12 @@ enum class Op { N, T };
 // cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128; -1 automatic).
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
 extern int g_gemm_fast_loads;   // 1: GemmArgs::fast_loads set on FP64 launches
+extern int g_tri_fold;          // 1: triangular launches with k-ranges keep round-robin XCD dealing
+// tri_tiles value of a lower-stored launch of T x T tiles
+inline int tri_mode(const GemmDesc& d)
+{
+    return (d.flags & (G_ALOWER | G_AUPPER | G_BLOWER | G_BUPPER)) && g_tri_fold ? 2 : 1;
+}
 
 // Same interface on split-BF16 MFMA (three bf16 products per FP64 product, FP32
 // accumulation inside a call, FP64 alpha/beta epilogue).

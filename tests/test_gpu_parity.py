@@ -94,10 +94,11 @@ def test_gemm_fast_loads_match_masked_loads(gpu, ta, tb, tile):
 
 @pytest.mark.parametrize("flags", [1, 2, 8, 16, 1 | 4 | 8])
 @pytest.mark.parametrize("tile", [0, 2, 3])
-@pytest.mark.parametrize("size", [150, 100])
+@pytest.mark.parametrize("size", [150, 100, 333])
 def test_gemm_kernel_triangular_flags(gpu, flags, tile, size):
-    """Triangle flags (k-range skipping, lower-stored C) against numpy, at a size that is
-    and one that is not a multiple of the 4-wide MFMA k step."""
+    """Triangle flags (k-range skipping, lower-stored C) against numpy, at sizes that are
+    and are not a multiple of the 4-wide MFMA k step, up to 6 x 6 tiles (every lower tile
+    of a round-robin-dealt launch written once: the others would keep the fill 7.0)."""
     say(f"gemm tile={tile} flags={flags} size={size}")
     rng = np.random.default_rng(flags)
     m = n = k = size
