@@ -41,7 +41,8 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * (0 padded rows, 1 unpadded rows with XOR-swizzled 16-byte chunks); key 4 = FP64 GEMM loads of
  * full k-steps through loop-invariant lane offsets (1) or per-element masked loads (0); key 8 =
  * lower-stored products with triangular operands keep the round-robin dealing of tiles over
- * the XCDs (1) or take the XCD-contiguous tile ranges of the other launches (0).
+ * the XCDs (1) or take the XCD-contiguous tile ranges of the other launches (0); key 9 = the
+ * same choice for full-tile products with triangular operands.
  * Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */

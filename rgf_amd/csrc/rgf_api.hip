@@ -1477,8 +1477,13 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         return old;
     }
     if (key == 8 && (value == 0 || value == 1)) {
-        const int old = g_tri_fold;
-        g_tri_fold = value;
+        const int old = g_rr_tri;
+        g_rr_tri = value;
+        return old;
+    }
+    if (key == 9 && (value == 0 || value == 1)) {
+        const int old = g_rr_bands;
+        g_rr_bands = value;
         return old;
     }
     return -1;

@@ -51,9 +51,10 @@ __global__ __launch_bounds__(256, gemm_occupancy(TM, TN)) void gemm_f64_kernel(G
     const int flags = D.flags;
     // XCD-aware tile order: workgroups are dealt round-robin over the 8 XCDs, so
     // remap the launch index to give each XCD a contiguous range of tiles (their
-    // A row-panels and B column-panels then meet in that XCD's L2).
+    // A row-panels and B column-panels then meet in that XCD's L2) -- unless the
+    // launch's tiles have unequal k-work (deal_rr), where the round-robin balances it.
     int g = blockIdx.x;
-    if (D.tri_tiles != 2) {   // (2: tiles of unequal k-work stay dealt round-robin)
+    if (!D.deal_rr) {
         const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
         g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
     }
@@ -264,7 +265,8 @@ static void launch_f64(Op ta, Op tb, const GemmArgs& args, dim3 grid, hipStream_
 }
 
 int g_gemm_fast_loads = 1;
-int g_tri_fold = 1;
+int g_rr_tri = 1;
+int g_rr_bands = 0;
 
 // Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, -1 = automatic.
 // Measured on MI355X at the sweep's shapes (ns = 2865, single and 4-matrix batches):
@@ -284,7 +286,7 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
         d.tiles_m = (d.m + TM - 1) / TM;
         d.tiles_n = (d.n + TN - 1) / TN;
-        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n && TM == TN ? tri_mode(d) : 0;
+        set_tile_order(d, (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n && TM == TN);
         d.zbase = z;
         z += d.count * d.count2;
         const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3_kernel(G
     select_desc_m(args, z, D);
     const int flags = D.flags;
     int g = blockIdx.x;
-    if (D.tri_tiles != 2) {   // (2: tiles of unequal k-work stay dealt round-robin)
+    if (!D.deal_rr) {
         const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
         g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
     }
@@ -247,7 +247,7 @@ int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
         if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
         d.tiles_m = (d.m + BT - 1) / BT;
         d.tiles_n = (d.n + BT - 1) / BT;
-        d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n ? tri_mode(d) : 0;
+        set_tile_order(d, (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n);
         d.zbase = z;
         z += d.count * d.count2;
         const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     select_desc_m(args, z, D);
     const int flags = D.flags;
     int g = blockIdx.x;
-    if (D.tri_tiles != 2) {   // (2: tiles of unequal k-work stay dealt round-robin)
+    if (!D.deal_rr) {
         const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
         g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
     }
@@ -539,7 +539,7 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     GemmArgs args{};
     d.tiles_m = (d.m + BT - 1) / BT;
     d.tiles_n = (d.n + BT - 1) / BT;
-    d.tri_tiles = (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n ? tri_mode(d) : 0;
+    set_tile_order(d, (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n);
     d.zbase = 0;
     const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
     args.d[0] = d;

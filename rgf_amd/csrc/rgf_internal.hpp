@@ -43,14 +43,16 @@ struct GemmDesc {
     double alpha, beta;
     // filled by the launcher
     int tiles_m, tiles_n, zbase, tri_tiles;
+    int deal_rr;                  // 1: tiles dealt round-robin over the XCDs (no contiguous ranges)
+    int pad_;
 };
 
-// Lower-triangular tile enumeration (GemmDesc::tri_tiles != 0): tile g of the launch is
-// (ti, tj), row by row.  tri_tiles == 2 marks launches whose k-range starts at the tile's
-// row or column (triangular operands): a row's work then varies ~(ti+1)(T-ti), and the
-// kernels keep the hardware's round-robin dealing of workgroups over the XCDs instead of
-// giving each XCD a contiguous tile range (which would hand one XCD all the long rows:
-// 2.3x the mean work at T = 45; round-robin: 1.01x).
+// Lower-triangular tile enumeration (GemmDesc::tri_tiles): tile g of the launch is (ti, tj),
+// row by row.  Launches whose k-range depends on the tile (triangular operands) have tiles
+// of unequal work; with deal_rr they keep the hardware's round-robin dealing of workgroups
+// over the XCDs instead of giving each XCD a contiguous tile range.  For the lower-stored
+// product with an upper op(A) (k from the tile row on) contiguous ranges give one XCD all
+// the long rows: 2.3x the mean work at 45 tile rows; round-robin: 1.01x.
 __device__ __forceinline__ bool tri_tile(int g, int T, int& ti, int& tj)
 {
     if (g >= T * (T + 1) / 2) return false;
@@ -76,11 +78,14 @@ enum class Op { N, T };
 // cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128; -1 automatic).
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
 extern int g_gemm_fast_loads;   // 1: GemmArgs::fast_loads set on FP64 launches
-extern int g_tri_fold;          // 1: triangular launches with k-ranges keep round-robin XCD dealing
-// tri_tiles value of a lower-stored launch of T x T tiles
-inline int tri_mode(const GemmDesc& d)
+extern int g_rr_tri;     // 1: lower-stored launches with triangular operands dealt round-robin
+extern int g_rr_bands;   // 1: the same for full-tile launches with triangular operands
+// tile order of a launch: triangular enumeration for lower-stored square tilings, and
+// round-robin XCD dealing where the k-range depends on the tile
+inline void set_tile_order(GemmDesc& d, bool tri)
 {
-    return (d.flags & (G_ALOWER | G_AUPPER | G_BLOWER | G_BUPPER)) && g_tri_fold ? 2 : 1;
+    d.tri_tiles = tri ? 1 : 0;
+    d.deal_rr = (d.flags & (G_ALOWER | G_AUPPER | G_BLOWER | G_BUPPER)) && (tri ? g_rr_tri : g_rr_bands) ? 1 : 0;
 }
 
 // Same interface on split-BF16 MFMA (three bf16 products per FP64 product, FP32
