@@ -42,7 +42,9 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * full k-steps through loop-invariant lane offsets (1) or per-element masked loads (0); key 8 =
  * lower-stored products with triangular operands keep the round-robin dealing of tiles over
  * the XCDs (1) or take the XCD-contiguous tile ranges of the other launches (0); key 9 = the
- * same choice for full-tile products with triangular operands.
+ * same choice for full-tile products with triangular operands; key 11 = the FP64 selected
+ * inversion launches X_i^T X_i and W^T Z as one launch of two products (1) or one after the
+ * other (0).
  * Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */

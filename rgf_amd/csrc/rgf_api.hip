@@ -768,6 +768,7 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
         const int r0 = last ? ns : 0;
         double* Sc = h->Sb[cur];
         double* Sn = h->Sb[cur ^ 1];
+        const double* xsum = nullptr;   // X^T X launched with W^T Z: added by the finishing pass
         GemmDesc dx = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
         batch2(dx, B, ms, ms, ss);
         const bool px = presplit_ok(h, dx);
@@ -791,16 +792,26 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
             }
+            // FP64: X^T X and W^T Z in one launch (two descriptors, twice the tiles of either);
+            // X^T X goes to the top-left ns x ns of Sc, dead once Z = Sc W is done, and the
+            // finishing pass adds it into Sn (the same sum as accumulating it, same bits)
+            const bool fuse = h->opt.precision == RGF_FP64 && g_selinv_fuse;
             if (px) {
                 if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &dx, h->split_x, h->split_x, s))) return rc;
-            } else if ((rc = prod(h, Op::T, Op::N, &dx, 1))) {
+            } else if (!fuse && (rc = prod(h, Op::T, Op::N, &dx, 1))) {
                 return rc;
             }
-            d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 1.0, G_CLOWER);
+            d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, fuse ? 0.0 : 1.0, G_CLOWER);
             batch2(d, B, fs, fs, ss);
             if (pw) {   // op(A) = W^T: the split of Z = Sb W's op(B)
                 if ((rc = split_b(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::T, Op::N, &d, rW, rT, s)))
                     return rc;
+            } else if (fuse) {
+                GemmDesc two[2] = {dx, d};
+                two[0].C = Sc;
+                two[0].Cin = Sc;
+                if ((rc = prod(h, Op::T, Op::N, two, 2))) return rc;
+                xsum = Sc;
             } else if ((rc = prod(h, Op::T, Op::N, &d, 1))) {
                 return rc;
             }
@@ -810,7 +821,7 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             return rc;
         }
         if ((rc = selinv_finish(Sn, lds, ss, ns, nd, h->Z, lds, fs, mF - nd, exact && mF > 0,
-                                diag_out + (long long)i * ns, n, B, s)))
+                                diag_out + (long long)i * ns, n, B, s, xsum, lds, ss)))
             return rc;
         if (so) {
             const double* Sm = Sn + so->mb * ss;
@@ -1479,6 +1490,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
     if (key == 8 && (value == 0 || value == 1)) {
         const int old = g_rr_tri;
         g_rr_tri = value;
+        return old;
+    }
+    if (key == 11 && (value == 0 || value == 1)) {
+        const int old = g_selinv_fuse;
+        g_selinv_fuse = value;
         return old;
     }
     if (key == 9 && (value == 0 || value == 1)) {

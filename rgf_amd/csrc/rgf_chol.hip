@@ -594,11 +594,14 @@ int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s, in
 
 // mirror the lower triangle into the upper one through 32x32 LDS tiles; optional
 // diagonal gather.
+// With X: the lower triangle of A first receives A + X (the workgroup owning a lower tile
+// reads, sums and writes it back before mirroring it; its diagonal comes from the sum).
 __global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, long long sA, int n, int T, double* diag,
-                                                     long long sDg)
+                                                     long long sDg, const double* X, int ldx, long long sX)
 {
     A += (long long)blockIdx.y * sA;
     if (diag) diag += (long long)blockIdx.y * sDg;
+    if (X) X += (long long)blockIdx.y * sX;
     __shared__ double t[32][33];
     const int bi = blockIdx.x % T, bj = blockIdx.x / T;   // destination tile (bi row, bj col), bi <= bj
     if (bi > bj) return;
@@ -606,21 +609,27 @@ __global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, long lo
     // source: tile (bj row-block, bi col-block) of the lower triangle
     for (int q = ty; q < 32; q += 8) {
         int r = bj * 32 + tx, c = bi * 32 + q;
-        t[q][tx] = (r < n && c < n) ? A[(long long)c * lda + r] : 0.0;   // t[c_local][r_local]
+        double v = (r < n && c < n) ? A[(long long)c * lda + r] : 0.0;
+        if (X && r < n && c < n && r >= c) {
+            v += X[(long long)c * ldx + r];
+            A[(long long)c * lda + r] = v;
+        }
+        t[q][tx] = v;   // t[c_local][r_local]
     }
     __syncthreads();
     for (int q = ty; q < 32; q += 8) {
         int r = bi * 32 + tx, c = bj * 32 + q;   // destination (r, c) = source (c, r)
         if (r < n && c < n && r < c) A[(long long)c * lda + r] = t[tx][q];
-        if (diag && r < n && r == c) diag[r] = A[(long long)c * lda + r];
+        if (diag && r < n && r == c) diag[r] = t[tx][q];
     }
 }
 
-int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s)
+int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s,
+                      const double* X, int ldx, long long sX)
 {
     if (n <= 0 || count <= 0) return 0;
     const int T = (n + 31) / 32;
-    hipLaunchKernelGGL(mirror_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, diag, sDg);
+    hipLaunchKernelGGL(mirror_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, diag, sDg, X, ldx, sX);
     return (int)hipGetLastError();
 }
 
@@ -657,9 +666,10 @@ __global__ void selinv_offdiag_kernel(double* S, int lds, long long sS, int ns, 
 }
 
 int selinv_finish(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz, long long sZ,
-                  int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s)
+                  int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s, const double* X,
+                  int ldx, long long sX)
 {
-    int rc = mirror_lower_diag(S, lds, sS, ns, diag_out, sDg, count, s);
+    int rc = mirror_lower_diag(S, lds, sS, ns, diag_out, sDg, count, s, X, ldx, sX);
     if (rc) return rc;
     if (exact && nd > 0) {
         hipLaunchKernelGGL(selinv_offdiag_kernel, dim3((ns + 255) / 256, count), dim3(256), 0, s, S, lds, sS, ns,

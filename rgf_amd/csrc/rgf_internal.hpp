@@ -78,6 +78,7 @@ enum class Op { N, T };
 // cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128; -1 automatic).
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
 extern int g_gemm_fast_loads;   // 1: GemmArgs::fast_loads set on FP64 launches
+extern int g_selinv_fuse;   // 1: the FP64 selected inversion launches X^T X and W^T Z together
 extern int g_rr_tri;     // 1: lower-stored launches with triangular operands dealt round-robin
 extern int g_rr_bands;   // 1: the same for full-tile launches with triangular operands
 // tile order of a launch: triangular enumeration for lower-stored square tilings, and
@@ -195,14 +196,19 @@ double gen_uniform_host(uint64_t seed, uint32_t mid, uint32_t idx);
 int scatter_csc(const uint64_t* ia, const uint64_t* ja, const double* a, long long nnz, int ns,
                 int nt, int nd, double* P, long long pstride, int ldp, double* Cdd, hipStream_t s);
 int mirror_lower(double* A, int lda, int n, hipStream_t s);
-int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s);
+// Mirror the lower triangle of A into its upper one (and gather the diagonal); with X the
+// lower triangle first receives A + X.
+int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s,
+                      const double* X = nullptr, int ldx = 0, long long sX = 0);
 
 // ---------------------------------------------------------------- selinv helpers
-// Finish one Takahashi step: mirror the lower triangle of S (ns x ns, ld lds) into
-// its upper triangle, write diag(S) to diag_out, and (exact mode) store
-// S_{d,i} = -Z[zrow0:zrow0+nd, :] into S[ns:ns+nd, 0:ns] and its transpose.
+// Finish one Takahashi step: (with X) add the lower triangle of X into that of S, mirror
+// the lower triangle of S (ns x ns, ld lds) into its upper triangle, write diag(S) to
+// diag_out, and (exact mode) store S_{d,i} = -Z[zrow0:zrow0+nd, :] into S[ns:ns+nd, 0:ns]
+// and its transpose.
 int selinv_finish(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz, long long sZ,
-                  int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s);
+                  int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s,
+                  const double* X = nullptr, int ldx = 0, long long sX = 0);
 int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s, int count = 1, long long sA = 0,
                  long long sO = 0);
 
