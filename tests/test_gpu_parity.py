@@ -670,3 +670,33 @@ def test_graph_replays_match_eager(gpu, prec):
         buf.free()
         eager.close()
         graph.close()
+
+
+# ------------------------------------------------------------------ variant switches keep the bits
+@pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
+@pytest.mark.parametrize("key,alt", [(4, 0), (8, 0), (11, 0)])
+def test_sweep_variant_switches_keep_bits(gpu, prec, key, alt):
+    """The adopted kernel variants change how the same products are loaded, dealt over
+    the XCDs or launched together, not what is summed: logdet and diag(A^-1) of a batch and
+    of a single matrix are bitwise those of the alternative (key 4 masked loads, 8 tile
+    ranges for triangular products, 11 separate launches of X^T X and W^T Z)."""
+    say(f"switch {key}={alt} prec={prec}")
+    lib = _lib.load()
+    opts = rgf_amd.default_options(precision=prec)
+
+    def run():
+        with rgf_amd.RGF.generated(330, 4, 5, seed=21, count=3, theta_mode=1, options=opts) as hb:
+            lb, db = hb.logdet_batch(), hb.selinv_diag_batch()
+        with rgf_amd.RGF.generated(200, 5, 0, seed=22, options=opts) as h1:
+            l1, d1 = h1.logDet(), h1.RGFdiag()
+        return lb, db, l1, d1
+
+    ref = run()
+    old = lib.rgf_debug_set_tuning(key, alt)
+    assert old >= 0
+    try:
+        got = run()
+    finally:
+        lib.rgf_debug_set_tuning(key, old)
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+    assert ref[2] == got[2] and np.array_equal(ref[3], got[3])
