@@ -44,7 +44,8 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * the XCDs (1) or take the XCD-contiguous tile ranges of the other launches (0); key 9 = the
  * same choice for full-tile products with triangular operands; key 11 = the FP64 selected
  * inversion launches X_i^T X_i and W^T Z as one launch of two products (1) or one after the
- * other (0).
+ * other (0); key 12 = with key 11 on, the FP64 sweep also computes W = F X of the next block
+ * in the launch of this block's Z = S W (1) or in a launch of its own (0).
  * Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */

@@ -760,6 +760,23 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
     __bf16* const rT = h->split;
     __bf16* const rW = h->split ? h->split + (size_t)B * split_entry_elems(m, m) : nullptr;
     int cur = 0;
+    // FP64 pipeline: W_j = F_j X_j needs only the factor, so W of the next block is
+    // computed in the launch of this block's Z = Sc W (two descriptors), alternating
+    // between h->W and Ftmp (idle outside the factor sweep)
+    const bool pipe = h->opt.precision == RGF_FP64 && g_selinv_fuse && g_selinv_pipe;
+    double* const wb[2] = {h->W, h->Ftmp};
+    auto wdesc = [&](int j) {
+        const double* Pj = l_panel(h, j);
+        const bool lj = j == nt - 1;
+        GemmDesc d = gdesc(lj ? Pj + 2 * ns : Pj + ns, ldp, Pj, ldp, wb[j & 1], lds, lj ? nd : m, ns, ns, 1.0, 0.0,
+                           G_BLOWER);
+        batch2(d, B, ms, ms, fs);
+        return d;
+    };
+    if (pipe && nd > 0) {
+        GemmDesc d = wdesc(nt - 1);
+        if ((rc = prod(h, Op::N, Op::N, &d, 1))) return rc;
+    }
     for (int i = nt - 1; i >= 0; i--) {
         const double* P = l_panel(h, i);
         const bool last = i == nt - 1;
@@ -773,7 +790,28 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
         batch2(dx, B, ms, ms, ss);
         const bool px = presplit_ok(h, dx);
         if (px && (rc = split_a(h, Op::T, dx, h->split_x))) return rc;
-        if (mF > 0) {
+        if (pipe) {
+            double* const Wi = wb[i & 1];
+            GemmDesc zz[2];
+            int nz = 0;
+            if (mF > 0) {
+                zz[nz] = gdesc(Sc + (long long)r0 * lds + r0, lds, Wi, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
+                batch2(zz[nz++], B, ss, fs, fs);
+            }
+            if (i > 0) zz[nz++] = wdesc(i - 1);   // into wb[(i - 1) & 1], free since W^T Z of block i + 1
+            if (nz && (rc = prod(h, Op::N, Op::N, zz, nz))) return rc;
+            if (mF > 0) {
+                GemmDesc d = gdesc(Wi, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 0.0, G_CLOWER);
+                batch2(d, B, fs, fs, ss);
+                GemmDesc two[2] = {dx, d};
+                two[0].C = Sc;
+                two[0].Cin = Sc;
+                if ((rc = prod(h, Op::T, Op::N, two, 2))) return rc;
+                xsum = Sc;
+            } else if ((rc = prod(h, Op::T, Op::N, &dx, 1))) {
+                return rc;
+            }
+        } else if (mF > 0) {
             GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
             batch2(d, B, ms, ms, fs);
             const bool pw = px && presplit_ok(h, d);
@@ -1490,6 +1528,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
     if (key == 8 && (value == 0 || value == 1)) {
         const int old = g_rr_tri;
         g_rr_tri = value;
+        return old;
+    }
+    if (key == 12 && (value == 0 || value == 1)) {
+        const int old = g_selinv_pipe;
+        g_selinv_pipe = value;
         return old;
     }
     if (key == 11 && (value == 0 || value == 1)) {

@@ -267,6 +267,7 @@ static void launch_f64(Op ta, Op tb, const GemmArgs& args, dim3 grid, hipStream_
 int g_gemm_fast_loads = 1;
 int g_rr_tri = 1;
 int g_selinv_fuse = 1;   // used by the selected-inversion sweep
+int g_selinv_pipe = 1;   // used by the selected-inversion sweep
 int g_rr_bands = 0;
 
 // Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, -1 = automatic.

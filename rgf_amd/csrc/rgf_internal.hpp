@@ -79,6 +79,7 @@ enum class Op { N, T };
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
 extern int g_gemm_fast_loads;   // 1: GemmArgs::fast_loads set on FP64 launches
 extern int g_selinv_fuse;   // 1: the FP64 selected inversion launches X^T X and W^T Z together
+extern int g_selinv_pipe;   // 1: ... and computes W of the next block in the launch of Z
 extern int g_rr_tri;     // 1: lower-stored launches with triangular operands dealt round-robin
 extern int g_rr_bands;   // 1: the same for full-tile launches with triangular operands
 // tile order of a launch: triangular enumeration for lower-stored square tilings, and

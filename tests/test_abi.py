@@ -85,7 +85,8 @@ def test_tuning_switches_validate_without_device():
     (changing nothing) otherwise; it touches no device."""
     lib = _lib.load()
     for key, good, bad in ((1, (32, 64), (16, 48)), (3, (0, 1), (2, -1)), (4, (0, 1), (2, 7)), (8, (0, 1), (2, 3)),
-                           (9, (0, 1), (2, -1)), (11, (0, 1), (2, 3))):
+                           (9, (0, 1), (2, -1)), (11, (0, 1), (2, 3)),
+                           (12, (0, 1), (2, 9))):
         cur = lib.rgf_debug_set_tuning(key, good[0])
         assert cur in good
         for v in bad:
