@@ -128,7 +128,7 @@ def test_gemm_kernel_triangular_flags(gpu, flags, tile, size):
     assert np.abs(Cf.reshape(m, n, order="F") - ref).max() < 1e-11
 
 
-@pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 333])
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 150, 200, 300, 333])
 def test_potrf_trtri_kernels(gpu, n):
     say(f"potrf/trtri n={n}")
     rng = np.random.default_rng(n)
