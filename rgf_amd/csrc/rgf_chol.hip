@@ -77,88 +77,38 @@ __device__ __forceinline__ void mfma_nt(v4d (&acc)[RA][2], const double (*X)[LP]
 // of a pass, on the launch's critical path, is half that of a 64-row block)
 constexpr int RB = 32;
 
-// Column block k0 of the blocked Cholesky, one launch: tile factor and inverse, then the
-// panel rows.  With prev_update the update of this column block by the previous panel
-// (columns k0-64..k0-1), which the trailing update left out, is applied first: to the
-// tile (A_kk -= P_k P_k^T) and to each row block (A_Rk -= P_R P_k^T).
-__global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, long long sA, int k0, int kb,
-                                                         int below, bool prev_update, double* xout, long long sX,
-                                                         double* logdet_slot, long long sP, int* info, int info_base,
-                                                         bool store_l)
+// ---- phases of the Cholesky tile kernels (called by all 256 lanes of the workgroup)
+// Ls -= P P^T on the lower quarters of the 64 x 64 tile (rows >= kb of P are zero).
+__device__ __forceinline__ void tile_sub_panel(double (*Ls)[LP], const double (*Pk)[LP], int wave, int l16, int lq)
 {
-    {
-        const long long b = blockIdx.y;   // batch entry
-        A += b * sA;
-        xout += b * sX;
-        logdet_slot += b * sP;
-        info += b;
-    }
-    const bool lead = blockIdx.x == 0;   // writes X_kk, the log-determinant part and info
-    // One LDS pool, the per-step broadcast arrays at its start: below 64 KB their
-    // addresses fit the instructions' immediate offsets (no address register set up per
-    // access in the pivot chain).
-    constexpr int TSZ = 2 * (NB / 2) * (NB / 2 + 2);
-    __shared__ __attribute__((aligned(16))) double pool[3 * NB + 4 * NB * LP + TSZ];
-    double* const colb = pool;
-    double* const rinv = pool + NB;
-    double* const dgl = pool + 2 * NB;
-    double (*const Ls)[LP] = reinterpret_cast<double (*)[LP]>(pool + 3 * NB);
-    double (*const Xs)[LP] = Ls + NB;
-    double (*const Pk)[LP] = Ls + 2 * NB;   // previous panel, rows of this tile
-    double (*const Pr)[LP] = Ls + 3 * NB;   // previous panel, rows of a row block
-    double (*const Ts)[NB / 2][NB / 2 + 2] =
-        reinterpret_cast<double (*)[NB / 2][NB / 2 + 2]>(pool + 3 * NB + 4 * NB * LP);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int l16 = lane & 15, lq = lane >> 4;
-    int bad = INT_MAX;   // first non-positive pivot (wave 0, uniform)
-    {
-        // all 16 loads of a lane in flight at once (masked lanes read the tile's corner)
-        double v[NB * NB / 256];
+    const int wm = wave & 1, wn = wave >> 1;
+    if (wn <= wm) {
+        v4d acc[2][2];
 #pragma unroll
-        for (int q = 0; q < NB * NB / 256; q++) {
-            const int r = lane, c = 4 * q + wave;
-            const bool in = r < kb && c < kb;
-            v[q] = A[in ? (long long)(k0 + c) * lda + (k0 + r) : (long long)k0 * lda + k0];
-        }
-        double w[NB * NB / 256];
-        if (prev_update) {
-            const double* P = A + (long long)(k0 - NB) * lda;
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int q = 0; q < NB * NB / 256; q++) {
-                const int r = lane, c = 4 * q + wave;
-                w[q] = P[r < kb ? (long long)c * lda + (k0 + r) : (long long)k0];
-            }
-        }
+            for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int q = 0; q < NB * NB / 256; q++) {
-            const int r = lane, c = 4 * q + wave;
-            Ls[r][c] = (r < kb && c < kb) ? (c <= r ? v[q] : 0.0) : (r == c ? 1.0 : 0.0);
-            Xs[r][c] = 0.0;
-            if (prev_update) Pk[r][c] = r < kb ? w[q] : 0.0;
-        }
+                for (int t = 0; t < 4; t++) acc[a][b][t] = Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
+        mfma_nt<2>(acc, Pk, Pk, -1.0, wm, wn, l16, lq);
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
     }
     __syncthreads();
-    if (prev_update) {   // A_kk -= P_k P_k^T (lower quarters; rows >= kb of P_k are zero)
-        const int wm = wave & 1, wn = wave >> 1;
-        if (wn <= wm) {
-            v4d acc[2][2];
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-#pragma unroll
-                    for (int t = 0; t < 4; t++) acc[a][b][t] = Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
-            mfma_nt<2>(acc, Pk, Pk, -1.0, wm, wn, l16, lq);
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-#pragma unroll
-                    for (int t = 0; t < 4; t++) Ls[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
-        }
-        __syncthreads();
-    }
+}
 
+// Cholesky of the tile in Ls (identity-padded beyond kb) in four 16-column panels; each
+// row's 1/sqrt and sqrt of its pivot go to rinv and dgl; bad receives the first
+// non-positive pivot (wave 0).
+__device__ __forceinline__ void tile_factor(double (*Ls)[LP], double* colb, double* rinv, double* dgl, int kb,
+                                            int& bad, int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
     // ---- factor
 #pragma unroll
     for (int K = 0; K < 4; K++) {
@@ -262,16 +212,32 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
         }
         __syncthreads();
     }
+}
+
+// Sum of log L_jj of the tile (fixed shuffle tree) into *logdet_slot and the first bad
+// pivot (1-based, offset info_base) into *info, by the lead workgroup.
+__device__ __forceinline__ void tile_logdet(const double* dgl, int kb, int bad, bool lead, double* logdet_slot,
+                                            int* info, int info_base, int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
     if (wave == 0) {
         double v = lane < kb ? log(dgl[lane]) : 0.0;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
         if (lane == 0 && lead) {
             *logdet_slot = v;
-            if (bad != INT_MAX) atomicMin(info, info_base + k0 + bad + 1);
+            if (bad != INT_MAX) atomicMin(info, info_base + bad + 1);
         }
     }
 
+}
+
+// X = L^-1 of the factored tile in Ls into Xs (Ts: merge scratch).
+__device__ __forceinline__ void tile_inverse(const double (*Ls)[LP], double (*Xs)[LP],
+                                             double (*Ts)[NB / 2][NB / 2 + 2], const double* rinv, int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
     // ---- inverse, level 0: wave w inverts diagonal block w (lane c < 16 owns column c)
     if (lane < 16) {
         const int o = 16 * wave, c = lane;
@@ -331,6 +297,74 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
         for (int t = 0; t < 4; t++) Xs[32 + bi + l16][bj + 4 * t + lq] = -acc[t];
     }
     __syncthreads();
+}
+
+// Column block k0 of the blocked Cholesky, one launch: tile factor and inverse, then the
+// panel rows.  With prev_update the update of this column block by the previous panel
+// (columns k0-64..k0-1), which the trailing update left out, is applied first: to the
+// tile (A_kk -= P_k P_k^T) and to each row block (A_Rk -= P_R P_k^T).
+__global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, long long sA, int k0, int kb,
+                                                         int below, bool prev_update, double* xout, long long sX,
+                                                         double* logdet_slot, long long sP, int* info, int info_base,
+                                                         bool store_l)
+{
+    {
+        const long long b = blockIdx.y;   // batch entry
+        A += b * sA;
+        xout += b * sX;
+        logdet_slot += b * sP;
+        info += b;
+    }
+    const bool lead = blockIdx.x == 0;   // writes X_kk, the log-determinant part and info
+    // One LDS pool, the per-step broadcast arrays at its start: below 64 KB their
+    // addresses fit the instructions' immediate offsets (no address register set up per
+    // access in the pivot chain).
+    constexpr int TSZ = 2 * (NB / 2) * (NB / 2 + 2);
+    __shared__ __attribute__((aligned(16))) double pool[3 * NB + 4 * NB * LP + TSZ];
+    double* const colb = pool;
+    double* const rinv = pool + NB;
+    double* const dgl = pool + 2 * NB;
+    double (*const Ls)[LP] = reinterpret_cast<double (*)[LP]>(pool + 3 * NB);
+    double (*const Xs)[LP] = Ls + NB;
+    double (*const Pk)[LP] = Ls + 2 * NB;   // previous panel, rows of this tile
+    double (*const Pr)[LP] = Ls + 3 * NB;   // previous panel, rows of a row block
+    double (*const Ts)[NB / 2][NB / 2 + 2] =
+        reinterpret_cast<double (*)[NB / 2][NB / 2 + 2]>(pool + 3 * NB + 4 * NB * LP);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    int bad = INT_MAX;   // first non-positive pivot (wave 0, uniform)
+    {
+        // all 16 loads of a lane in flight at once (masked lanes read the tile's corner)
+        double v[NB * NB / 256];
+#pragma unroll
+        for (int q = 0; q < NB * NB / 256; q++) {
+            const int r = lane, c = 4 * q + wave;
+            const bool in = r < kb && c < kb;
+            v[q] = A[in ? (long long)(k0 + c) * lda + (k0 + r) : (long long)k0 * lda + k0];
+        }
+        double w[NB * NB / 256];
+        if (prev_update) {
+            const double* P = A + (long long)(k0 - NB) * lda;
+#pragma unroll
+            for (int q = 0; q < NB * NB / 256; q++) {
+                const int r = lane, c = 4 * q + wave;
+                w[q] = P[r < kb ? (long long)c * lda + (k0 + r) : (long long)k0];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NB * NB / 256; q++) {
+            const int r = lane, c = 4 * q + wave;
+            Ls[r][c] = (r < kb && c < kb) ? (c <= r ? v[q] : 0.0) : (r == c ? 1.0 : 0.0);
+            Xs[r][c] = 0.0;
+            if (prev_update) Pk[r][c] = r < kb ? w[q] : 0.0;
+        }
+    }
+    __syncthreads();
+    if (prev_update) tile_sub_panel(Ls, Pk, wave, l16, lq);
+
+    tile_factor(Ls, colb, rinv, dgl, kb, bad, tid);
+    tile_logdet(dgl, kb, bad, lead, logdet_slot, info, info_base + k0, tid);
+    tile_inverse(Ls, Xs, Ts, rinv, tid);
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e & (NB - 1), c = e >> 6;
         if (lead) {
@@ -411,6 +445,7 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
         const int below = n - k0 - kb;
         const bool odd = p & 1;
         double* xt = xdiag + (long long)p * NB * NB;
+
         // every workgroup of a matrix factors the tile (in LDS) and then produces its
         // share of the RB-row blocks of the panel below
         const int nrb = (below + RB - 1) / RB;
