@@ -28,6 +28,7 @@ WORKLOADS = {
     "c0": (42, 3, 0),
 }
 PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+PEAK_BF16_MFMA = 2500.0  # TFLOP/s, MI355X dense BF16 matrix (no sparsity)
 PEAK_HBM = 8000.0       # GB/s
 
 
@@ -50,7 +51,9 @@ def parse():
     p.add_argument("--hw-queues", type=int, default=16,
                    help="c4: GPU_MAX_HW_QUEUES for the serving process (1..32; HIP's default is 4)")
     p.add_argument("--budget-gb", type=float, default=160.0,
-                   help="serving: HBM budget of live handles (the device block cache takes up to 40%% more)")
+                   help="serving: HBM budget of live handles (the arena keeps up to --cache-gb more)")
+    p.add_argument("--cache-gb", type=float, default=115.0,
+                   help="serving: idle device memory the library's arena keeps for reuse (GiB)")
     p.add_argument("--c3", action="store_true", help="c3: 64 matrices (1500, 60, 5), per-instance theta")
     p.add_argument("--only-batched", choices=["fp64", "mixed"], default=None,
                    help="run only the batched measurement of one precision (for profiling)")
@@ -122,6 +125,18 @@ def cpu_baseline(ns, nt, nd, seed, blocks):
     dt, tf = run(nts, threads)
     nts1 = max(2, nts // 8)
     dt1, tf1 = run(nts1, 1)
+    # the two smaller workloads in full (every block), same threads
+    full = {}
+    for name in ("c0", "c1"):
+        fns, fnt, fnd = WORKLOADS[name]
+        env = dict(os.environ, MKL_NUM_THREADS=str(threads), OMP_NUM_THREADS=str(threads))
+        code = (f"import sys; sys.path.insert(0, {ROOT!r}); import bench; "
+                f"bench._cpu_sample({fns}, {fnt}, {fnd}, {seed})")
+        out = subprocess.run([sys.executable, "-c", code], env=env, check=True, capture_output=True, text=True)
+        fdt = json.loads(out.stdout.strip().splitlines()[-1])["s"]
+        fF, fS = rgf_amd.canonical_flops(fns, fnt, fnd)
+        full[name] = {"ns": fns, "nt": fnt, "nd": fnd, "s_per_matrix": round(fdt, 5),
+                      "TFLOPs_canonical": round((fF + fS) / fdt / 1e12, 4)}
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -139,6 +154,7 @@ def cpu_baseline(ns, nt, nd, seed, blocks):
         "s_per_matrix_extrapolated": round(dt / nts * nt, 3),
         "one_thread": {"value": round(tf1, 4), "blocks": nts1, "s": round(dt1, 2),
                        "s_per_matrix_extrapolated": round(dt1 / nts1 * nt, 2)},
+        "full_workloads": full,
         "host": {"nproc": os.cpu_count(), "model": model},
     }
 
@@ -191,7 +207,8 @@ def run_serve(args):
 
     for name, prec in (("fp64", rgf_amd.RGF_FP64), ("mixed_bf16x3", rgf_amd.RGF_MIXED_BF16X3)):
         reqs = serve.poisson_workload(args.requests, args.rate, seed=args.seed, followup_frac=0.1)
-        srv = serve.Server(serve.GPUExecutor(prec), slots=args.slots, budget_bytes=int(args.budget_gb * (1 << 30)),
+        srv = serve.Server(serve.GPUExecutor(prec, device_cache_gb=args.cache_gb), slots=args.slots,
+                           budget_bytes=int(args.budget_gb * (1 << 30)),
                            progress=lambda d, t, nm=name: print(f"serve {nm}: {d}/{t} done", file=sys.stderr,
                                                                 flush=True))
         c0 = cache_stats()
@@ -343,6 +360,12 @@ def main():
     ph["solve_1rhs_GBps"] = info.bytes_solve_per_rhs / (ph["solve_1rhs_ms"] * 1e-3) / 1e9
     ph["factorize_TFLOPs"] = F / (ph["factorize_ms"] * 1e-3) / 1e12
     ph["selinv_TFLOPs"] = S / (ph["selinv_ms"] * 1e-3) / 1e12
+    # canonical F and S put the triangular inverses (trtri, nt ns^3/3 + nd^3/3) in the
+    # selected inversion; here they run inside factorize (the factor stores L_ii^-1), so
+    # the per-phase rates of the work as it runs move that term across
+    trtri = nt * ns ** 3 / 3.0 + nd ** 3 / 3.0
+    ph["factorize_TFLOPs_as_run"] = (F + trtri) / (ph["factorize_ms"] * 1e-3) / 1e12
+    ph["selinv_TFLOPs_as_run"] = (S - trtri) / (ph["selinv_ms"] * 1e-3) / 1e12
     ph = {k: round(v, 4) for k, v in ph.items()}
     x64 = x.download()
     fp64_diag = diag.download()
@@ -389,6 +412,26 @@ def main():
             m.factorize(); ldm = m.logDet(); m.selinv_diag_device(md)
         m.synchronize()
         dtm = (time.perf_counter() - t0) / args.steps
+        # split-BF16 products of one extra (untimed) step, HIP events around each launch on
+        # the library's stream (an eager handle: events are not read back from graph replays)
+        bf = None
+        if not args.no_kernel_timing:
+            me = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
+                                       options=rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3))
+            me.factorize(); me.selinv_diag_device(md); me.synchronize()
+            rgf_amd.kernel_timing_enable(8)
+            me.factorize(); me.logDet(); me.selinv_diag_device(md); me.synchronize()
+            bms, bfl, bn = rgf_amd.kernel_timing_read()
+            rgf_amd.kernel_timing_enable(-1)
+            me.close()
+            if bms > 0:
+                bf = {"kernel": "gemm_bf16x3s_kernel / gemm_bf16x3_kernel (split-BF16 products, all transposes)",
+                      "bound": "mfma", "unit": "TFLOP/s",
+                      # three bf16 MFMA products (hi*hi, hi*lo, lo*hi) per FP64 product
+                      "achieved": round(3 * bfl / (bms * 1e-3) / 1e12, 2), "peak": PEAK_BF16_MFMA,
+                      "frac": round(3 * bfl / (bms * 1e-3) / 1e12 / PEAK_BF16_MFMA, 4),
+                      "launches": bn, "avg_launch_ms": round(bms / bn, 4),
+                      "share_of_step": round(bms * 1e-3 / dtm, 3)}
         xm = rgf_amd.DeviceBuffer(m.n * 16)
         m.solve_device(xm, b, 16)   # with graphs, the first call captures the solve pieces
         m.synchronize()
@@ -399,6 +442,14 @@ def main():
         mixed = {
             "s_per_matrix": round(dtm, 4),
             "TFLOPs_canonical": round((F + S) / dtm / 1e12, 4),
+            "roofline": {
+                # (i) the step's canonical flops (the FP64 path's numerator) against the FP64
+                # MFMA peak, directly comparable with the FP64 path's fraction
+                "step": {"bound": "mfma", "unit": "TFLOP/s", "achieved": round((F + S) / dtm / 1e12, 3),
+                         "peak": PEAK_FP64_MFMA, "frac": round((F + S) / dtm / 1e12 / PEAK_FP64_MFMA, 4)},
+                # (ii) the split-BF16 product kernels' executed MFMA work against the BF16 peak
+                "split_bf16_products": bf,
+            },
             "speedup_vs_fp64": round(dt / dtm, 3),
             "logdet_rel_err_vs_fp64": abs(ldm - ld) / abs(ld),
             "diag_max_rel_err_vs_fp64": float(np.max(np.abs(dmix / fp64_diag - 1))),
@@ -466,6 +517,13 @@ def main():
         "graph_replay": graph_replay,
         "mixed_bf16x3": mixed,
         "batched": batched,
+        # the north-star comparison like for like: mixed batch against the FP64 batch of the
+        # same count (and single matrix against single matrix)
+        "mixed_vs_fp64": {
+            "batched_like_for_like": batched.get("mixed_speedup_vs_fp64_batched") if batched else None,
+            "single_like_for_like": mixed["speedup_vs_fp64"] if mixed else None,
+            "batch_count": args.batch,
+        },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -2,16 +2,30 @@
 //
 // rjanalik/RGF exposes `template<class T> class RGF` (RGF.H:11-80).  This class
 // keeps those method names and signatures for T = double so an existing caller
-// (e.g. main.C:86-113) compiles unchanged; each method forwards to the C entry
-// point that replaces it and throws rgf_error on a non-OK status (the reference
-// ignores LAPACK info and exits on allocation failure).
+// (e.g. main.C:18-134) compiles with `#include "RGF.hpp"` in place of
+// `#include "RGF.H"`; each method forwards to the C entry point that replaces it
+// and throws rgf_error on a non-OK status (the reference ignores LAPACK info and
+// exits on allocation failure).  The reference's header also brings in the wall
+// clock `get_time` (RGF.H:8 -> Utilities.H:16) that its driver times each phase
+// with; it is provided here with the same meaning.
 #pragma once
+
+#include <sys/time.h>
 
 #include <cstddef>
 #include <stdexcept>
 #include <string>
 
 #include "rgf.h"
+
+// Utilities.H:16 / Utilities.C:26-32: wall-clock seconds since the epoch minus t0
+// (t = get_time(0.0); ...; t = get_time(t) gives the elapsed seconds).
+inline double get_time(double t0)
+{
+    timeval tv;
+    gettimeofday(&tv, nullptr);
+    return (double)tv.tv_sec + (double)tv.tv_usec * 1e-6 - t0;
+}
 
 class rgf_error : public std::runtime_error {
 public:

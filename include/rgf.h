@@ -32,7 +32,8 @@ typedef enum {
 
 typedef enum {
     RGF_FP64 = 0,          /* every product in FP64 on MFMA */
-    RGF_MIXED_FP32 = 1,    /* products in FP32 MFMA, FP64 potrf / accumulation / refinement */
+    /* 1 is not used: an FP32-product mode would cap the product speed-up at the 2x of
+       FP32 over FP64 MFMA peak, so only the split-BF16 mode is provided */
     RGF_MIXED_BF16X3 = 2   /* products as split-BF16 (3 terms) on MFMA, FP64 elsewhere */
 } rgf_precision;
 
@@ -130,6 +131,16 @@ int rgf_residual_norm(rgf_handle_t h, const double *x, const double *b, double *
 /* RGF<T>::residualNormNormalized(T* x, T* b), RGF.H:659-663. */
 int rgf_residual_norm_normalized(rgf_handle_t h, const double *x, const double *b, double *out);
 
+/* Device memory of handles comes from a per-device arena that keeps freed memory for
+ * reuse (creating and destroying handles then calls neither hipMalloc nor hipFree, which
+ * waits for the whole device).  Extension: the idle bytes the arena keeps per device
+ * (default 4 GiB, or RGF_DEVICE_CACHE_GB in the environment); lowering it releases
+ * wholly free memory beyond the new limit at once.  *previous (may be NULL) receives the
+ * old limit. */
+int rgf_set_device_cache_limit(size_t bytes, size_t *previous);
+/* Extension: release every wholly free block of the arena of `device` (-1: all devices). */
+int rgf_trim_device_cache(int device);
+
 /* Message of the last failed call on h (or of the last failed create when h is NULL). */
 const char *rgf_last_error(rgf_handle_t h);
 
@@ -162,7 +173,8 @@ int rgf_export_matrix_mf(rgf_handle_t h, double *mf);
 int rgf_export_factor_mf(rgf_handle_t h, double *mf);
 
 /* Per-process GEMM kernel timing with HIP events on the launching stream:
- * op_class 0 = NN, 1 = NT, 2 = TN, 3 = TT, -1 = off.  read() synchronizes and
+ * op_class 0 = NN, 1 = NT, 2 = TN, 3 = TT (FP64), 4..7 the same on split-BF16,
+ * 8 = every split-BF16 class, 9 = every FP64 class, -1 = off.  read() synchronizes and
  * returns the summed kernel time (ms), the summed canonical flops and the launch count.
  * A measurement aid: enable it only while one thread drives the library. */
 int rgf_kernel_timing_enable(int op_class);

@@ -14,11 +14,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (RGF_EDEVICE, RGF_EINVAL, RGF_ENOMEM, RGF_ENOTSPD, RGF_ESTATE, RGF_FP64,
-                   RGF_MIXED_BF16X3, RGF_MIXED_FP32, RGF_OK, RGF_SELINV_EXACT, RGF_SELINV_REFERENCE,
+                   RGF_MIXED_BF16X3, RGF_OK, RGF_SELINV_EXACT, RGF_SELINV_REFERENCE,
                    DeviceBuffer, GenParams, Info, Options, gpu_count)
 
 __all__ = ["RGF", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
-           "default_gen_params", "gpu_count", "RGF_FP64", "RGF_MIXED_FP32", "RGF_MIXED_BF16X3",
+           "default_gen_params", "gpu_count", "RGF_FP64", "RGF_MIXED_BF16X3",
            "RGF_SELINV_EXACT", "RGF_SELINV_REFERENCE", "RGF_OK", "RGF_EINVAL", "RGF_ENOTSPD",
            "RGF_ENOMEM", "RGF_EDEVICE", "RGF_ESTATE", "mf_size", "canonical_flops"]
 
@@ -260,6 +260,18 @@ def footprint_bytes(ns: int, nt: int, nd: int, count: int = 1, options: Options 
     """Device bytes a handle of this shape and options holds (admission control)."""
     opt = C.byref(options) if options is not None else None
     return int(_lib.load().rgf_footprint_bytes(ns, nt, nd, count, opt))
+
+
+def set_device_cache_limit(nbytes: int) -> int:
+    """Idle device bytes the library's arena keeps per device for reuse; returns the old limit."""
+    prev = C.c_size_t()
+    _check(_lib.load().rgf_set_device_cache_limit(int(nbytes), C.byref(prev)), None)
+    return prev.value
+
+
+def trim_device_cache(device: int = -1) -> None:
+    """Give every wholly free block of the arena back to the device (-1: all devices)."""
+    _check(_lib.load().rgf_trim_device_cache(int(device)), None)
 
 
 def kernel_timing_enable(op_class: int) -> None:

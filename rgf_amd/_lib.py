@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librgf_hip.so")
 
 RGF_OK, RGF_EINVAL, RGF_ENOTSPD, RGF_ENOMEM, RGF_EDEVICE, RGF_ESTATE = range(6)
-RGF_FP64, RGF_MIXED_FP32, RGF_MIXED_BF16X3 = range(3)
+RGF_FP64, RGF_MIXED_BF16X3 = 0, 2
 RGF_SELINV_EXACT, RGF_SELINV_REFERENCE = range(2)
 
 STATUS_NAMES = {0: "RGF_OK", 1: "RGF_EINVAL", 2: "RGF_ENOTSPD", 3: "RGF_ENOMEM", 4: "RGF_EDEVICE",
@@ -27,7 +27,8 @@ EXPORTS = [
     "rgf_selinv_diag_device", "rgf_selinv_blocks_device", "rgf_synchronize", "rgf_get_info", "rgf_export_matrix_mf",
     "rgf_export_factor_mf", "rgf_kernel_timing_enable", "rgf_kernel_timing_read",
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
-    "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats",
+    "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats", "rgf_set_device_cache_limit",
+    "rgf_trim_device_cache",
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
@@ -105,6 +106,8 @@ def load():
         lib.rgf_logdet_batch.argtypes = [H, dp]
         lib.rgf_refine_stats.argtypes = [H, C.POINTER(C.c_int), dp]
         lib.rgf_selinv_diag_batch.argtypes = [H, dp]
+        lib.rgf_set_device_cache_limit.argtypes = [C.c_size_t, C.POINTER(C.c_size_t)]
+        lib.rgf_trim_device_cache.argtypes = [C.c_int]
         lib.rgf_debug_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int,
                                        dp, C.c_int, C.c_double, dp, C.c_int, C.c_int]
         lib.rgf_debug_gemm.restype = C.c_int

@@ -16,6 +16,7 @@
 #include "../../include/rgf.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <map>
 #include <mutex>
@@ -138,22 +139,20 @@ int dev_status(rgf_handle_s* h, int hip_rc, const char* where)
 // from hipMalloc, sub-allocated best-fit and coalesced on free, so creating and
 // destroying handles calls neither hipMalloc nor hipFree once the arena has grown to the
 // working set -- hipFree waits for the whole device, which would stall the other requests
-// of the serving loop.  Free bytes are kept up to a cap (40% of the device's memory, or
-// RGF_DEVICE_CACHE_GB); beyond it wholly free slabs are released.
-size_t cache_cap()
+// of a serving loop.  Free bytes are kept up to a cap (4 GiB by default, RGF_DEVICE_CACHE_GB
+// or rgf_set_device_cache_limit); beyond it wholly free slabs are released, so a library
+// embedded in a larger process does not sit on idle device memory after its handles are
+// gone (rgf_trim_device_cache releases every wholly free slab at once).
+std::atomic<size_t>& cache_cap_ref()
 {
-    static const size_t cap = [] {
+    static std::atomic<size_t> cap{[] {
         const char* v = getenv("RGF_DEVICE_CACHE_GB");
         if (v) return (size_t)(std::max(0.0, atof(v)) * (double)(1ull << 30));
-        size_t freeb = 0, total = 0;
-        if (hipMemGetInfo(&freeb, &total) != hipSuccess || total == 0) {
-            (void)hipGetLastError();
-            return (size_t)48 << 30;
-        }
-        return total / 10 * 4;
-    }();
+        return (size_t)4 << 30;
+    }()};
     return cap;
 }
+size_t cache_cap() { return cache_cap_ref().load(std::memory_order_relaxed); }
 struct HipSlabs {
     static int grow(void** p, size_t n)
     {
@@ -225,6 +224,16 @@ size_t split_need(int ns, int nd, int count)
 }
 size_t split_x_need(int ns, int count) { return (size_t)count * split_entry_elems(ns, ns); }
 
+// doubles of the skinny products' split-k partials: the solves and the residual (one
+// matrix, m <= max(n, 2ns+nd)) and the factor sweep's arrow-row update (count matrices
+// of m = ns at once, which needs count * 16 * ns)
+size_t solve_work_elems(int ns, int nt, int nd, int count)
+{
+    const long long n = (long long)ns * nt + nd;
+    const int mx = (int)std::max<long long>(n, 2LL * ns + nd);
+    return std::max(skinny_work_elems(mx, mx, 16), (size_t)count * 16 * (size_t)ns);
+}
+
 size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision)
 {
     const size_t ldp = (size_t)r8(2LL * ns + nd), ldf = (size_t)r8((long long)ns + nd);
@@ -238,7 +247,8 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     size_t per = ldp * ns * nt * (keep ? 2 : 1) + cnd * (keep ? 3 : 2) + ldf * ns + tt + T * 4096 + nt * T + 1 +
                  (size_t)std::max(nt * nd * nd, 1);
     per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
-    size_t once = 3 * ((size_t)ns * nt + nd) * 16 + 64 * (size_t)std::max<long long>((long long)ns * nt + nd, 2LL * ns + nd) * 16;
+    // solve buffers (Y, X, Bv, R, Dc: n x 16 each) and the skinny products' partials
+    size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
     size_t bytes = (per * count + once) * sizeof(double);
     if (precision == RGF_MIXED_BF16X3) bytes += (split_need(ns, nd, count) + split_x_need(ns, count)) * 2;
     return bytes;
@@ -295,8 +305,7 @@ int ensure_solve_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->Bv, n16))) return rc;
     if ((rc = dalloc(h, &h->R, n16))) return rc;
     if ((rc = dalloc(h, &h->Dc, n16))) return rc;
-    const int mx = std::max<int>((int)h->n, 2 * h->ns + h->nd);
-    h->work_elems = skinny_work_elems(mx, mx, 16);
+    h->work_elems = solve_work_elems(h->ns, h->nt, h->nd, h->count);
     if ((rc = dalloc(h, &h->work, h->work_elems))) return rc;
     return RGF_OK;
 }
@@ -1171,6 +1180,29 @@ int rgf_destroy(rgf_handle_t h)
     for (auto& a : h->allocs) cache_free(h->device, a.first, a.second);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
+    return RGF_OK;
+}
+
+int rgf_set_device_cache_limit(size_t bytes, size_t* previous)
+{
+    const size_t old = cache_cap_ref().exchange(bytes);
+    if (previous) *previous = old;
+    for (int d = 0; d < 64; d++)
+        if (g_arena[d].free_bytes() > bytes) {
+            hipSetDevice(d);
+            g_arena[d].trim_to(bytes);
+        }
+    return RGF_OK;
+}
+
+int rgf_trim_device_cache(int device)
+{
+    if (device < -1 || device >= 64) return set_err(nullptr, RGF_EINVAL, "device %d out of range", device);
+    for (int d = 0; d < 64; d++)
+        if ((device < 0 || d == device) && g_arena[d].free_bytes() > 0) {
+            hipSetDevice(d);
+            g_arena[d].trim_to(0);
+        }
     return RGF_OK;
 }
 

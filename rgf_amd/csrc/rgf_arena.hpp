@@ -94,6 +94,17 @@ public:
         return true;
     }
 
+    // release wholly free slabs, largest first, while more than `keep` bytes are free
+    void trim_to(size_t keep)
+    {
+        std::vector<void*> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            trim(keep, drop);
+        }
+        for (void* q : drop) Backend::release(q);
+    }
+
     size_t free_bytes() const
     {
         std::lock_guard<std::mutex> lk(mu_);

@@ -448,8 +448,10 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
 
         // every workgroup of a matrix factors the tile (in LDS) and then produces its
         // share of the RB-row blocks of the panel below
+        // (with store_diag_l the lead workgroup writes L_kk over the tile that the other
+        // workgroups of the launch read at their start: one workgroup per matrix then)
         const int nrb = (below + RB - 1) / RB;
-        const int nf = std::max(1, std::min(nrb, std::max(1, 256 / count)));
+        const int nf = store_diag_l ? 1 : std::max(1, std::min(nrb, std::max(1, 256 / count)));
         hipLaunchKernelGGL(potrf_tile_kernel, dim3(nf, count), dim3(256), 0, s, A, lda, sA, k0, kb, below, odd, xt,
                            sX, logdet_parts + p, sP, info, info_base, store_diag_l);
         hipError_t e = hipGetLastError();

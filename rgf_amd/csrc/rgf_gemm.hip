@@ -311,7 +311,10 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
 
 void* timing_begin(int op_class, const GemmDesc* d, int nd, hipStream_t s)
 {
-    if (op_class != g_timing_class) return nullptr;
+    // class 8 selects every split-BF16 class (4..7), class 9 every FP64 class (0..3)
+    const bool on = op_class == g_timing_class || (g_timing_class == 8 && op_class >= 4 && op_class <= 7) ||
+                    (g_timing_class == 9 && op_class >= 0 && op_class <= 3);
+    if (!on) return nullptr;
     auto* tl = new TimedLaunch{};
     hipEventCreate(&tl->a);
     hipEventCreate(&tl->b);

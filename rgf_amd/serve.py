@@ -88,9 +88,14 @@ class Executor:
 class GPUExecutor(Executor):
     """librgf_hip.so: one handle (one stream) per request."""
 
-    def __init__(self, precision: int, refine_tol: float = 1e-12):
+    def __init__(self, precision: int, refine_tol: float = 1e-12, device_cache_gb: Optional[float] = None):
+        """device_cache_gb: idle device memory the library's arena may keep for reuse
+        (None: the library's default).  A serving process sets it high: handles come and
+        go continuously, and giving slabs back calls hipFree, which waits for the device."""
         import rgf_amd
         self.rgf = rgf_amd
+        if device_cache_gb is not None:
+            rgf_amd.set_device_cache_limit(int(device_cache_gb * (1 << 30)))
         self.precision = precision
         self.keep = 1 if precision != rgf_amd.RGF_FP64 else 0   # refinement needs A
         self.refine_tol = refine_tol

@@ -1,4 +1,5 @@
-"""Test inputs shared by the CPU and GPU suites (formulas only; no reference data)."""
+"""Test inputs shared by the CPU and GPU suites: formulas, plus the reference's INLA test
+matrix data/A_126_126_ns42_nt3.dat stored as data (golden/inla_126_csc.npz)."""
 from __future__ import annotations
 
 import json
@@ -14,6 +15,16 @@ GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 def goldens() -> dict:
     with open(os.path.join(GOLDEN_DIR, "goldens.json")) as f:
         return json.load(f)
+
+
+def inla_126():
+    """The reference's INLA matrix (ns 42, nt 3, nd 0) as lower CSC: (ia, ja, a, ns, nt, nd)."""
+    z = np.load(os.path.join(GOLDEN_DIR, "inla_126_csc.npz"))
+    return z["ia"], z["ja"], z["a"], int(z["ns"]), int(z["nt"]), int(z["nd"])
+
+
+def inla_126_sha256() -> str:
+    return str(np.load(os.path.join(GOLDEN_DIR, "inla_126_csc.npz"))["sha256"])
 
 
 def rhs_formula(n: int, nrhs: int = 1) -> np.ndarray:

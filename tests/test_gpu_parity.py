@@ -233,13 +233,15 @@ def _csc_case(name):
     if name.startswith("closed"):
         ns, nt, nd = (3, 3, 0) if name == "closed_9x9" else (3, 3, 2)
         ia, ja, a = oracle.closed_form_csc(ns, nt, nd)
+    elif name == "inla_126":   # the reference's own INLA matrix (data/A_126_126_ns42_nt3.dat)
+        ia, ja, a, ns, nt, nd = fixtures.inla_126()
     else:
         A, ns, nt, nd = fixtures.spde_like(nd=G[name]["nd"])
         ia, ja, a = oracle.dense_to_csc_lower(A, ns, nt, nd)
     return ia, ja, a, ns, nt, nd
 
 
-@pytest.mark.parametrize("name", ["closed_9x9", "closed_11x11", "spde_126", "spde_126_nd2"])
+@pytest.mark.parametrize("name", ["closed_9x9", "closed_11x11", "inla_126", "spde_126", "spde_126_nd2"])
 def test_csc_fixture_goldens(gpu, name):
     say(f"csc fixture {name}")
     ia, ja, a, ns, nt, nd = _csc_case(name)
@@ -255,8 +257,15 @@ def test_csc_fixture_goldens(gpu, name):
         d = s.RGFdiag()
         assert np.abs(d / np.array(g["diag_inv"]) - 1).max() <= 1e-9
         r = s.residualNormNormalized(x[:, 0].copy(), b[:, 0].copy())
-        r0 = oracle.residual_norm(ia, ja, a, n, x[:, 0].copy(), b[:, 0].copy(), normalized=True)
-        assert r < 1e-12 and r == pytest.approx(r0, rel=0.5, abs=1e-15)
+        assert r < 1e-12
+        # at a perturbed x the residual is O(1e-3) of b: both norms against numpy to 1e-12
+        A = oracle.csc_to_dense(ia, ja, a, n)
+        xp = x[:, 0] + 1e-3 * np.abs(xg[:, 0]).max() * np.sin(np.arange(n) + 0.3)
+        rp = np.linalg.norm(b[:, 0] - A @ xp)
+        assert rp > 1e-4 * np.linalg.norm(b[:, 0])
+        assert s.residualNorm(xp.copy(), b[:, 0].copy()) == pytest.approx(rp, rel=1e-12)
+        assert s.residualNormNormalized(xp.copy(), b[:, 0].copy()) == pytest.approx(rp / np.linalg.norm(b[:, 0]),
+                                                                                    rel=1e-12)
     opts = rgf_amd.default_options(selinv_mode=rgf_amd.RGF_SELINV_REFERENCE)
     with rgf_amd.RGF(ia, ja, a, ns, nt, nd, opts) as s:
         d = s.RGFdiag()
@@ -701,3 +710,59 @@ def test_sweep_variant_switches_keep_bits(gpu, prec, key, alt):
         lib.rgf_debug_set_tuning(key, old)
     assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
     assert ref[2] == got[2] and np.array_equal(ref[3], got[3])
+
+
+# ------------------------------------------------------------------ footprints and large batches
+@pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
+@pytest.mark.parametrize("ns,nt,nd,count", [(130, 4, 5, 1), (200, 3, 0, 1), (64, 5, 3, 3)])
+def test_footprint_covers_device_bytes(gpu, prec, ns, nt, nd, count):
+    """rgf_footprint_bytes (the serving loop's admission figure) is at least what a handle
+    holds after factorize, 16-RHS solve and selected inversion, up to the 256-byte
+    rounding of each of its allocations."""
+    say(f"footprint prec={prec} ns={ns} nt={nt} nd={nd} count={count}")
+    opts = rgf_amd.default_options(precision=prec, keep_matrix=1)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=4, count=count, theta_mode=1, options=opts) as h:
+        h.factorize()
+        h.solve(None, np.ones((h.n, 16)), 16)
+        h.selinv_diag_batch()
+        held = h.info().device_bytes
+    fp = rgf_amd.footprint_bytes(ns, nt, nd, count, opts)
+    assert fp + 256 * 40 >= held, (fp, held)
+    assert fp <= 1.5 * held + (1 << 20)
+
+
+def test_large_batch_count_with_arrow(gpu):
+    """More matrices than 64 * nt in one handle with nd > 0: the arrow-row update's
+    split-k partials for the whole batch fit its buffer; sampled instances equal
+    single-matrix handles bitwise and the last matches the oracle."""
+    ns, nt, nd, count = 48, 2, 3, 200
+    say(f"large batch ns={ns} nt={nt} nd={nd} count={count}")
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=6, count=count, theta_mode=1) as hb:
+        lds = hb.logdet_batch()
+        diags = hb.selinv_diag_batch()
+    for b in (0, 129, count - 1):
+        td = 0.5 + 1.5 * oracle.gen_uniform(6, b, 0)
+        tc = 0.1 + 0.9 * oracle.gen_uniform(6, b, 1)
+        with rgf_amd.RGF.generated(ns, nt, nd, seed=6, matrix_id=b, theta_d=td, theta_c=tc) as h1:
+            assert h1.logDet() == lds[b]
+            assert np.array_equal(h1.RGFdiag(), diags[b])
+            L = oracle.factor(h1.export_matrix_mf(), ns, nt, nd)
+        assert abs(lds[b] - oracle.logdet(L, ns, nt, nd)) <= 1e-11 * abs(lds[b])
+        assert np.abs(diags[b] / oracle.selinv(L, ns, nt, nd, exact=True) - 1).max() <= 1e-10
+
+
+def test_generated_residual_at_perturbed_x(gpu):
+    """residualNorm / residualNormNormalized (RGF.H:631-663) against numpy's dense A x at a
+    perturbed solution, on a generated matrix with an arrowhead."""
+    ns, nt, nd = 150, 4, 5
+    say(f"residual norms ns={ns} nt={nt} nd={nd}")
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=41) as s:
+        A = oracle.mf_to_dense(s.export_matrix_mf(), ns, nt, nd)
+        b = np.random.default_rng(2).standard_normal(s.n)
+        x = s.solve(None, b, 1)
+        assert s.residualNormNormalized(x, b) < 1e-14
+        for scale in (1e-3, 1.0):
+            xp = x + scale * np.cos(np.arange(s.n) * 0.7)
+            r = np.linalg.norm(b - A @ xp)
+            assert s.residualNorm(xp, b) == pytest.approx(r, rel=1e-12)
+            assert s.residualNormNormalized(xp, b) == pytest.approx(r / np.linalg.norm(b), rel=1e-12)
