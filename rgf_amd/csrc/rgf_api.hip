@@ -226,12 +226,12 @@ size_t split_x_need(int ns, int count) { return (size_t)count * split_entry_elem
 
 // doubles of the skinny products' split-k partials: the solves and the residual (one
 // matrix, m <= max(n, 2ns+nd)) and the factor sweep's arrow-row update (count matrices
-// of m = ns at once, which needs count * 16 * ns)
+// of m = k = ns at once, each split as it would be alone)
 size_t solve_work_elems(int ns, int nt, int nd, int count)
 {
     const long long n = (long long)ns * nt + nd;
     const int mx = (int)std::max<long long>(n, 2LL * ns + nd);
-    return std::max(skinny_work_elems(mx, mx, 16), (size_t)count * 16 * (size_t)ns);
+    return std::max(skinny_work_elems(mx, mx, 16), skinny_batch_work_elems(Op::N, ns, ns, count));
 }
 
 size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision)

@@ -170,6 +170,9 @@ int corner_factor(const double* Dbase, long long dstride, long long sM, int ldd,
 // R is k x nr (ld ldr), Y is m x nr (ld ldy), nr <= 16.  tri: 0 none, 1 = op(A) lower,
 // 2 = op(A) upper.  Uses a split-k partial buffer `work` of size >= skinny_work(m, k, nr).
 size_t skinny_work_elems(int m, int k, int nr);
+// Partials of skinny_f64s over `count` matrices of one product shape (the split of k
+// into slabs depends on m and k only, so each matrix is summed as it would be alone).
+size_t skinny_batch_work_elems(Op ta, int m, int k, int count);
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
                double* work, hipStream_t s);

@@ -26,16 +26,28 @@ struct SkinnyPlan {
     int slabs, kslab;
 };
 
-// k is split into slabs (partials reduced afterwards) until the launch has ~2048
-// workgroups: the products stream A once from HBM and need many loads in flight.
-// The partials of all `count` matrices must fit the caller's buffer (cap elements).
-static SkinnyPlan plan(int wgm, int k, int chunk, int nrt, int m, int count, size_t cap)
+// k is split into slabs (partials reduced afterwards) until one matrix's launch has
+// ~2048 workgroups: the products stream A once from HBM and need many loads in flight.
+// The split depends on (m, k) alone -- not on the batch count or the caller's buffer --
+// so a matrix of a batch is summed in the same order as the same matrix alone (bitwise
+// equal results); at most 64 slabs, which skinny_work_elems provides for one matrix.
+static SkinnyPlan plan(int wgm, int k, int chunk)
 {
-    int slabs = std::max(1, std::min(cdiv(k, chunk), cdiv(TARGET_WG, (long long)wgm * count)));
-    slabs = (int)std::min<size_t>((size_t)slabs, cap / ((size_t)count * nrt * m));
+    int slabs = std::max(1, std::min({cdiv(k, chunk), cdiv(TARGET_WG, (long long)wgm), 64}));
     int kslab = cdiv(cdiv(k, slabs), chunk) * chunk;
     slabs = cdiv(k, kslab);
     return {std::max(slabs, 1), kslab};
+}
+
+static SkinnyPlan plan_for(Op ta, int m, int k)
+{
+    return ta == Op::N ? plan(cdiv(m, 256), k, KCH) : plan(cdiv(m, 4), k, KCT);
+}
+
+size_t skinny_batch_work_elems(Op ta, int m, int k, int count)
+{
+    const SkinnyPlan p = plan_for(ta, m, k);
+    return (size_t)count * (size_t)(p.slabs > 1 ? p.slabs : 1) * MAXR * (size_t)m;
 }
 
 size_t skinny_work_elems(int m, int k, int nr)
@@ -209,9 +221,9 @@ template <int NR>
 static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                          const SkinnyIO& io, double beta, double* work, size_t work_elems, int count, hipStream_t s)
 {
-    const SkinnyPlan p = ta == Op::N ? plan(cdiv(m, 256), k, KCH, NR, m, count, work_elems)
-                                     : plan(cdiv(m, 4), k, KCT, NR, m, count, work_elems);
+    const SkinnyPlan p = plan_for(ta, m, k);
     const int direct = p.slabs == 1;
+    if (!direct && (size_t)count * p.slabs * NR * m > work_elems) return (int)hipErrorInvalidValue;
     if (ta == Op::N) {
         dim3 grid(cdiv(m, 256), p.slabs, count);
         hipLaunchKernelGGL((skinny_n_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
@@ -239,7 +251,7 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
                 long long sA, long long sR, long long sY0, long long sY)
 {
     if (m <= 0 || nr <= 0 || count <= 0) return 0;
-    if (nr > MAXR || (size_t)count * MAXR * m > work_elems) return (int)hipErrorInvalidValue;
+    if (nr > MAXR) return (int)hipErrorInvalidValue;
     const SkinnyIO io{R, rsk, rsj, Y0, y0si, y0sj, Y, ysi, ysj, sA, sR, sY0, sY};
     if (k <= 0) {   // Y = beta*Y0
         if (count == 1 && ysi == 1 && y0si == 1) return copy2d(Y0, (int)y0sj, Y, (int)ysj, m, nr, beta, s);

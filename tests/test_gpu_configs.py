@@ -72,7 +72,9 @@ def test_c3_batch(gpu, prec):
         _C3_FP64["logdets"] = lds.copy()
     else:
         assert abs(lds[17] - ld0) <= 1e-8 * abs(ld0)
-        assert np.abs(x17 - x0).max() <= 1e-9 * np.abs(x0).max()   # refined in FP64
+        # the batch holds factors only (keep_matrix=0: 64 matrices of 2.2 GB), so the mixed
+        # solve is not refined against A here; refined mixed solves: c4 test and the parity suite
+        assert np.abs(x17 - x0).max() <= 1e-6 * np.abs(x0).max()
         assert np.abs(diags[17] / d0 - 1).max() <= 1e-4
         ref = _C3_FP64.get("logdets")
         if ref is None:   # the FP64 case did not run in this session: compute it
@@ -86,7 +88,7 @@ def test_c3_batch(gpu, prec):
     say(f"  oracle on instance 17: logdet rel {abs(lds[17] - ld0) / abs(ld0):.2e}")
 
 
-C4_SEED = 3
+C4_SEED = 8   # 40 draws with 4 follow-ups and 4 requests small enough for the oracle
 
 
 @pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
