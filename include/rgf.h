@@ -146,6 +146,16 @@ const char *rgf_last_error(rgf_handle_t h);
 
 /* ---- device-resident extensions (pointers are device memory of the handle's device) ---- */
 int rgf_solve_device(rgf_handle_t h, double *x_dev, const double *b_dev, size_t nrhs);
+/* No counterpart in the reference.  rgf_solve_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): it allocates nothing and keeps its solve buffers in a static 128 MiB device
+ * array per GPU (held until the process ends).  Same results as rgf_solve_device, bit for
+ * bit.  The handle must be factorized already (RGF_ESTATE otherwise); a solve whose
+ * buffers exceed the array is rejected with RGF_EINVAL (FP64: n * 8 bytes per right-hand
+ * side per pass, passes shrink to fit; refined mixed precision: 16 columns per pass, about
+ * 3 * 16 * n * 8 bytes, 4 * 16 * n * 8 in place).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race. */
+int rgf_solve_device_noalloc(rgf_handle_t h, double *x_dev, const double *b_dev, size_t nrhs);
 /* diag(A^-1) of every matrix of the handle into device memory (count x n). */
 int rgf_selinv_diag_device(rgf_handle_t h, double *diag_dev);
 /* Extension (gradients of INLA-type objectives need more than the diagonal; the reference

@@ -28,7 +28,7 @@ EXPORTS = [
     "rgf_export_factor_mf", "rgf_kernel_timing_enable", "rgf_kernel_timing_read",
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats", "rgf_set_device_cache_limit",
-    "rgf_trim_device_cache",
+    "rgf_trim_device_cache", "rgf_solve_device_noalloc",
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
@@ -89,6 +89,7 @@ def load():
         lib.rgf_last_error.argtypes = [H]
         lib.rgf_last_error.restype = C.c_char_p
         lib.rgf_solve_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_size_t]
+        lib.rgf_solve_device_noalloc.argtypes = [H, C.c_void_p, C.c_void_p, C.c_size_t]
         lib.rgf_selinv_diag_device.argtypes = [H, C.c_void_p]
         lib.rgf_selinv_blocks_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.rgf_synchronize.argtypes = [H]

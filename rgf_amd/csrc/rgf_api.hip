@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace rgf {
@@ -310,6 +311,20 @@ int ensure_solve_buffers(rgf_handle_s* h)
     return RGF_OK;
 }
 
+// Buffers of a solve: Y (n x 16), the refinement's residual R and correction Dc, Bv (b
+// kept for an in-place refined solve), and the skinny products' split-k partials.  A
+// handle's own (ensure_solve_buffers) or the no-allocation entry point's static scratch.
+struct SolveBufs {
+    double* Y;
+    double* R;
+    double* Dc;
+    double* Bv;
+    double* work;
+    size_t work_elems;
+};
+
+SolveBufs handle_bufs(rgf_handle_s* h) { return {h->Y, h->R, h->Dc, h->Bv, h->work, h->work_elems}; }
+
 int ensure_selinv_buffers(rgf_handle_s* h)
 {
     if (h->W) return RGF_OK;
@@ -581,13 +596,14 @@ int ensure_factorized(rgf_handle_s* h)
 //   forward   y_i = X_i (b_i - B_{i-1} y_{i-1}),  y_d = X_dd (b_d - sum D_i y_i)
 //   backward  x_d = X_dd^T y_d,  x_i = X_i^T (y_i - B_i^T x_{i+1} - D_i^T x_d)
 // x, b: device, n x nr (ld n), nr <= 16; Y is scratch.
-int solve_block(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
+int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const double* b, int nr)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp;
     const int n = (int)h->n;
     hipStream_t s = h->stream;
-    double* Y = h->Y;
-    double* w = h->work;
+    double* Y = sb.Y;
+    double* w = sb.work;
+    const size_t we = sb.work_elems;
     const double* Xdd = h->Xdd + (long long)mb * h->cstride;
     int rc;
     if (x != b && (rc = copy2d(b, n, x, n, n, nr, 1.0, s))) return rc;   // x holds r during the forward pass
@@ -600,35 +616,36 @@ int solve_block(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
         if (i > 0) {
             const double* Pm = l_panel(h, i - 1, mb);
             if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, n, 1.0, ri, n,
-                                 ri, n, w, s)))
+                                 ri, n, w, s, we)))
                 return rc;
         }
-        if ((rc = skinny_f64(Op::N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we))) return rc;
         if (nd > 0 &&
-            (rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, yi, n, 1.0, xd, n, xd, n, w, s)))
+            (rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, yi, n, 1.0, xd, n, xd, n, w, s, we)))
             return rc;
     }
     if (nd > 0) {
-        if ((rc = skinny_f64(Op::N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s))) return rc;
-        if ((rc = skinny_f64(Op::T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we))) return rc;
+        if ((rc = skinny_f64(Op::T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we))) return rc;
     }
     for (int i = nt - 1; i >= 0; i--) {
         double* P = l_panel(h, i, mb);
         double* yi = Y + (long long)i * ns;
         double* xi = x + (long long)i * ns;
         if (nd > 0 &&
-            (rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s)))
+            (rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s, we)))
             return rc;
         if (i < nt - 1 &&
             (rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, P + ns, ldp, x + (long long)(i + 1) * ns, n, 1.0, yi, n,
-                             yi, n, w, s)))
+                             yi, n, w, s, we)))
             return rc;
-        if ((rc = skinny_f64(Op::T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we))) return rc;
     }
     return 0;
 }
 
-int block_matvec_residual(rgf_handle_s* h, int mb, const double* x, const double* b, double* r, int nr);
+int block_matvec_residual(rgf_handle_s* h, const SolveBufs& sb, int mb, const double* x, const double* b, double* r,
+                          int nr);
 
 // column 2-norms of an n x nr array (ld n), one workgroup per column, fixed-order tree
 __global__ __launch_bounds__(1024) void colnorm2_kernel(const double* r, long long n, double* out)
@@ -654,22 +671,26 @@ __global__ void axpy_kernel(double* y, const double* x, long long total)
 
 // Solve with the (mixed-precision) factor, then FP64 iterative refinement against
 // the resident FP64 matrix: r = b - A x; x += M^-1 r, until max_c ||r_c|| / ||b_c||
-// <= refine_tol or refine_iters steps.  b must not alias x.
-int solve_refined(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
+// <= refine_tol or refine_iters steps.  b must not alias x.  eager: launch every piece
+// directly (no graph capture, which allocates).
+int solve_refined(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const double* b, int nr, bool eager)
 {
     const long long n = h->n;
     hipStream_t s = h->stream;
-    int rc = run_piece(h, 0, mb, nr, x, b, [&] { return solve_block(h, mb, x, b, nr); });
+    auto piece = [&](int kind, const void* px, const void* pb, auto&& fn) {
+        return eager ? fn() : run_piece(h, kind, mb, nr, px, pb, fn);
+    };
+    int rc = piece(0, x, b, [&] { return solve_block(h, sb, mb, x, b, nr); });
     if (rc) return rc;
     h->last_refine_iters = 0;
     h->last_refine_res = 0.0;
     if (!h->Ap || h->opt.refine_iters <= 0) return 0;
     hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, b, n, h->scal + 32);
     for (int it = 0; it <= h->opt.refine_iters; it++) {
-        rc = run_piece(h, 1, mb, nr, x, b, [&] {
-            const int r = block_matvec_residual(h, mb, x, b, h->R, nr);
+        rc = piece(1, x, b, [&] {
+            const int r = block_matvec_residual(h, sb, mb, x, b, sb.R, nr);
             if (r) return r;
-            hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, h->R, n, h->scal + 16);
+            hipLaunchKernelGGL(colnorm2_kernel, dim3(nr), dim3(1024), 0, s, sb.R, n, h->scal + 16);
             return (int)hipGetLastError();
         });
         if (rc) return rc;
@@ -681,12 +702,12 @@ int solve_refined(rgf_handle_s* h, int mb, double* x, const double* b, int nr)
         for (int c = 0; c < nr; c++) worst = std::max(worst, v[16 + c] > 0 ? v[c] / v[16 + c] : v[c]);
         h->last_refine_res = worst;
         if (worst <= h->opt.refine_tol || it == h->opt.refine_iters) break;
-        rc = run_piece(h, 2, mb, nr, x, nullptr, [&] {
-            const int r = solve_block(h, mb, h->Dc, h->R, nr);
+        rc = piece(2, x, nullptr, [&] {
+            const int r = solve_block(h, sb, mb, sb.Dc, sb.R, nr);
             if (r) return r;
             const long long total = n * nr;
             hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)), dim3(256), 0,
-                               s, x, h->Dc, total);
+                               s, x, sb.Dc, total);
             return (int)hipGetLastError();
         });
         if (rc) return rc;
@@ -702,19 +723,20 @@ int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
     if ((rc = ensure_solve_buffers(h))) return rc;
     const long long n = h->n;
     const bool refine = h->opt.precision != RGF_FP64;
+    const SolveBufs sb = handle_bufs(h);
     for (size_t c0 = 0; c0 < nrhs; c0 += 16) {
         const int nr = (int)std::min<size_t>(16, nrhs - c0);
         if (refine) {
             const double* bb = b + c0 * n;
             if (bb == x + c0 * n) {   // in place: keep b for the residual
-                if ((rc = copy2d(bb, (int)n, h->Bv, (int)n, (int)n, nr, 1.0, h->stream))) return dev_status(h, rc, "solve");
-                bb = h->Bv;
+                if ((rc = copy2d(bb, (int)n, sb.Bv, (int)n, (int)n, nr, 1.0, h->stream))) return dev_status(h, rc, "solve");
+                bb = sb.Bv;
             }
-            rc = solve_refined(h, h->cur, x + c0 * n, bb, nr);
+            rc = solve_refined(h, sb, h->cur, x + c0 * n, bb, nr, false);
         } else {
             double* xc = x + c0 * n;
             const double* bc = b + c0 * n;
-            rc = run_piece(h, 0, h->cur, nr, xc, bc, [&] { return solve_block(h, h->cur, xc, bc, nr); });
+            rc = run_piece(h, 0, h->cur, nr, xc, bc, [&] { return solve_block(h, sb, h->cur, xc, bc, nr); });
         }
         if (rc) return dev_status(h, rc, "solve");
     }
@@ -918,11 +940,13 @@ __global__ __launch_bounds__(1024) void norm2_kernel(const double* r, long long 
     if (threadIdx.x == 0) *out = sqrt(red[0]);
 }
 
-int block_matvec_residual(rgf_handle_s* h, int mb, const double* x, const double* b, double* r, int nr)
+int block_matvec_residual(rgf_handle_s* h, const SolveBufs& sb, int mb, const double* x, const double* b, double* r,
+                          int nr)
 {
     const int ns = h->ns, nt = h->nt, nd = h->nd, ldp = h->ldp, n = (int)h->n;
     hipStream_t s = h->stream;
-    double* w = h->work;
+    double* w = sb.work;
+    const size_t we = sb.work_elems;
     int rc;
     if ((rc = copy2d(b, n, r, n, n, nr, 1.0, s))) return rc;
     const double* xd = x + (long long)nt * ns;
@@ -931,20 +955,20 @@ int block_matvec_residual(rgf_handle_s* h, int mb, const double* x, const double
         const double* A = h->Ap + (long long)mb * h->mstride + (long long)t * h->pstride;
         const double* xt = x + (long long)t * ns;
         double* rt = r + (long long)t * ns;
-        if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s))) return rc;
+        if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s, we))) return rc;
         if (t < nt - 1) {
-            if ((rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, A + ns, ldp, xt + ns, n, 1.0, rt, n, rt, n, w, s))) return rc;
-            if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A + ns, ldp, xt, n, 1.0, rt + ns, n, rt + ns, n, w, s)))
+            if ((rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, A + ns, ldp, xt + ns, n, 1.0, rt, n, rt, n, w, s, we))) return rc;
+            if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A + ns, ldp, xt, n, 1.0, rt + ns, n, rt + ns, n, w, s, we)))
                 return rc;
         }
         if (nd > 0) {
-            if ((rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, A + 2 * ns, ldp, xd, n, 1.0, rt, n, rt, n, w, s))) return rc;
-            if ((rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s))) return rc;
+            if ((rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, A + 2 * ns, ldp, xd, n, 1.0, rt, n, rt, n, w, s, we))) return rc;
+            if ((rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s, we))) return rc;
         }
     }
     if (nd > 0 &&
         (rc = skinny_f64(Op::N, 0, nd, nd, nr, -1.0, h->Add + (long long)mb * h->cstride, nd, xd, n, 1.0, rd, n, rd,
-                         n, w, s)))
+                         n, w, s, we)))
         return rc;
     return 0;
 }
@@ -960,7 +984,7 @@ int residual_norm_host(rgf_handle_s* h, const double* x, const double* b, double
     hipError_t e = hipMemcpyAsync(h->X, x, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(h->Bv, b, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return dev_status(h, (int)e, "residual upload");
-    rc = block_matvec_residual(h, h->cur, h->X, h->Bv, h->Y, 1);
+    rc = block_matvec_residual(h, handle_bufs(h), h->cur, h->X, h->Bv, h->Y, 1);
     if (rc) return dev_status(h, rc, "residual");
     hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, h->Y, h->n, h->scal);
     hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, h->Bv, h->n, h->scal + 1);
@@ -1052,6 +1076,112 @@ int create_generated(size_t ns, size_t nt, size_t nd, size_t count, const rgf_ge
     }
     *out = h;
     return RGF_OK;
+}
+
+// ------------------------------------------------------------------ no-allocation solve
+// For callers that cannot allocate device memory (code under HIP graph capture, tight
+// serving loops, harnesses that forbid hipMalloc): the solve buffers live in one static
+// device array per GPU instead of the handle.  Every call on a GPU shares that GPU's
+// array (held until the process ends), so calls that overlap on one GPU race; nothing in
+// it needs to start at zero (every buffer is written before it is read).
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+constexpr size_t kScratchElems = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchElems];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[64] = {};   // device address of `scratch` on each device
+
+// The array's device address on the current device (hipGetSymbolAddress once per device;
+// the array's name is a host-side stand-in and must never reach a kernel).
+int scratch_on_device(int dev, double** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
+        }
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Partials the solve (and, for the refinement, the block matvec) needs with nr columns:
+// the largest over the skinny products of one matrix's sweep.
+size_t solve_call_work(const rgf_handle_s* h, int nr)
+{
+    const int ns = h->ns, nd = h->nd;
+    size_t w = 0;
+    const Op N = Op::N, T = Op::T;
+    for (auto c : {std::make_tuple(N, ns, ns), std::make_tuple(N, nd, ns), std::make_tuple(N, nd, nd),
+                   std::make_tuple(T, nd, nd), std::make_tuple(T, ns, nd), std::make_tuple(T, ns, ns)})
+        w = std::max(w, skinny_call_work(std::get<0>(c), std::get<1>(c), std::get<2>(c), nr));
+    return w;
+}
+
+inline size_t r32(size_t v) { return (v + 31) / 32 * 32; }   // 256-byte steps
+
+// scratch doubles of a chunk of nr columns (refined: R, Dc and, in place, Bv besides Y)
+size_t noalloc_need(const rgf_handle_s* h, int nr, bool refine, bool in_place)
+{
+    const size_t col = r32((size_t)h->n * nr);
+    return col * (1 + (refine ? 2 : 0) + (refine && in_place ? 1 : 0)) + r32(solve_call_work(h, nr));
+}
+
+int solve_device_noalloc(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
+{
+    const long long n = h->n;
+    const bool refine = h->opt.precision != RGF_FP64 && h->Ap && h->opt.refine_iters > 0;
+    const bool in_place = x == b;
+    // FP64 columns are independent of their chunk (same products, same order), so the
+    // chunk shrinks to what the scratch holds; a refined chunk keeps the 16 columns of
+    // rgf_solve_device, whose refinement stops on the chunk's worst column
+    int nr = (int)std::min<size_t>(16, nrhs);
+    if (!refine)
+        while (nr > 1 && noalloc_need(h, nr, false, false) > kScratchElems) nr /= 2;
+    const size_t need = noalloc_need(h, nr, refine, in_place);
+    if (need > kScratchElems)
+        return set_err(h, RGF_EINVAL,
+                       "solve of n = %lld with %d right-hand sides per pass needs %zu bytes of scratch; the static "
+                       "scratch of the no-allocation entry point holds %zu bytes (use rgf_solve_device)",
+                       n, nr, need * sizeof(double), kScratchBytes);
+    if (!h->factorized)
+        return set_err(h, RGF_ESTATE, "the no-allocation solve needs a factorized handle (call rgf_factorize first)");
+    double* base = nullptr;
+    int rc = scratch_on_device(h->device, &base);
+    if (rc) return dev_status(h, rc, "scratch address");
+    const size_t col = r32((size_t)n * nr);
+    SolveBufs sb{};
+    sb.Y = base;
+    size_t off = col;
+    if (refine) {
+        sb.R = base + off; off += col;
+        sb.Dc = base + off; off += col;
+        if (in_place) { sb.Bv = base + off; off += col; }
+    }
+    sb.work = base + off;
+    sb.work_elems = kScratchElems - off;
+    h->last_refine_iters = 0;
+    h->last_refine_res = 0.0;
+    for (size_t c0 = 0; c0 < nrhs; c0 += (size_t)nr) {
+        const int nc = (int)std::min<size_t>((size_t)nr, nrhs - c0);
+        double* xc = x + c0 * n;
+        const double* bc = b + c0 * n;
+        if (refine) {
+            if (in_place) {
+                if ((rc = copy2d(bc, (int)n, sb.Bv, (int)n, (int)n, nc, 1.0, h->stream))) break;
+                bc = sb.Bv;
+            }
+            rc = solve_refined(h, sb, h->cur, xc, bc, nc, true);
+        } else {
+            rc = solve_block(h, sb, h->cur, xc, bc, nc);
+        }
+        if (rc) break;
+    }
+    return rc ? dev_status(h, rc, "solve (no allocation)") : RGF_OK;
 }
 
 }  // namespace
@@ -1262,6 +1392,14 @@ int rgf_solve_device(rgf_handle_t h, double* x_dev, const double* b_dev, size_t 
     return do_solve_device(h, x_dev, b_dev, nrhs);
 }
 
+int rgf_solve_device_noalloc(rgf_handle_t h, double* x_dev, const double* b_dev, size_t nrhs)
+{
+    if (!h || !x_dev || !b_dev) return RGF_EINVAL;
+    if (nrhs == 0) return RGF_OK;
+    hipSetDevice(h->device);
+    return solve_device_noalloc(h, x_dev, b_dev, nrhs);
+}
+
 int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
 {
     if (!h || !x || !b) return RGF_EINVAL;
@@ -1276,9 +1414,10 @@ int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
         const size_t bytes = (size_t)n * nr * sizeof(double);
         hipError_t e = hipMemcpyAsync(h->Bv, b + c0 * n, bytes, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) return dev_status(h, (int)e, "solve upload");
+        const SolveBufs sb = handle_bufs(h);
         rc = h->opt.precision != RGF_FP64
-                 ? solve_refined(h, h->cur, h->X, h->Bv, nr)
-                 : run_piece(h, 0, h->cur, nr, h->X, h->Bv, [&] { return solve_block(h, h->cur, h->X, h->Bv, nr); });
+                 ? solve_refined(h, sb, h->cur, h->X, h->Bv, nr, false)
+                 : run_piece(h, 0, h->cur, nr, h->X, h->Bv, [&] { return solve_block(h, sb, h->cur, h->X, h->Bv, nr); });
         if (rc) return dev_status(h, rc, "solve");
         e = hipMemcpyAsync(x + c0 * n, h->X, bytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

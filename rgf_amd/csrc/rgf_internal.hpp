@@ -173,9 +173,13 @@ size_t skinny_work_elems(int m, int k, int nr);
 // Partials of skinny_f64s over `count` matrices of one product shape (the split of k
 // into slabs depends on m and k only, so each matrix is summed as it would be alone).
 size_t skinny_batch_work_elems(Op ta, int m, int k, int count);
+// work_elems: capacity of `work` (0: skinny_work_elems(m, k, nr), the most any call needs)
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
-               double* work, hipStream_t s);
+               double* work, hipStream_t s, size_t work_elems = 0);
+// Partial doubles one call of this shape with nr right-hand sides actually writes (0 when
+// k is not split).
+size_t skinny_call_work(Op ta, int m, int k, int nr);
 // Same with element strides: R(kk, c) = R[kk * rsk + c * rsj], Y(i, c) = Y[i * ysi + c * ysj]
 // (Y0 likewise), e.g. row-major operands as transposed views, over `count` matrices at
 // strides sA, sR, sY0, sY in one launch; work holds work_elems doubles (>= count * 16 * m).

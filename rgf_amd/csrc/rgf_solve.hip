@@ -266,10 +266,19 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
 
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
-               double* work, hipStream_t s)
+               double* work, hipStream_t s, size_t work_elems)
 {
     return skinny_f64s(ta, tri, m, k, nr, alpha, A, lda, R, 1, ldr, beta, Y0, 1, ldy0, Y, 1, ldy, work,
-                       skinny_work_elems(m, k, nr), s);
+                       work_elems ? work_elems : skinny_work_elems(m, k, nr), s);
+}
+
+size_t skinny_call_work(Op ta, int m, int k, int nr)
+{
+    if (m <= 0 || k <= 0 || nr <= 0) return 0;
+    const SkinnyPlan p = plan_for(ta, m, k);
+    if (p.slabs == 1) return 0;
+    const int NR = nr == 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : nr <= 8 ? 8 : 16;   // launch_skinny's buckets
+    return (size_t)p.slabs * NR * (size_t)m;
 }
 
 }  // namespace rgf
