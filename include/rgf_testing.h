@@ -14,8 +14,8 @@ extern "C" {
  * arrays, column-major; ta/tb: 0 = N, 1 = T; flags as the kernel's GemmFlags
  * (1 lower-C stores, 2/4 op(A) lower/upper, 8/16 op(B) lower/upper); flag 256 runs the
  * split-BF16 kernel of the mixed-precision path instead of the FP64 one, flag 512
- * the split-BF16 product with operands split into bf16 hi/lo copies first; bits 12-13
- * force the FP64 tile shape (1 = 64x64, 2 = 128x64, 3 = 128x128; 0 automatic).
+ * the split-BF16 product with operands split into bf16 hi/lo copies first; bits 12-14
+ * force the FP64 tile shape (1 = 64x64, 2 = 128x64, 3 = 128x128, 4 = 32x32; 0 automatic).
  * A is (ta ? k x m : m x k) with ld lda, B is (tb ? n x k : k x n) with ld ldb. */
 int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int flags);
@@ -45,7 +45,8 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * same choice for full-tile products with triangular operands; key 11 = the FP64 selected
  * inversion launches X_i^T X_i and W^T Z as one launch of two products (1) or one after the
  * other (0); key 12 = with key 11 on, the FP64 sweep also computes W = F X of the next block
- * in the launch of this block's Z = S W (1) or in a launch of its own (0).
+ * in the launch of this block's Z = S W (1) or in a launch of its own (0); key 13 = FP64 launches of
+ * fewer 64 x 64 workgroups than the value take 32 x 32 tiles (0: never; any value >= 0).
  * Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */

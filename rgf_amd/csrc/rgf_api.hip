@@ -1576,7 +1576,7 @@ extern "C" int rgf_debug_gemm(int ta, int tb, int m, int n, int k, double alpha,
         rc = gemm_bf16x3_split(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, (__bf16*)sc.p, sce, s);
     } else {
         rc = (flags & 0x100) ? gemm_bf16x3(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s)
-                             : gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s, ((flags >> 12) & 3) - 1);
+                             : gemm_f64(ta ? Op::T : Op::N, tb ? Op::T : Op::N, &d, 1, s, ((flags >> 12) & 7) - 1);
     }
     if (rc) return dev_status(nullptr, rc, "debug_gemm");
     e = hipDeviceSynchronize();
@@ -1658,9 +1658,9 @@ extern "C" int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int fla
     auto launch = [&]() {
         const Op oa = ta ? Op::T : Op::N, ob = tb ? Op::T : Op::N;
         if (flags & 0x200) return gemm_bf16x3_split(oa, ob, &d, (__bf16*)sc.p, sce, nullptr);
-        // bits 12-13: FP64 tile shape + 1 (0 = automatic)
+        // bits 12-14: FP64 tile shape + 1 (0 = automatic)
         return (flags & 0x100) ? gemm_bf16x3(oa, ob, &d, 1, nullptr)
-                               : gemm_f64(oa, ob, &d, 1, nullptr, ((flags >> 12) & 3) - 1);
+                               : gemm_f64(oa, ob, &d, 1, nullptr, ((flags >> 12) & 7) - 1);
     };
     int rc = launch();
     hipEvent_t e0, e1;
@@ -1709,6 +1709,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
     if (key == 11 && (value == 0 || value == 1)) {
         const int old = g_selinv_fuse;
         g_selinv_fuse = value;
+        return old;
+    }
+    if (key == 13 && value >= 0) {
+        const int old = g_small_tile_wgs;
+        g_small_tile_wgs = value;
         return old;
     }
     if (key == 9 && (value == 0 || value == 1)) {

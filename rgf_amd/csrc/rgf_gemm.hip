@@ -17,6 +17,7 @@
 #include "rgf_internal.hpp"
 #include "../../include/rgf.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace rgf {
@@ -26,7 +27,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int BK = 16, PADL = 1;
 
 // resident workgroups per CU the register budget of each tile shape allows
-constexpr int gemm_occupancy(int tm, int tn) { return tm * tn >= 128 * 128 ? 2 : (tm * tn >= 128 * 64 ? 3 : 5); }
+constexpr int gemm_occupancy(int tm, int tn)
+{
+    return tm * tn >= 128 * 128 ? 2 : (tm * tn >= 128 * 64 ? 3 : (tm * tn >= 64 * 64 ? 5 : 8));
+}
 
 __device__ __forceinline__ void select_desc(const GemmArgs& args, int z, GemmDesc& D)
 {
@@ -269,17 +273,37 @@ int g_rr_tri = 1;
 int g_selinv_fuse = 1;   // used by the selected-inversion sweep
 int g_selinv_pipe = 1;   // used by the selected-inversion sweep
 int g_rr_bands = 0;
+int g_small_tile_wgs = 1024;   // launches of fewer 64x64 workgroups than this take 32x32 tiles
 
-// Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, -1 = automatic.
+// 64x64 workgroups a launch of these descriptors would have
+static long long wgs64(const GemmDesc* descs, int ndesc)
+{
+    long long w = 0;
+    for (int q = 0; q < ndesc; q++) {
+        const GemmDesc& d = descs[q];
+        if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
+        const long long tm = (d.m + 63) / 64, tn = (d.n + 63) / 64;
+        const long long t = ((d.flags & G_CLOWER) && tm == tn) ? tm * (tm + 1) / 2 : tm * tn;
+        w += t * d.count * std::max(d.count2, 1);
+    }
+    return w;
+}
+
+// Tile shape of a launch: cfg 0 = 64x64, 1 = 128x64, 2 = 128x128, 3 = 32x32, -1 = automatic.
 // Measured on MI355X at the sweep's shapes (ns = 2865, single and 4-matrix batches):
-// 64x64 at 5 workgroups per CU is the fastest or within a few percent everywhere
-// (the larger tiles lose occupancy), so the automatic choice is 64x64.
+// 64x64 at 5 workgroups per CU is the fastest or within a few percent for launches that
+// fill the chip (the larger tiles lose occupancy); launches of fewer than four 64x64
+// workgroups per CU (the upper merges of L^-1, the late rank-128 Cholesky updates) leave
+// SIMDs with one or two waves and run at a fraction of the rate, so they take 32x32 tiles
+// at up to 8 workgroups per CU (c2 FP64 step 0.489 -> 0.463 s; thresholds of 512 .. 1536
+// workgroups within 1 %, 2100 slower).  Every tile shape sums each output over the same k-steps in
+// the same order (tiles only skip k-steps of exact zeros), so the shape never changes a bit.
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
 {
     if (ndesc <= 0) return 0;
     if (ndesc > GEMM_MAXD) return (int)hipErrorInvalidValue;
-    if (cfg < 0) cfg = 0;
-    const int TM = cfg >= 1 ? 128 : 64, TN = cfg == 2 ? 128 : 64;
+    if (cfg < 0) cfg = wgs64(descs, ndesc) < g_small_tile_wgs ? 3 : 0;
+    const int TM = cfg == 3 ? 32 : (cfg >= 1 ? 128 : 64), TN = cfg == 3 ? 32 : (cfg == 2 ? 128 : 64);
     GemmArgs args{};
     int z = 0, maxtiles = 0, nd = 0;
     for (int q = 0; q < ndesc; q++) {
@@ -301,7 +325,8 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
     dim3 grid(maxtiles, z);
     const int cls = (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0);
     void* tok = timing_begin(cls, args.d, nd, s);
-    if (cfg == 2) launch_f64<128, 128>(ta, tb, args, grid, s);
+    if (cfg == 3) launch_f64<32, 32>(ta, tb, args, grid, s);
+    else if (cfg == 2) launch_f64<128, 128>(ta, tb, args, grid, s);
     else if (cfg == 1) launch_f64<128, 64>(ta, tb, args, grid, s);
     else launch_f64<64, 64>(ta, tb, args, grid, s);
     hipError_t err = hipGetLastError();

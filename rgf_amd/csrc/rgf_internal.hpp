@@ -75,13 +75,14 @@ enum class Op { N, T };
 
 // Launch one GEMM "call" made of up to GEMM_MAXD descriptors that share the
 // transposes (all entries are independent).  Returns hipError_t as int.
-// cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128; -1 automatic).
+// cfg selects the tile shape (0 = 64x64, 1 = 128x64, 2 = 128x128, 3 = 32x32; -1 automatic).
 int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg = -1);
 extern int g_gemm_fast_loads;   // 1: GemmArgs::fast_loads set on FP64 launches
 extern int g_selinv_fuse;   // 1: the FP64 selected inversion launches X^T X and W^T Z together
 extern int g_selinv_pipe;   // 1: ... and computes W of the next block in the launch of Z
 extern int g_rr_tri;     // 1: lower-stored launches with triangular operands dealt round-robin
 extern int g_rr_bands;   // 1: the same for full-tile launches with triangular operands
+extern int g_small_tile_wgs;   // automatic tile shape: 32x32 below this many 64x64 workgroups
 // tile order of a launch: triangular enumeration for lower-stored square tilings, and
 // round-robin XCD dealing where the k-range depends on the tile
 inline void set_tile_order(GemmDesc& d, bool tri)

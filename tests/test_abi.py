@@ -95,3 +95,6 @@ def test_tuning_switches_validate_without_device():
         assert lib.rgf_debug_set_tuning(key, cur) == good[1]
     for key in (0, 2, 5, 99):
         assert lib.rgf_debug_set_tuning(key, 1) == -1
+    cur = lib.rgf_debug_set_tuning(13, 0)   # threshold of the 32 x 32 FP64 tiles: any value >= 0
+    assert cur >= 0 and lib.rgf_debug_set_tuning(13, -1) == -1
+    assert lib.rgf_debug_set_tuning(13, cur) == 0

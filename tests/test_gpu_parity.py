@@ -32,10 +32,10 @@ def dptr(a):
 # ------------------------------------------------------------------ kernels
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("m,n,k", [(16, 16, 4), (64, 64, 16), (130, 70, 33), (5, 200, 129), (300, 270, 40)])
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 def test_gemm_kernel(gpu, ta, tb, m, n, k, tile):
     """FP64 product against numpy, for the automatic tile shape (0) and each forced
-    one (1 = 64x64, 2 = 128x64, 3 = 128x128; bits 12-13 of the debug flags)."""
+    one (1 = 64x64, 2 = 128x64, 3 = 128x128, 4 = 32x32; bits 12-14 of the debug flags)."""
     say(f"gemm tile={tile} ta={ta} tb={tb} m={m} n={n} k={k}")
     rng = np.random.default_rng(m * 7 + n + k)
     lda, ldb, ldc = (k if ta else m) + 3, (n if tb else k) + 1, m + 2
@@ -55,7 +55,7 @@ def test_gemm_kernel(gpu, ta, tb, m, n, k, tile):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
 def test_gemm_fast_loads_match_masked_loads(gpu, ta, tb, tile):
     """The FP64 kernel's full-k-step loads (loop-invariant lane offsets, edge rows clamped
     to the last valid row) give the same bits as per-element masked loads, on ragged
@@ -93,7 +93,7 @@ def test_gemm_fast_loads_match_masked_loads(gpu, ta, tb, tile):
 
 
 @pytest.mark.parametrize("flags", [1, 2, 8, 16, 1 | 4 | 8])
-@pytest.mark.parametrize("tile", [0, 2, 3])
+@pytest.mark.parametrize("tile", [0, 2, 3, 4])
 @pytest.mark.parametrize("size", [150, 100, 333])
 def test_gemm_kernel_triangular_flags(gpu, flags, tile, size):
     """Triangle flags (k-range skipping, lower-stored C) against numpy, at sizes that are
@@ -126,6 +126,42 @@ def test_gemm_kernel_triangular_flags(gpu, flags, tile, size):
                                     dptr(B.reshape(-1, order="F").copy()), k, 0.0, dptr(Cf), m, flags | (tile << 12))
     assert rc == 0
     assert np.abs(Cf.reshape(m, n, order="F") - ref).max() < 1e-11
+
+
+@pytest.mark.parametrize("ta,tb,flags", [(0, 0, 0), (0, 1, 1), (1, 0, 1 | 4 | 8), (0, 0, 8), (0, 1, 16), (0, 0, 2),
+                                         (1, 1, 0)])
+@pytest.mark.parametrize("m,n,k", [(333, 333, 333), (200, 130, 97), (64, 64, 128)])
+def test_gemm_tile_shapes_same_bits(gpu, ta, tb, flags, m, n, k):
+    """Every FP64 tile shape sums each output over the same k-steps in the same order (a
+    tile only skips k-steps of exact zeros of a triangular operand), so the automatic
+    choice of shape never changes a bit: all shapes give bitwise the same C."""
+    say(f"gemm tile shapes same bits ta={ta} tb={tb} flags={flags} {m}x{n}x{k}")
+    if flags & (1 | 2 | 4 | 8 | 16) and flags != 0:
+        m = n = k = max(m, n, k)
+    rng = np.random.default_rng(m + n + k + flags)
+    lda, ldb, ldc = (k if ta else m), (n if tb else k), m
+    A = rng.standard_normal((lda, m if ta else k))
+    B = rng.standard_normal((ldb, k if tb else n))
+    # triangular operands hold explicit zeros in their other triangle
+    if flags & 2:
+        A = np.tril(A) if not ta else np.triu(A)
+    if flags & 4:
+        A = np.triu(A) if not ta else np.tril(A)
+    if flags & 8:
+        B = np.tril(B) if not tb else np.triu(B)
+    if flags & 16:
+        B = np.triu(B) if not tb else np.tril(B)
+    C0 = rng.standard_normal(ldc * n)
+    outs = []
+    for tile in (1, 2, 3, 4, 0):
+        Cf = C0.copy()
+        rc = _lib.load().rgf_debug_gemm(ta, tb, m, n, k, 0.75, dptr(A.reshape(-1, order="F").copy()), lda,
+                                        dptr(B.reshape(-1, order="F").copy()), ldb, 0.5, dptr(Cf), ldc,
+                                        flags | (tile << 12))
+        assert rc == 0
+        outs.append(Cf)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
 
 
 @pytest.mark.parametrize("n", [1, 5, 64, 65, 150, 200, 300, 333])
