@@ -12,6 +12,7 @@
 
 #include <sys/time.h>
 
+#include <complex>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -26,6 +27,9 @@ inline double get_time(double t0)
     gettimeofday(&tv, nullptr);
     return (double)tv.tv_sec + (double)tv.tv_usec * 1e-6 - t0;
 }
+
+// Types.H:41-45: the reference's complex scalar
+typedef std::complex<double> CPX;
 
 class rgf_error : public std::runtime_error {
 public:
@@ -84,4 +88,53 @@ private:
     }
     rgf_handle_t h_ = nullptr;
     size_t n_ = 0;
+};
+
+// RGF<CPX>, RGF.H:756-780: the same interface for complex matrices, in the complex-symmetric
+// algebra of the reference's block updates (A = A^T = L L^T; logDet returns the complex
+// 2 sum log L_jj).  std::complex<double> arrays are passed to the C ABI as interleaved
+// (re, im) doubles, the layout the C++ standard gives them.
+template <>
+class RGF<CPX> {
+public:
+    RGF(size_t* ia, size_t* ja, CPX* a, size_t ns, size_t nt, size_t nd, const rgf_options* opts = nullptr)
+    {
+        check(rgf_z_create_csc(ia, ja, reinterpret_cast<const double*>(a), ns, nt, nd, opts, &h_), nullptr);
+    }
+    ~RGF() { if (h_) rgf_z_destroy(h_); }
+    RGF(const RGF&) = delete;
+    RGF& operator=(const RGF&) = delete;
+
+    void factorize() { check(rgf_z_factorize(h_), h_); }
+    void solve(CPX* b, size_t nrhs) { check(rgf_z_solve(h_, dp(b), dp(b), nrhs), h_); }
+    void solve(CPX* x, CPX* b, size_t nrhs) { check(rgf_z_solve(h_, dp(x), dp(b), nrhs), h_); }
+    void RGFdiag(CPX* diag) { check(rgf_z_selinv_diag(h_, dp(diag)), h_); }
+    CPX logDet()
+    {
+        double v[2] = {0.0, 0.0};
+        check(rgf_z_logdet(h_, v), h_);
+        return CPX(v[0], v[1]);
+    }
+    double residualNorm(CPX* x, CPX* b)
+    {
+        double v = 0.0;
+        check(rgf_z_residual_norm(h_, dp(x), dp(b), &v), h_);
+        return v;
+    }
+    double residualNormNormalized(CPX* x, CPX* b)
+    {
+        double v = 0.0;
+        check(rgf_z_residual_norm_normalized(h_, dp(x), dp(b), &v), h_);
+        return v;
+    }
+
+    rgf_zhandle_t handle() const { return h_; }
+
+private:
+    static double* dp(CPX* p) { return reinterpret_cast<double*>(p); }
+    static void check(int rc, rgf_zhandle_t h)
+    {
+        if (rc != RGF_OK) throw rgf_error(rc, rgf_z_last_error(h));
+    }
+    rgf_zhandle_t h_ = nullptr;
 };

@@ -182,6 +182,35 @@ int rgf_export_matrix_mf(rgf_handle_t h, double *mf);
  * (the representation kept on the device), the couplings B_i and arrows D_i as in L. */
 int rgf_export_factor_mf(rgf_handle_t h, double *mf);
 
+/* ---- RGF<std::complex<double>> (RGF.H:756-780 instantiates the class for CPX) ----
+ * Complex-symmetric algebra, as the reference's complex block updates (transposes without
+ * conjugation, RGF.H:260, 276-308): A = A^T, A = L L^T with L_jj the principal square root
+ * of each pivot, logDet = 2 sum log L_jj (principal logarithms, so its imaginary part is not
+ * reduced modulo 2 pi).  Complex arrays are interleaved (re, im) doubles, the layout of
+ * std::complex<double>; the CSC arrays and shapes are as rgf_create_csc's.  FP64 only; the
+ * matrix stays resident (residual, re-factorization); RGF_ENOTSPD reports a zero or
+ * non-finite pivot (1-based column in rgf_z_last_error).  The handle is independent of the
+ * FP64 handle type. */
+typedef struct rgf_zhandle_s *rgf_zhandle_t;
+/* RGF<CPX>::RGF(size_t* ia, size_t* ja, CPX* a, size_t ns, size_t nt, size_t nd), RGF.H:85-146 */
+int rgf_z_create_csc(const size_t *ia, const size_t *ja, const double *a, size_t ns, size_t nt, size_t nd,
+                     const rgf_options *opts, rgf_zhandle_t *out);
+/* RGF<CPX>::~RGF(), RGF.H:151-170 */
+int rgf_z_destroy(rgf_zhandle_t h);
+/* RGF<CPX>::factorize(), RGF.H:510-546 */
+int rgf_z_factorize(rgf_zhandle_t h);
+/* RGF<CPX>::solve(CPX* x, CPX* b, size_t nrhs), RGF.H:550-578: n x nrhs, column-major, ld n */
+int rgf_z_solve(rgf_zhandle_t h, double *x, const double *b, size_t nrhs);
+/* CPX RGF<CPX>::logDet(), RGF.H:616-627: out[0] = real part, out[1] = imaginary part */
+int rgf_z_logdet(rgf_zhandle_t h, double *out);
+/* RGF<CPX>::RGFdiag(CPX* diag), RGF.H:582-612: diag(A^-1), n complex values */
+int rgf_z_selinv_diag(rgf_zhandle_t h, double *diag);
+/* RGF<CPX>::residualNorm(CPX* x, CPX* b), RGF.H:631-655: ||b - A x||_2 */
+int rgf_z_residual_norm(rgf_zhandle_t h, const double *x, const double *b, double *out);
+/* RGF<CPX>::residualNormNormalized(CPX* x, CPX* b), RGF.H:659-663 */
+int rgf_z_residual_norm_normalized(rgf_zhandle_t h, const double *x, const double *b, double *out);
+const char *rgf_z_last_error(rgf_zhandle_t h);
+
 /* Per-process GEMM kernel timing with HIP events on the launching stream:
  * op_class 0 = NN, 1 = NT, 2 = TN, 3 = TT (FP64), 4..7 the same on split-BF16,
  * 8 = every split-BF16 class, 9 = every FP64 class, -1 = off.  read() synchronizes and

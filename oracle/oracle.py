@@ -320,3 +320,139 @@ def reference_selinv_dense(A: np.ndarray, ns: int, nt: int, nd: int) -> np.ndarr
         Snext = X.T @ T @ X
         out[sl] = np.diag(Snext)
     return out
+
+
+# ---------------------------------------------------------------- complex (RGF<CPX>)
+# The reference's complex instantiation (RGF.H:756-780) runs the same sweep with block
+# updates that transpose without conjugating (trsm 'R','L','T','N' at RGF.H:260, 287, 317;
+# gemm 'N','T' at RGF.H:276-308; selinv gemm 'T','N' at RGF.H:442-501): complex-symmetric
+# algebra, A = A^T = L L^T.  Restated here with dense numpy blocks (test infrastructure).
+def zchol_sym(S: np.ndarray) -> np.ndarray:
+    """Complex-symmetric Cholesky L L^T = S (lower part of S read), principal square root
+    of each pivot (the diagonal-block potrf of RGF.H:254, 284, 312, 335 in that algebra)."""
+    S = np.tril(np.array(S, dtype=np.complex128, copy=True))
+    n = S.shape[0]
+    L = np.zeros_like(S)
+    for j in range(n):
+        d = S[j, j]
+        if d == 0 or not np.isfinite(d):
+            raise np.linalg.LinAlgError(f"zero pivot at column {j + 1}")
+        s = np.sqrt(d)
+        L[j, j] = s
+        L[j + 1:, j] = S[j + 1:, j] / s
+        S[j + 1:, j + 1:] -= np.tril(np.outer(L[j + 1:, j], L[j + 1:, j]))
+    return L
+
+
+def zfactor_blocks(A: np.ndarray, ns: int, nt: int, nd: int) -> dict:
+    """FirstStageFactor (RGF.H:233-340) in complex-symmetric algebra on dense blocks of A:
+    S_i = A_ii - B_{i-1} B_{i-1}^T, L_ii = chol(S_i), B_i = A_{i+1,i} L_ii^-T,
+    D_i = (A_di - D_{i-1} B_{i-1}^T) L_ii^-T, L_dd = chol(A_dd - sum D_i D_i^T)."""
+    A = np.asarray(A, dtype=np.complex128)
+    blk = lambda r, c: A[r * ns:(r + 1) * ns, c * ns:(c + 1) * ns]
+    Lii, Bs, Ds = [], [], []
+    Sdd = A[nt * ns:, nt * ns:].copy() if nd > 0 else None
+    for i in range(nt):
+        S = blk(i, i).copy()
+        Ad = A[nt * ns:, i * ns:(i + 1) * ns].copy() if nd > 0 else None
+        if i > 0:
+            S -= Bs[i - 1] @ Bs[i - 1].T
+            if nd > 0:
+                Ad -= Ds[i - 1] @ Bs[i - 1].T
+        L = zchol_sym(S)
+        X = np.linalg.inv(L)
+        Lii.append(L)
+        if i < nt - 1:
+            Bs.append(blk(i + 1, i) @ X.T)
+        if nd > 0:
+            D = Ad @ X.T
+            Ds.append(D)
+            Sdd -= D @ D.T
+    Ldd = zchol_sym(Sdd) if nd > 0 else None
+    return {"ns": ns, "nt": nt, "nd": nd, "L": Lii, "B": Bs, "D": Ds, "Ldd": Ldd}
+
+
+def zblocks_to_dense(F: dict) -> np.ndarray:
+    """The dense lower-triangular factor L of zfactor_blocks (for L L^T = A)."""
+    ns, nt, nd = F["ns"], F["nt"], F["nd"]
+    n = ns * nt + nd
+    L = np.zeros((n, n), dtype=np.complex128)
+    for i in range(nt):
+        L[i * ns:(i + 1) * ns, i * ns:(i + 1) * ns] = F["L"][i]
+        if i < nt - 1:
+            L[(i + 1) * ns:(i + 2) * ns, i * ns:(i + 1) * ns] = F["B"][i]
+        if nd > 0:
+            L[nt * ns:, i * ns:(i + 1) * ns] = F["D"][i]
+    if nd > 0:
+        L[nt * ns:, nt * ns:] = F["Ldd"]
+    return L
+
+
+def zlogdet(F: dict) -> complex:
+    """logDet (RGF.H:616-627, Blas.H:1084-1092): 2 sum log L_jj, principal logarithms."""
+    s = sum(np.log(np.diag(L)).sum() for L in F["L"])
+    if F["nd"] > 0:
+        s += np.log(np.diag(F["Ldd"])).sum()
+    return complex(2.0 * s)
+
+
+def zsolve(F: dict, b: np.ndarray) -> np.ndarray:
+    """SecondStageSolve (RGF.H:344-418): forward y_i = L_ii^-1 (b_i - B_{i-1} y_{i-1}),
+    y_d = L_dd^-1 (b_d - sum D_i y_i); backward x_d = L_dd^-T y_d,
+    x_i = L_ii^-T (y_i - B_i^T x_{i+1} - D_i^T x_d)."""
+    ns, nt, nd = F["ns"], F["nt"], F["nd"]
+    b = np.asarray(b, dtype=np.complex128)
+    shape = b.shape
+    b = b.reshape(ns * nt + nd, -1)
+    y = np.zeros_like(b)
+    x = np.zeros_like(b)
+    for i in range(nt):
+        r = b[i * ns:(i + 1) * ns].copy()
+        if i > 0:
+            r -= F["B"][i - 1] @ y[(i - 1) * ns:i * ns]
+        y[i * ns:(i + 1) * ns] = np.linalg.solve(F["L"][i], r)
+    if nd > 0:
+        r = b[nt * ns:].copy()
+        for i in range(nt):
+            r -= F["D"][i] @ y[i * ns:(i + 1) * ns]
+        y[nt * ns:] = np.linalg.solve(F["Ldd"], r)
+        x[nt * ns:] = np.linalg.solve(F["Ldd"].T, y[nt * ns:])
+    for i in range(nt - 1, -1, -1):
+        r = y[i * ns:(i + 1) * ns].copy()
+        if i < nt - 1:
+            r -= F["B"][i].T @ x[(i + 1) * ns:(i + 2) * ns]
+        if nd > 0:
+            r -= F["D"][i].T @ x[nt * ns:]
+        x[i * ns:(i + 1) * ns] = np.linalg.solve(F["L"][i].T, r)
+    return x.reshape(shape)
+
+
+def zselinv(F: dict, exact: bool = True) -> np.ndarray:
+    """diag(A^-1) by the Takahashi recursion (ThirdStageRGF, RGF.H:422-506) with X_i = L_ii^-1:
+    S_dd = X_dd^T X_dd; S_ii = X_i^T [I + B_i^T S_{i+1} B_i + D_i^T S_dd D_i] X_i, plus with
+    `exact` the arrow cross terms and S_{d,i} = -(S_{d,i+1} G_i + S_dd H_i) that the
+    reference's formula leaves out (exact for every nd)."""
+    ns, nt, nd = F["ns"], F["nt"], F["nd"]
+    out = np.zeros(ns * nt + nd, dtype=np.complex128)
+    Sdd = Sd_next = S_next = None
+    if nd > 0:
+        Xdd = np.linalg.inv(F["Ldd"])
+        Sdd = Xdd.T @ Xdd
+        out[nt * ns:] = np.diag(Sdd)
+    for i in range(nt - 1, -1, -1):
+        X = np.linalg.inv(F["L"][i])
+        S = X.T @ X
+        Sd = None
+        if nd > 0:
+            H = F["D"][i] @ X
+            S = S + H.T @ Sdd @ H
+            Sd = -(Sdd @ H)
+        if i < nt - 1:
+            G = F["B"][i] @ X
+            S = S + G.T @ S_next @ G
+            if nd > 0 and exact:
+                S = S + G.T @ Sd_next.T @ H + H.T @ Sd_next @ G
+                Sd = Sd - Sd_next @ G
+        out[i * ns:(i + 1) * ns] = np.diag(S)
+        S_next, Sd_next = S, Sd
+    return out

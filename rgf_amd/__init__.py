@@ -17,7 +17,7 @@ from ._lib import (RGF_EDEVICE, RGF_EINVAL, RGF_ENOMEM, RGF_ENOTSPD, RGF_ESTATE,
                    RGF_MIXED_BF16X3, RGF_OK, RGF_SELINV_EXACT, RGF_SELINV_REFERENCE,
                    DeviceBuffer, GenParams, Info, Options, gpu_count)
 
-__all__ = ["RGF", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
+__all__ = ["RGF", "ZRGF", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
            "default_gen_params", "gpu_count", "RGF_FP64", "RGF_MIXED_BF16X3",
            "RGF_SELINV_EXACT", "RGF_SELINV_REFERENCE", "RGF_OK", "RGF_EINVAL", "RGF_ENOTSPD",
            "RGF_ENOMEM", "RGF_EDEVICE", "RGF_ESTATE", "mf_size", "canonical_flops"]
@@ -241,6 +241,99 @@ class RGF:
     def close(self) -> None:
         if self._h:
             _lib.load().rgf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _zcheck(rc: int, h) -> None:
+    if rc != RGF_OK:
+        msg = _lib.load().rgf_z_last_error(h)
+        raise RGFError(rc, msg.decode() if msg else "")
+
+
+def _zptr(a: np.ndarray):
+    assert a.dtype == np.complex128 and (a.flags.c_contiguous or a.flags.f_contiguous)
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class ZRGF:
+    """RGF<std::complex<double>> of the reference (RGF.H:756-780) on one MI355X, in the
+    complex-symmetric algebra of its block updates (A = A^T = L L^T).  Same constructor
+    (lower-triangle CSC, complex values) and methods as RGF; logDet returns a complex."""
+
+    def __init__(self, ia, ja, a, ns: int, nt: int, nd: int, options: Options | None = None):
+        lib = _lib.load()
+        self.ns, self.nt, self.nd = int(ns), int(nt), int(nd)
+        self.n = self.ns * self.nt + self.nd
+        self._h = C.c_void_p()
+        ia = np.ascontiguousarray(ia, dtype=np.uint64)
+        ja = np.ascontiguousarray(ja, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if ia.size != self.n + 1:
+            raise RGFError(RGF_EINVAL, f"ia has {ia.size} entries, expected n+1 = {self.n + 1}")
+        szp = C.POINTER(C.c_size_t)
+        rc = lib.rgf_z_create_csc(ia.ctypes.data_as(szp), ja.ctypes.data_as(szp), _zptr(a), self.ns, self.nt,
+                                  self.nd, C.byref(options) if options is not None else None, C.byref(self._h))
+        _zcheck(rc, None)
+
+    def factorize(self) -> None:                      # RGF.H:510-546
+        _zcheck(_lib.load().rgf_z_factorize(self._h), self._h)
+
+    def solve(self, x, b=None, nrhs: int | None = None):
+        """solve(b, nrhs) or solve(x, b, nrhs) (RGF.H:550-578); column-major n x nrhs."""
+        if b is None or isinstance(b, (int, np.integer)):
+            if isinstance(b, (int, np.integer)):
+                nrhs = int(b)
+            b = x
+        bb = np.asfortranarray(b, dtype=np.complex128)
+        if nrhs is None:
+            nrhs = 1 if bb.ndim == 1 else bb.shape[1]
+        if bb.size != self.n * nrhs:
+            raise RGFError(RGF_EINVAL, f"rhs has {bb.size} values, expected n*nrhs = {self.n * nrhs}")
+        out = np.empty_like(bb)
+        _zcheck(_lib.load().rgf_z_solve(self._h, _zptr(out), _zptr(bb), nrhs), self._h)
+        if x is not None and isinstance(x, np.ndarray) and x.flags.writeable and x.size == out.size:
+            x.reshape(-1, order="F")[:] = out.reshape(-1, order="F")
+        return out
+
+    def RGFdiag(self, diag: np.ndarray | None = None) -> np.ndarray:   # RGF.H:582-612
+        out = np.empty(self.n, dtype=np.complex128) if diag is None else diag
+        _zcheck(_lib.load().rgf_z_selinv_diag(self._h, _zptr(out)), self._h)
+        return out
+
+    def logDet(self) -> complex:                      # RGF.H:616-627
+        v = np.zeros(2)
+        _zcheck(_lib.load().rgf_z_logdet(self._h, _dptr(v)), self._h)
+        return complex(v[0], v[1])
+
+    def residualNorm(self, x, b) -> float:            # RGF.H:631-655
+        v = C.c_double()
+        _zcheck(_lib.load().rgf_z_residual_norm(self._h, _zptr(np.ascontiguousarray(x, dtype=np.complex128)),
+                                                _zptr(np.ascontiguousarray(b, dtype=np.complex128)), C.byref(v)),
+                self._h)
+        return v.value
+
+    def residualNormNormalized(self, x, b) -> float:  # RGF.H:659-663
+        v = C.c_double()
+        _zcheck(_lib.load().rgf_z_residual_norm_normalized(
+            self._h, _zptr(np.ascontiguousarray(x, dtype=np.complex128)),
+            _zptr(np.ascontiguousarray(b, dtype=np.complex128)), C.byref(v)), self._h)
+        return v.value
+
+    def close(self) -> None:
+        if self._h:
+            _lib.load().rgf_z_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -29,6 +29,8 @@ EXPORTS = [
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats", "rgf_set_device_cache_limit",
     "rgf_trim_device_cache", "rgf_solve_device_noalloc",
+    "rgf_z_create_csc", "rgf_z_destroy", "rgf_z_factorize", "rgf_z_solve", "rgf_z_logdet", "rgf_z_selinv_diag",
+    "rgf_z_residual_norm", "rgf_z_residual_norm_normalized", "rgf_z_last_error",
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
@@ -108,6 +110,17 @@ def load():
         lib.rgf_refine_stats.argtypes = [H, C.POINTER(C.c_int), dp]
         lib.rgf_selinv_diag_batch.argtypes = [H, dp]
         lib.rgf_set_device_cache_limit.argtypes = [C.c_size_t, C.POINTER(C.c_size_t)]
+        lib.rgf_z_create_csc.argtypes = [szp, szp, dp, C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(Options),
+                                         C.POINTER(H)]
+        lib.rgf_z_destroy.argtypes = [H]
+        lib.rgf_z_factorize.argtypes = [H]
+        lib.rgf_z_solve.argtypes = [H, dp, dp, C.c_size_t]
+        lib.rgf_z_logdet.argtypes = [H, dp]
+        lib.rgf_z_selinv_diag.argtypes = [H, dp]
+        lib.rgf_z_residual_norm.argtypes = [H, dp, dp, dp]
+        lib.rgf_z_residual_norm_normalized.argtypes = [H, dp, dp, dp]
+        lib.rgf_z_last_error.argtypes = [H]
+        lib.rgf_z_last_error.restype = C.c_char_p
         lib.rgf_trim_device_cache.argtypes = [C.c_int]
         lib.rgf_debug_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int,
                                        dp, C.c_int, C.c_double, dp, C.c_int, C.c_int]
@@ -125,7 +138,7 @@ def load():
         lib.rgf_debug_set_tuning.restype = C.c_int
         for name in EXPORTS:
             getattr(lib, name).restype = getattr(lib, name).restype if name in (
-                "rgf_last_error", "rgf_default_options", "rgf_default_gen_params") else C.c_int
+                "rgf_last_error", "rgf_z_last_error", "rgf_default_options", "rgf_default_gen_params") else C.c_int
         lib.rgf_footprint_bytes.restype = C.c_size_t
         _lib = lib
         return lib

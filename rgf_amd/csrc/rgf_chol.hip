@@ -536,15 +536,19 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, lon
 }
 
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
-                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod)
+                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod, long long plane)
 {
     auto run = [&](GemmDesc* d, int nd) {
         return prod ? prod->fn(prod->ctx, Op::N, Op::N, d, nd, s) : gemm_f64(Op::N, Op::N, d, nd, s);
     };
     const int T = (n + NB - 1) / NB;
-    hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, xdiag, sX);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return (int)e;
+    for (int pl = 0; pl < (plane ? 2 : 1); pl++) {   // the diagonal tiles of each plane
+        const long long o = pl * plane;
+        hipLaunchKernelGGL(trtri_diag_kernel, dim3(T * T, count), dim3(256), 0, s, A + o, lda, sA, n, T,
+                           xdiag ? xdiag + o : nullptr, sX);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+    }
     for (int sz = 1; sz < T; sz *= 2) {
         const int G = (T - sz + 2 * sz - 1) / (2 * sz);   // groups with a non-empty right half
         const int n1 = sz * NB;

@@ -152,9 +152,12 @@ struct ProdFn {
     void* ctx;
 };
 // In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.  The
-// merge products run through `prod` (null: FP64 GEMM).
+// merge products run through `prod` (null: FP64 GEMM).  plane != 0: complex L stored as
+// two planes (imaginary part `plane` elements after the real part, for A, tmp and xdiag
+// alike); the diagonal tiles of both planes are placed and `prod` multiplies complex.
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
-                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod = nullptr);
+                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod = nullptr,
+                      long long plane = 0);
 int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
            hipStream_t s, int count = 1, long long sS = 0, long long sD = 0);

@@ -76,3 +76,37 @@ def spde_like(nx: int = 6, ny: int = 7, nt: int = 3, nd: int = 0, kappa2: float 
 def closed_form(ns: int, nt: int, nd: int):
     ia, ja, a = oracle.closed_form_csc(ns, nt, nd)
     return ia, ja, a
+
+
+def complex_symmetric(ns: int, nt: int, nd: int, seed: int = 0, imag_scale: float = 0.4):
+    """A complex-symmetric (A = A^T) block-arrow matrix: the generated real benchmark matrix
+    plus i times a symmetric N(0, imag_scale^2) matrix on the same pattern (and an imaginary
+    diagonal), so that every pivot of L L^T is complex and nonzero.  Returns dense A."""
+    A = oracle.mf_to_dense(oracle.generate(seed, ns, nt, nd), ns, nt, nd).astype(np.complex128)
+    rng = np.random.default_rng(seed + 17)
+    n = A.shape[0]
+    M = np.zeros((n, n))
+    for c in range(n):
+        bc = c // ns
+        for r in range(c, n):
+            br = r // ns
+            if c >= ns * nt or r >= ns * nt or br == bc or br == bc + 1:
+                M[r, c] = rng.standard_normal()
+    M = np.tril(M) + np.tril(M, -1).T
+    return A + 1j * imag_scale * M
+
+
+def complex_csc_lower(A: np.ndarray, ns: int, nt: int, nd: int):
+    """Lower-triangle CSC (uint64 indices, complex values) of the block-arrow pattern of A."""
+    n = A.shape[0]
+    ia, ja, a = [0], [], []
+    for c in range(n):
+        bc = c // ns
+        for r in range(c, n):
+            br = r // ns
+            if c >= ns * nt or r >= ns * nt or br == bc or br == bc + 1:
+                if A[r, c] != 0:
+                    ja.append(r)
+                    a.append(A[r, c])
+        ia.append(len(ja))
+    return np.array(ia, dtype=np.uint64), np.array(ja, dtype=np.uint64), np.array(a, dtype=np.complex128)

@@ -75,3 +75,12 @@ def test_reference_style_caller_runs(gpu, tmp_path):
     assert np.abs(x - xg).max() <= 1e-12 * np.abs(xg).max()
     d = np.array([float(ln.split()[1]) for ln in lines if ln.startswith("diag ")])
     assert np.abs(d / np.array(g["diag_inv"]) - 1).max() <= 1e-12
+
+
+def test_complex_caller_builds_against_rgf_hpp(tmp_path):
+    """RGF<CPX> (RGF.H:756-780) through RGF.hpp: a complex caller compiles and links."""
+    exe = tmp_path / "dropin_complex"
+    r = subprocess.run(["g++", "-std=c++11", "-O1", "-Wall", "-Wno-unused-result", "-I", INC,
+                        os.path.join(ROOT, "tests", "native", "dropin_complex.cpp"), "-o", str(exe),
+                        "-L", LIBDIR, "-lrgf_hip", "-Wl,-rpath," + LIBDIR], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
