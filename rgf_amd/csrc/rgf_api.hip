@@ -221,7 +221,9 @@ int validate_shape(size_t ns, size_t nt, size_t nd)
 // (every product has m, n, k <= ns + nd)
 size_t split_need(int ns, int nd, int count)
 {
-    return split_scratch_elems(ns + nd, ns + nd, ns + nd, count);
+    // two operands of the largest product, plus the copy of Z = Sb W that the selected
+    // inversion's product writes for W^T Z
+    return split_scratch_elems(ns + nd, ns + nd, ns + nd, count) + (size_t)count * split_entry_elems(ns + nd, ns + nd);
 }
 size_t split_x_need(int ns, int count) { return (size_t)count * split_entry_elems(ns, ns); }
 
@@ -413,11 +415,6 @@ int split_a(rgf_handle_s* h, Op ta, const GemmDesc& d, __bf16* dst)
 {
     return split_operand(d.A, d.lda, d.m, d.k, ta == Op::T, d.count, d.sA, d.count2, d.sA2, dst, h->stream);
 }
-int split_b(rgf_handle_s* h, Op tb, const GemmDesc& d, __bf16* dst)
-{
-    return split_operand(d.B, d.ldb, d.n, d.k, tb == Op::N, d.count, d.sB, d.count2, d.sB2, dst, h->stream);
-}
-
 // A product whose two operands have the same split copy (op(A) = op(B)^T, e.g. the
 // Schur update B B^T): split once
 int prod_same(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d)
@@ -451,12 +448,25 @@ int factor_sweep(rgf_handle_s* h)
     int rc;
     hipLaunchKernelGGL(set_int_kernel, dim3((B + 255) / 256), dim3(256), 0, s, h->dinfo, B, INT_MAX);
     if ((rc = (int)hipGetLastError())) return rc;
+    // F = [A_{i+1,i}; A_di - D_{i-1} B_{i-1}^T], the operand of [B_i; D_i] = F X_i^T: staged in
+    // Ftmp, except where that product splits its operands first (the mixed path's pre-split
+    // products): then F is assembled in its own rows of the panel, which the product
+    // overwrites only after the split -- no copy of the coupling block
+    auto f_in_panel = [&](int i) {
+        const int mF = (i < nt - 1 ? ns : 0) + nd;
+        return h->opt.precision == RGF_MIXED_BF16X3 && mF >= 128 && ns >= 128;
+    };
     for (int i = 0; i < nt; i++) {
         double* P = l_panel(h, i);
         double* A = a_panel(h, i);
+        const bool fp = f_in_panel(i);
+        double* const Fd = fp ? P + 2 * ns : h->Ftmp + ns;   // arrow rows of F
+        const int ldF = fp ? ldp : ldf;
+        const long long sF = fp ? ms : fs;
         if (i == 0) {
             if (h->Ap && (rc = copy2d(A, ldp, P, ldp, ns, ns, 1.0, s, B, ms, ms))) return rc;
-            if (nd > 0 && (rc = copy2d(A + 2 * ns, ldp, h->Ftmp + ns, ldf, nd, ns, 1.0, s, B, ms, fs))) return rc;
+            if (nd > 0 && Fd != A + 2 * ns && (rc = copy2d(A + 2 * ns, ldp, Fd, ldF, nd, ns, 1.0, s, B, ms, sF)))
+                return rc;
         } else {
             const double* Pm = l_panel(h, i - 1);
             GemmDesc d = gdesc(Pm + ns, ldp, Pm + ns, ldp, P, ldp, ns, ns, ns, -1.0, 1.0, G_CLOWER);
@@ -468,8 +478,8 @@ int factor_sweep(rgf_handle_s* h)
             for (int c0 = 0; c0 < nd; c0 += 16) {
                 const int nc = std::min(16, nd - c0);
                 rc = skinny_f64s(Op::N, 0, ns, ns, nc, -1.0, Pm + ns, ldp, Pm + 2 * ns + c0, ldp, 1, 1.0,
-                                 A + 2 * ns + c0, ldp, 1, h->Ftmp + ns + c0, ldf, 1, h->work, h->work_elems, s, B,
-                                 ms, ms, ms, fs);
+                                 A + 2 * ns + c0, ldp, 1, Fd + c0, ldF, 1, h->work, h->work_elems, s, B,
+                                 ms, ms, ms, sF);
                 if (rc) return rc;
             }
             if ((rc = prod_same(h, Op::N, Op::T, &d))) return rc;
@@ -481,13 +491,14 @@ int factor_sweep(rgf_handle_s* h)
                                     h->opt.precision == RGF_MIXED_BF16X3 ? &pf : nullptr)))
             return rc;
         const bool has_next = i < nt - 1;
-        if (has_next && (rc = copy2d(A + ns, ldp, h->Ftmp, ldf, ns, ns, 1.0, s, B, ms, fs))) return rc;
+        double* const Fc = fp ? P + ns : h->Ftmp;   // coupling rows of F
+        if (has_next && Fc != A + ns && (rc = copy2d(A + ns, ldp, Fc, ldF, ns, ns, 1.0, s, B, ms, sF))) return rc;
         const int mF = (has_next ? ns : 0) + nd;
         if (mF > 0) {
-            const double* src = h->Ftmp + (has_next ? 0 : ns);
+            const double* src = has_next ? Fc : Fd;
             double* dst = P + (has_next ? ns : 2 * ns);
-            GemmDesc d = gdesc(src, ldf, P, ldp, dst, ldp, mF, ns, ns, 1.0, 0.0, G_BUPPER);
-            batch2(d, B, fs, ms, ms);
+            GemmDesc d = gdesc(src, ldF, P, ldp, dst, ldp, mF, ns, ns, 1.0, 0.0, G_BUPPER);
+            batch2(d, B, sF, ms, ms);
             if ((rc = prod(h, Op::N, Op::T, &d, 1))) return rc;
         }
     }
@@ -790,6 +801,7 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
     // second half of h->split holds W, the first half the operand used once.
     __bf16* const rT = h->split;
     __bf16* const rW = h->split ? h->split + (size_t)B * split_entry_elems(m, m) : nullptr;
+    __bf16* const rZ = h->split ? h->split + 2 * (size_t)B * split_entry_elems(m, m) : nullptr;
     int cur = 0;
     // FP64 pipeline: W_j = F_j X_j needs only the factor, so W of the next block is
     // computed in the launch of this block's Z = Sc W (two descriptors), alternating
@@ -846,17 +858,17 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
             batch2(d, B, ms, ms, fs);
             const bool pw = px && presplit_ok(h, d);
-            if (pw) {   // op(B) = X: the split of X^T X's op(A)
-                if ((rc = split_a(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, h->split_x, s)))
+            if (pw) {   // op(B) = X: the split of X^T X's op(A); the product writes the split of W
+                if ((rc = split_a(h, Op::N, d, rT)) ||
+                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, h->split_x, s, rW)))
                     return rc;
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
             }
             d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
             batch2(d, B, ss, fs, fs);
-            if (pw) {
-                if ((rc = split_b(h, Op::N, d, rW)) || (rc = split_a(h, Op::N, d, rT)) ||
-                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, rW, s)))
+            if (pw) {   // op(B) = W from the W product; the product writes the split of Z
+                if ((rc = split_a(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, rW, s, rZ)))
                     return rc;
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
@@ -872,9 +884,8 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             }
             d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, fuse ? 0.0 : 1.0, G_CLOWER);
             batch2(d, B, fs, fs, ss);
-            if (pw) {   // op(A) = W^T: the split of Z = Sb W's op(B)
-                if ((rc = split_b(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::T, Op::N, &d, rW, rT, s)))
-                    return rc;
+            if (pw) {   // op(A) = W^T: the split of Z = Sb W's op(B); op(B) = Z from the Z product
+                if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &d, rW, rZ, s))) return rc;
             } else if (fuse) {
                 GemmDesc two[2] = {dx, d};
                 two[0].C = Sc;

@@ -330,11 +330,17 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
 // (chunk ^ (row / (16 / CPR)) mod CPR): any 16 lanes that store 16 chunks of 16 / CPR
 // consecutive rows, or read one chunk of 16 consecutive rows, touch 16 distinct
 // 16-byte bank groups.  Without SW rows are padded by 16 bytes.
+// Cs (may be null): the epilogue also writes the split of the stored C, op(C)^T layout --
+// row j of the copy is column j of C, k-contiguous over C's rows, pitch ldkc, zero for rows
+// M .. ldkc-1 -- at Cs + entry * sEC (lo at + loC): what split_operand would write for C used
+// as op(B) (N) or op(A) (T) of a later product, from the same values, so it replaces that pass.
 template <int BT, int KB = 32, bool SW = false>
 __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
                                                                               const __bf16* Bs, long long sEA,
                                                                               long long sEB, long long loA,
-                                                                              long long loB, int ldka, int ldkb)
+                                                                              long long loB, int ldka, int ldkb,
+                                                                              __bf16* Cs, long long sEC,
+                                                                              long long loC, int ldkc)
 {
     constexpr int WT = BT / 2;
     constexpr int NF = WT / 16;
@@ -474,6 +480,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
+    __bf16* const Ce = Cs ? Cs + (long long)(D.zbase + bz) * sEC : nullptr;
 #pragma unroll
     for (int a = 0; a < NF; a++) {
         const int gi = i0 + wm * WT + a * 16 + l16;
@@ -482,10 +489,17 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int gj = j0 + wn * WT + b * 16 + 4 * lq + r;
+                double v = 0.0;
                 if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
-                    double v = alpha * (double)acc[a][b][r];
+                    v = alpha * (double)acc[a][b][r];
                     if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
                     C[(long long)gj * ldc + gi] = v;
+                }
+                if (Ce && gi < ldkc && gj < N) {
+                    __bf16 h, l;
+                    split_bf16(v, h, l);   // rows beyond M: zero (the k padding of the copy)
+                    Ce[(long long)gj * ldkc + gi] = h;
+                    Ce[loC + (long long)gj * ldkc + gi] = l;
                 }
             }
         }
@@ -526,7 +540,8 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
     return (int)hipGetLastError();
 }
 
-int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, const __bf16* Bs, hipStream_t s)
+int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, const __bf16* Bs, hipStream_t s,
+                         __bf16* Cs)
 {
     GemmDesc d = *d0;
     if (d.count2 < 1) d.count2 = 1;
@@ -547,9 +562,13 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     dim3 grid(nt, entries);
     void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
     const long long a1 = 2 * mp * ldk, b1 = 2 * np * ldk, a2 = mp * ldk, b2 = np * ldk;
+    // the split of C as op(B) (N) / op(A) (T) of a later product: n rows of k = m
+    const int ldkc = (d.m + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+    const long long c1 = 2 * np * ldkc, c2 = np * ldkc;
     const bool k64 = g_presplit_kb == 64, sw = g_presplit_swizzle != 0;
 #define RGF_PRESPLIT(bt, kb, swz) \
-    hipLaunchKernelGGL((gemm_bf16x3s_kernel<bt, kb, swz>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, ldk)
+    hipLaunchKernelGGL((gemm_bf16x3s_kernel<bt, kb, swz>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, \
+                       ldk, Cs, c1, c2, ldkc)
     if (BT == 128) {
         if (k64) { if (sw) RGF_PRESPLIT(128, 64, true); else RGF_PRESPLIT(128, 64, false); }
         else     { if (sw) RGF_PRESPLIT(128, 32, true); else RGF_PRESPLIT(128, 32, false); }

@@ -109,7 +109,11 @@ int gemm_bf16x3_split(Op ta, Op tb, GemmDesc* d, __bf16* scratch, size_t scratch
 size_t split_entry_elems(int rows, int k);
 int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int count, long long s1, int count2,
                   long long s2, __bf16* dst, hipStream_t s);
-int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s);
+// Cs (may be null): the product also writes the bf16 hi/lo split of its stored C into Cs, as
+// split_operand(C, ldc, n, m, kcontig = true, ...) would (one entry per matrix of the
+// descriptor, split_entry_elems(n, m) each): C used as op(B) (N) or op(A) (T) of a later product.
+int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s,
+                         __bf16* Cs = nullptr);
 extern int g_presplit_kb;   // k per LDS stage of gemm_bf16x3_presplit (32 or 64)
 extern int g_presplit_swizzle;   // 1: its LDS rows unpadded with XOR-swizzled chunks
 
