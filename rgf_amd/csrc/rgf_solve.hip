@@ -5,8 +5,11 @@
 // solve is a product here and the whole solve is a stream over the factor.
 //
 // N: one lane per row of op(A) (coalesced column reads, 8 columns in flight), the
-//    R chunk staged in LDS; k split into slabs across workgroups for memory-level
-//    parallelism, partials reduced in a fixed order (deterministic).
+//    R chunk staged in LDS; with one or two right-hand sides 64 rows per workgroup, four
+//    lanes per row each over a quarter of every k-chunk (four times the workgroups: the
+//    launch is otherwise too small to keep HBM busy); k split into slabs across
+//    workgroups for memory-level parallelism, partials reduced in a fixed order
+//    (deterministic).
 // T: one wave per output row (a column of A read contiguously by 64 lanes, 4 loads
 //    in flight), wave-level reduction; k split into slabs the same way.
 #include "rgf_internal.hpp"
@@ -17,6 +20,7 @@ namespace rgf {
 constexpr int MAXR = 16;
 constexpr int KCH = 128;   // k-chunk of R staged in LDS (N kernel)
 constexpr int KU = 8;      // A loads in flight per lane
+constexpr int NROWS = 64;  // rows of op(A) per workgroup of the N kernel
 constexpr int KCT = 256;   // k-chunk of the T kernel (4 per lane x 64 lanes)
 constexpr int TARGET_WG = 2048;
 
@@ -39,14 +43,20 @@ static SkinnyPlan plan(int wgm, int k, int chunk)
     return {std::max(slabs, 1), kslab};
 }
 
-static SkinnyPlan plan_for(Op ta, int m, int k)
+// the N kernel takes NROWS rows per workgroup for one or two right-hand sides (its R
+// staging would dominate at more), 256 rows otherwise
+static inline bool n4_kernel(int nr) { return nr <= 2; }
+
+static SkinnyPlan plan_for(Op ta, int m, int k, int nr)
 {
-    return ta == Op::N ? plan(cdiv(m, 256), k, KCH) : plan(cdiv(m, 4), k, KCT);
+    if (ta == Op::T) return plan(cdiv(m, 4), k, KCT);
+    return plan(cdiv(m, n4_kernel(nr) ? NROWS : 256), k, KCH);
 }
 
 size_t skinny_batch_work_elems(Op ta, int m, int k, int count)
 {
-    const SkinnyPlan p = plan_for(ta, m, k);
+    // the most slabs any column count gives (fewer rows per workgroup never take more)
+    const SkinnyPlan p = plan_for(ta, m, k, MAXR);
     return (size_t)count * (size_t)(p.slabs > 1 ? p.slabs : 1) * MAXR * (size_t)m;
 }
 
@@ -119,6 +129,80 @@ __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int
         }
     }
     if (i >= m) return;
+    if (direct) {
+#pragma unroll
+        for (int c = 0; c < NR; c++) {
+            if (c >= nr) break;
+            double v = alpha * acc[c];
+            if (beta != 0.0) v += beta * io.Y0[(long long)i * io.y0si + (long long)c * io.y0sj];
+            io.Y[(long long)i * io.ysi + (long long)c * io.ysj] = v;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + i] = acc[c];
+    }
+}
+
+// N4 (one or two right-hand sides): 64 rows of op(A) = A per workgroup, each row by four lanes (one per wave), lane q
+// taking the q-th quarter of every KCH-wide k-chunk (column reads coalesced across the
+// wave, KU columns in flight per lane); the four partial sums of a row are added in a
+// fixed order through LDS
+template <int NR>
+__global__ __launch_bounds__(256) void skinny_n4_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
+                                                       const double* __restrict__ A, int lda, SkinnyIO io,
+                                                       double beta, double* work, int direct)
+{
+    __shared__ double Rs[KCH][NR];
+    __shared__ double red[4][NR][NROWS];
+    const int tid = threadIdx.x, row = tid & (NROWS - 1), q = tid >> 6;
+    const int i = blockIdx.x * NROWS + row;
+    const int s = blockIdx.y;
+    {
+        const long long b = blockIdx.z;
+        A += b * io.sA;
+        io.R += b * io.sR;
+        io.Y0 += b * io.sY0;
+        io.Y += b * io.sY;
+        work += b * (long long)gridDim.y * NR * m;
+    }
+    const int kb = s * kslab, ke = min(k, kb + kslab);
+    double acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) acc[c] = 0.0;
+    const int rowmax = min(m, (int)(blockIdx.x * NROWS + NROWS)) - 1;
+    int kend = ke;
+    if (tri == 1) kend = min(kend, rowmax + 1);   // op(A) lower: A[i][kk] == 0 for kk > i
+    int kbeg = kb;
+    if (tri == 2) kbeg = max(kbeg, (int)(blockIdx.x * NROWS));   // op(A) upper
+    constexpr int KQ = KCH / 4;   // columns of a chunk per lane
+    for (int k0 = kbeg; k0 < kend; k0 += KCH) {
+        const int kc = min(KCH, kend - k0);
+        __syncthreads();
+        for (int e = tid; e < KCH * NR; e += 256) {
+            const int kk = e % KCH, c = e / KCH;
+            Rs[kk][c] = (kk < kc && c < nr) ? io.R[(long long)(k0 + kk) * io.rsk + (long long)c * io.rsj] : 0.0;
+        }
+        __syncthreads();
+        if (i < m) {
+            const int q0 = q * KQ;
+            const double* Ap = A + (long long)(k0 + q0) * lda + i;
+            for (int kk = 0; kk < KQ; kk += KU) {
+                double a[KU];
+#pragma unroll
+                for (int u = 0; u < KU; u++) a[u] = q0 + kk + u < kc ? Ap[(long long)(kk + u) * lda] : 0.0;
+#pragma unroll
+                for (int u = 0; u < KU; u++)
+#pragma unroll
+                    for (int c = 0; c < NR; c++) acc[c] = fma(a[u], Rs[q0 + kk + u][c], acc[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NR; c++) red[q][c][row] = acc[c];
+    __syncthreads();
+    if (q != 0 || i >= m) return;
+#pragma unroll
+    for (int c = 0; c < NR; c++) acc[c] = ((red[0][c][row] + red[1][c][row]) + red[2][c][row]) + red[3][c][row];
     if (direct) {
 #pragma unroll
         for (int c = 0; c < NR; c++) {
@@ -221,13 +305,19 @@ template <int NR>
 static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                          const SkinnyIO& io, double beta, double* work, size_t work_elems, int count, hipStream_t s)
 {
-    const SkinnyPlan p = plan_for(ta, m, k);
+    const SkinnyPlan p = plan_for(ta, m, k, nr);
     const int direct = p.slabs == 1;
     if (!direct && (size_t)count * p.slabs * NR * m > work_elems) return (int)hipErrorInvalidValue;
     if (ta == Op::N) {
-        dim3 grid(cdiv(m, 256), p.slabs, count);
-        hipLaunchKernelGGL((skinny_n_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
-                           beta, work, direct);
+        if (n4_kernel(nr)) {
+            dim3 grid(cdiv(m, NROWS), p.slabs, count);
+            hipLaunchKernelGGL((skinny_n4_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda,
+                               io, beta, work, direct);
+        } else {
+            dim3 grid(cdiv(m, 256), p.slabs, count);
+            hipLaunchKernelGGL((skinny_n_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
+                               beta, work, direct);
+        }
     } else {
         dim3 grid(cdiv(m, 4), p.slabs, count);
         hipLaunchKernelGGL((skinny_t_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
@@ -275,7 +365,7 @@ int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double*
 size_t skinny_call_work(Op ta, int m, int k, int nr)
 {
     if (m <= 0 || k <= 0 || nr <= 0) return 0;
-    const SkinnyPlan p = plan_for(ta, m, k);
+    const SkinnyPlan p = plan_for(ta, m, k, nr);
     if (p.slabs == 1) return 0;
     const int NR = nr == 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : nr <= 8 ? 8 : 16;   // launch_skinny's buckets
     return (size_t)p.slabs * NR * (size_t)m;
