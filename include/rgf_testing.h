@@ -46,8 +46,10 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * inversion launches X_i^T X_i and W^T Z as one launch of two products (1) or one after the
  * other (0); key 12 = with key 11 on, the FP64 sweep also computes W = F X of the next block
  * in the launch of this block's Z = S W (1) or in a launch of its own (0); key 13 = FP64 launches of
- * fewer 64 x 64 workgroups than the value take 32 x 32 tiles (0: never; any value >= 0).
- * Returns the previous value,
+ * fewer 64 x 64 workgroups than the value take 32 x 32 tiles (0: never; any value >= 0); key 14 =
+ * pre-split split-BF16 products without triangle flags and with at least this many 256 x 128
+ * workgroups take the 256 x 128 kernel (0: every product, flags included; any value >= 0,
+ * 2^30 in effect never).  Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */
 int rgf_debug_set_tuning(int key, int value);

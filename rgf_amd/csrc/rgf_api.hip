@@ -1727,6 +1727,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         g_small_tile_wgs = value;
         return old;
     }
+    if (key == 14 && value >= 0) {
+        const int old = g_wide_min_wgs;
+        g_wide_min_wgs = value;
+        return old;
+    }
     if (key == 9 && (value == 0 || value == 1)) {
         const int old = g_rr_bands;
         g_rr_bands = value;

@@ -267,17 +267,32 @@ int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s)
 
 // ======================================================================== pre-split operands
 // The kernel above converts every operand element once per output tile that reads it.
-// Here each operand is split once into bf16 hi/lo arrays in "k-contiguous" layout
-// (op(A) as [row][k], op(B)^T as [col][k], row pitch ldk = roundup(K, 64), zero
-// padded), and the product kernel streams them with 16-byte loads straight into LDS.
+// Here each operand is split once into bf16 hi/lo copies of op(A) as [row][k] and
+// op(B)^T as [col][k] (k padded with zeros to ldk = roundup(K, 64)), stored in 16 x 32
+// blocks of one kilobyte (split_index), and the product kernels stream them into LDS
+// one contiguous block per wave-wide 16-byte load.
 namespace rgf {
 
 constexpr int SPLIT_PITCH = 64;
 
+// Layout of a split copy (one plane: hi, or lo): rows padded to a multiple of 128, k to the
+// pitch ldk (a multiple of 64), stored as 16-row x 32-k blocks of 1 KB -- block (g, kc) of
+// rows 16g.. and k 32kc.. at element ((g * ldk / 32) + kc) * 512 -- and inside a block as 64
+// 16-byte chunks: chunk 4 (r % 16) + ((k / 8) % 4 XOR (r / 4) % 4) holds k .. k + 7 (k a
+// multiple of 8) of row r.  One wave's 16-byte loads of a block are then one contiguous
+// kilobyte, and a block copied chunk for chunk into LDS is a 16-row x 32-k tile whose rows
+// (64 bytes) have their chunks XOR-swizzled by (row / 4) % 4: the sixteen rows of one chunk
+// column fall in sixteen distinct 16-byte bank groups.
+__host__ __device__ __forceinline__ long long split_index(int r, int k, int nkb)
+{
+    return ((long long)(r >> 4) * nkb + (k >> 5)) * 512 +
+           ((((r & 15) << 2) | (((k >> 3) & 3) ^ ((r >> 2) & 3))) << 3) + (k & 7);
+}
+
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-// dst[e][r*ldk + k] = split(op(X)[r][k]) for r < R, k < ldk (0 beyond K); hi at dst,
-// lo at dst + lo_off.  op(X)[r][k] = X[r + k*ld] (kcontig = false) or X[k + r*ld].
+// dst[e] plane at split_index(r, k) = split(op(X)[r][k]) for r < R, k < ldk (0 beyond
+// K); hi at dst, lo at dst + lo_off.  op(X)[r][k] = X[r + k*ld] (kcontig = false) or X[k + r*ld].
 // grid: (ldk/64, ceil(R/32), entries).  A 32 x 64 tile is read coalesced along its
 // contiguous edge into LDS (all eight loads of a lane in flight: out-of-range lanes read
 // element 0 and keep zero); each lane then converts 4 consecutive k of a row and stores
@@ -318,7 +333,7 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
                 vh[j] = h;
                 vl[j] = l;
             }
-            const long long o = (long long)gr * ldk + k0 + kq;
+            const long long o = split_index(gr, k0 + kq, ldk >> 5);   // 4 consecutive k of one chunk
             *(bf16x4*)(dst + o) = vh;
             *(bf16x4*)(dst + lo_off + o) = vl;
         }
@@ -330,9 +345,9 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
 // (chunk ^ (row / (16 / CPR)) mod CPR): any 16 lanes that store 16 chunks of 16 / CPR
 // consecutive rows, or read one chunk of 16 consecutive rows, touch 16 distinct
 // 16-byte bank groups.  Without SW rows are padded by 16 bytes.
-// Cs (may be null): the epilogue also writes the split of the stored C, op(C)^T layout --
-// row j of the copy is column j of C, k-contiguous over C's rows, pitch ldkc, zero for rows
-// M .. ldkc-1 -- at Cs + entry * sEC (lo at + loC): what split_operand would write for C used
+// Cs (may be null): the epilogue also writes the split of the stored C as op(C)^T -- row j
+// of the copy is column j of C, its k C's rows, pitch ldkc, zero for rows M .. ldkc-1 -- at
+// Cs + entry * sEC (lo at + loC), in the blocked layout: what split_operand would write for C used
 // as op(B) (N) or op(A) (T) of a later product, from the same values, so it replaces that pass.
 template <int BT, int KB = 32, bool SW = false>
 __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
@@ -402,27 +417,37 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 #pragma unroll
         for (int b = 0; b < NF; b++) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-    // chunk c (0..BT*CPR-1) of a [BT][KB] tile: row c/CPR, k offset 8*(c%CPR)
+    // Chunk c (0..BT*CPR-1) of a [BT][KB] stage: 1 KB block c / 64 of the stage (row group
+    // g, k block kh), chunk p = c % 64 of that block -- row 16g + p / 4, logical k-chunk
+    // 4kh + ((p % 4) XOR (p / 16) % 4); 64 lanes read one block, a contiguous kilobyte.
     bf16x8 rah[CH], ral[CH], rbh[CH], rbl[CH];
+    constexpr int KBB = KB / 32;   // 32-k blocks per stage
+    auto chunk_rk = [](int c, int& r, int& j) {
+        const int b = c >> 6, p = c & 63, g = b / KBB, kh = b % KBB;
+        r = 16 * g + (p >> 2);
+        j = 4 * kh + ((p & 3) ^ ((p >> 4) & 3));
+    };
     // bf16 offset in an LDS row of logical 16-byte chunk j of row r
     auto lcol = [](int r, int j) { return 8 * (SW ? (j ^ ((r / RPG) % CPR)) : j); };
     // Loads without masks: the split copies hold roundup(rows, 128) rows of a 64-aligned
     // pitch, so every row and k of a tile is inside its entry.  Rows beyond M (N) hold
     // unwritten values that reach only C entries that are never stored; k beyond K is
-    // zero.  Lane offsets from the tile origin (bytes) are loop-invariant 32-bit values.
+    // zero.  Lane offsets from the entry (bytes) are loop-invariant 32-bit values; a stage
+    // at k0 adds k0 / 32 blocks.
+    const int nkba = ldka >> 5, nkbb = ldkb >> 5;
     unsigned lofa[CH], lofb[CH];
 #pragma unroll
     for (int q = 0; q < CH; q++) {
-        const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
-        lofa[q] = 2u * ((unsigned)r * (unsigned)ldka + kk);
-        lofb[q] = 2u * ((unsigned)r * (unsigned)ldkb + kk);
+        const int c = tid + 256 * q, b = c >> 6, g = b / KBB, kh = b % KBB, p = c & 63;
+        lofa[q] = 2u * (((unsigned)((i0 >> 4) + g) * (unsigned)nkba + kh) * 512u + 8u * p);
+        lofb[q] = 2u * (((unsigned)((j0 >> 4) + g) * (unsigned)nkbb + kh) * 512u + 8u * p);
     }
-    const char* const Ah0 = (const char*)(Ahg + (long long)i0 * ldka);
-    const char* const Al0 = (const char*)(Ahg + loA + (long long)i0 * ldka);
-    const char* const Bh0 = (const char*)(Bhg + (long long)j0 * ldkb);
-    const char* const Bl0 = (const char*)(Bhg + loB + (long long)j0 * ldkb);
+    const char* const Ah0 = (const char*)Ahg;
+    const char* const Al0 = (const char*)(Ahg + loA);
+    const char* const Bh0 = (const char*)Bhg;
+    const char* const Bl0 = (const char*)(Bhg + loB);
     auto load = [&](int k0) {
-        const long long kb = 2LL * k0;
+        const long long kb = 2LL * 512 * (k0 >> 5);
 #pragma unroll
         for (int q = 0; q < CH; q++) {
             rah[q] = *(const bf16x8*)(Ah0 + kb + lofa[q]);
@@ -434,8 +459,9 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
     auto store = [&]() {
 #pragma unroll
         for (int q = 0; q < CH; q++) {
-            const int c = tid + 256 * q, r = c / CPR, kk = 8 * (c % CPR);
-            const int col = lcol(r, kk / 8);
+            int r, j;
+            chunk_rk(tid + 256 * q, r, j);
+            const int col = lcol(r, j);
             *(bf16x8*)&Ah[r][col] = rah[q];
             *(bf16x8*)&Al[r][col] = ral[q];
             *(bf16x8*)&Bh[r][col] = rbh[q];
@@ -498,8 +524,215 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
                 if (Ce && gi < ldkc && gj < N) {
                     __bf16 h, l;
                     split_bf16(v, h, l);   // rows beyond M: zero (the k padding of the copy)
-                    Ce[(long long)gj * ldkc + gi] = h;
-                    Ce[loC + (long long)gj * ldkc + gi] = l;
+                    const long long o = split_index(gj, gi, ldkc >> 5);
+                    Ce[o] = h;
+                    Ce[loC + o] = l;
+                }
+            }
+        }
+    }
+}
+
+// Wide pre-split product for launches of many tiles (batches): 256 x 128 tiles, four waves
+// of 128 x 64 (each op(A) fragment read from LDS feeds four MFMA columns, each op(B)
+// fragment eight rows: 3/4 of the LDS bytes per MFMA of the 128 x 128 kernel), one
+// workgroup per CU.  K advances in stages of 32 through three LDS buffers (48 KB each)
+// filled by buffer loads straight into LDS: every wave-wide 16-byte load copies one
+// kilobyte block of a split copy, whose chunk order is the LDS tile's (rows of 64 bytes,
+// chunks XOR-swizzled by (row / 4) % 4).  Stage s + 3 is loaded while stage s is computed;
+// the fragments of stage s + 1 are read from LDS into a second register set during the
+// MFMAs of stage s.  Each output gets the MFMAs of the 128 x 128 kernel in the same order
+// (k-blocks of 32 ascending, hi*lo, lo*hi, hi*hi per block), so both give the same bits.
+// rowsA, rowsB: rows of the split copies (multiples of 128); tile rows beyond them read the
+// last 16-row group (they feed only C entries that are never stored).  Lower-stored square
+// products (tri_tiles) take, per 128-wide column, the tiles from its diagonal down, so each
+// covers two 128 x 128 tiles on or below the diagonal.  An isolated full 2865^3 product,
+// batch of 8: 1.21 ms against 1.50 ms on the 128 x 128 kernel, same bits.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, const __bf16* As, const __bf16* Bs,
+                                                              long long sEA, long long sEB, long long loA, long long loB,
+                                                              int ldka, int ldkb, int rowsA, int rowsB, __bf16* Cs,
+                                                              long long sEC, long long loC, int ldkc)
+{
+    constexpr int BM = 256, BN = 128, KB = 32, NS = 3;
+    constexpr int SA = BM * KB, SB = BN * KB, STAGE = 2 * SA + 2 * SB;   // bf16 per array / stage
+    __shared__ __attribute__((aligned(16))) __bf16 lds[NS * STAGE];
+
+    const int z = blockIdx.y;
+    GemmDesc D;
+    select_desc_m(args, z, D);
+    const int flags = D.flags;
+    int g = blockIdx.x;
+    if (!D.deal_rr) {
+        const int G = gridDim.x, q = G >> 3, rem = G & 7, xcd = g & 7, loc = g >> 3;
+        g = xcd < rem ? xcd * (q + 1) + loc : rem * (q + 1) + (xcd - rem) * q + loc;
+    }
+    int i0, j0;
+    if (D.tri_tiles) {   // lower-stored square C: column tj's tiles start at its diagonal
+        // (rows 128 tj + 256 q), so each covers two 128 x 128 tiles on or below it
+        const int T = D.tiles_n;   // 128-wide tile columns (= 128-row tile rows)
+        int tj = 0, rem = g;
+        while (tj < T && rem >= (T - tj + 1) / 2) rem -= (T - tj++ + 1) / 2;
+        if (tj >= T) return;
+        i0 = BN * tj + BM * rem;
+        j0 = BN * tj;
+    } else {             // bands of 8 tile-rows, column-major inside a band
+        if (g >= D.tiles_m * D.tiles_n) return;
+        const int band = g / (8 * D.tiles_n);
+        const int ti0 = band * 8, gsz = min(D.tiles_m - ti0, 8);
+        const int in = g - band * 8 * D.tiles_n;
+        i0 = (ti0 + in % gsz) * BM;
+        j0 = (in / gsz) * BN;
+        if ((flags & G_CLOWER) && j0 > i0 + BM - 1) return;
+    }
+
+    const int bz = z - D.zbase;
+    const long long b1 = bz % D.count, b2 = bz / D.count;
+    const __bf16* Ahg = As + (long long)(D.zbase + bz) * sEA;
+    const __bf16* Bhg = Bs + (long long)(D.zbase + bz) * sEB;
+    double* C = D.C + b1 * D.sC + b2 * D.sC2;
+    const double* Cin = D.Cin + b1 * D.sCin + b2 * D.sCin2;
+    const int M = D.m, N = D.n, K = D.k;
+
+    int klo = 0, khi = K;
+    if (flags & G_ALOWER) khi = min(khi, i0 + BM);
+    if (flags & G_AUPPER) klo = max(klo, i0);
+    if (flags & G_BLOWER) klo = max(klo, j0);
+    if (flags & G_BUPPER) khi = min(khi, j0 + BN);
+    klo = (klo / KB) * KB;
+    const int nk = khi > klo ? (khi - klo + KB - 1) / KB : 0;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
+
+    // load q of wave w copies 16-row group 4q + w of the tile (k-block of the stage in soffset)
+    int offa[4], offb[2];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        offa[q] = (min((i0 >> 4) + 4 * q + wave, (rowsA >> 4) - 1) * (ldka >> 5)) * 1024 + 16 * lane;
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+        offb[q] = (min((j0 >> 4) + 4 * q + wave, (rowsB >> 4) - 1) * (ldkb >> 5)) * 1024 + 16 * lane;
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc((void*)Ahg, (short)0, (int)(2 * sEA), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc((void*)Bhg, (short)0, (int)(2 * sEB), 0x00020000);
+    const int loAb = (int)(2 * loA), loBb = (int)(2 * loB);
+    auto issue = [&](int s, int buf) {
+        const int kb = ((klo >> 5) + s) * 1024;
+        __bf16* st = lds + buf * STAGE;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void_t*)(st + (4 * q + wave) * 512), 16, offa[q], kb, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void_t*)(st + SA + (4 * q + wave) * 512), 16, offa[q],
+                                                     kb + loAb, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void_t*)(st + 2 * SA + (4 * q + wave) * 512), 16, offb[q],
+                                                     kb, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void_t*)(st + 2 * SA + SB + (4 * q + wave) * 512), 16,
+                                                     offb[q], kb + loBb, 0, 0);
+        }
+    };
+
+    v4f acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    struct Frag {
+        bf16x8 ah[8], al[8], bh[4], bl[4];
+    };
+    Frag F0, F1;
+    auto read_frags = [&](Frag& F, int buf) {
+        const __bf16* sAh = lds + buf * STAGE;
+        const __bf16* sAl = sAh + SA;
+        const __bf16* sBh = sAh + 2 * SA;
+        const __bf16* sBl = sBh + SB;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int c = wn * 64 + b * 16 + l16, o = c * KB + 8 * (lq ^ ((c >> 2) & 3));
+            F.bh[b] = *(const bf16x8*)(sBh + o);
+            F.bl[b] = *(const bf16x8*)(sBl + o);
+        }
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const int r = wm * 128 + a * 16 + l16, o = r * KB + 8 * (lq ^ ((r >> 2) & 3));
+            F.ah[a] = *(const bf16x8*)(sAh + o);
+            F.al[a] = *(const bf16x8*)(sAl + o);
+        }
+    };
+    auto mfmas = [&](const Frag& F) {
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                v4f t = acc[a][b];
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.bl[b], F.ah[a], t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.bh[b], F.al[a], t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.bh[b], F.ah[a], t, 0, 0, 0);
+                acc[a][b] = t;
+            }
+    };
+    // stage s: its fragments are in Fc; those of stage s + 1 are read into Fn meanwhile
+    auto step = [&](int s, Frag& Fc, Frag& Fn) {
+        // this wave's reads of Fc and its loads of stage s + 1 complete (the loads of
+        // stage s + 2, if any, stay in flight: 12 per stage); then every wave's
+        const int later = min(NS - 2, nk - 2 - s);                 // stages loading after s + 1
+        if (later >= 1) __builtin_amdgcn_s_waitcnt(0x0070 | 12);   // vmcnt(12) lgkmcnt(0)
+        else __builtin_amdgcn_s_waitcnt(0x0070);                    // vmcnt(0) lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        if (s + NS < nk) issue(s + NS, s % NS);   // the buffer stage s was read from
+        read_frags(Fn, (s + 1) % NS);             // unconditional: no branch between reads and MFMAs
+        mfmas(Fc);
+        // one LDS read after every four MFMAs
+#pragma unroll
+        for (int q = 0; q < 24; q++) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (s < nk) issue(s, s);
+    if (nk > 0) {
+        if (nk >= 3) __builtin_amdgcn_s_waitcnt(0x0F70 | (24 & 15) | ((24 >> 4) << 14));   // vmcnt(24)
+        else if (nk == 2) __builtin_amdgcn_s_waitcnt(0x0F70 | 12);
+        else __builtin_amdgcn_s_waitcnt(0x0F70);
+        __builtin_amdgcn_s_barrier();
+        read_frags(F0, 0);
+    }
+    int s = 0;
+    for (; s + 1 < nk; s += 2) {
+        step(s, F0, F1);
+        step(s + 1, F1, F0);
+    }
+    if (s < nk) step(s, F0, F1);
+
+    const double alpha = D.alpha, beta = D.beta;
+    const int ldc = D.ldc, ldcin = D.ldcin;
+    __bf16* const Ce = Cs ? Cs + (long long)(D.zbase + bz) * sEC : nullptr;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const int gi = i0 + wm * 128 + a * 16 + l16;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gj = j0 + wn * 64 + b * 16 + 4 * lq + r;
+                double v = 0.0;
+                if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
+                    v = alpha * (double)acc[a][b][r];
+                    if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
+                    C[(long long)gj * ldc + gi] = v;
+                }
+                if (Ce && gi < ldkc && gj < N) {
+                    __bf16 h, l;
+                    split_bf16(v, h, l);   // rows beyond M: zero (the k padding of the copy)
+                    const long long o = split_index(gj, gi, ldkc >> 5);
+                    Ce[o] = h;
+                    Ce[loC + o] = l;
                 }
             }
         }
@@ -515,6 +748,8 @@ int g_presplit_kb = 64;
 // 0.293 -> 0.287 s, batch of 8 0.1235 -> 0.1219 s per matrix), same bits; padded rows
 // stay selectable for measurements
 int g_presplit_swizzle = 1;
+// launches of at least this many 256 x 128 workgroups take the wide kernel
+int g_wide_min_wgs = 1536;
 
 size_t split_entry_elems(int rows, int k)
 {
@@ -548,10 +783,40 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     if (d.m <= 0 || d.n <= 0 || d.count <= 0) return 0;
     const int entries = d.count * d.count2;
     const int ldk = (d.k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
-    const long long mp = (d.m + 127) / 128 * 128, np = (d.n + 127) / 128 * 128;
     long long work128 = (long long)((d.m + 127) / 128) * ((d.n + 127) / 128) * entries;
     const int BT = work128 >= 768 ? 128 : 64;
     GemmArgs args{};
+    const long long mp = (d.m + 127) / 128 * 128, np = (d.n + 127) / 128 * 128;
+    // the 256 x 128 kernel: launches of many tiles (one workgroup per CU), buffer ranges below 2 GB
+    const long long tw_m = (d.m + 255) / 256, tw_n = (d.n + 127) / 128;
+    // lower-stored square C: per 128-wide column, ceil(rows on or below its diagonal / 256) tiles
+    const bool wtri = (d.flags & G_CLOWER) && d.m == d.n;
+    long long wtiles = tw_m * tw_n;
+    if (wtri) {
+        wtiles = 0;
+        for (long long tj = 0; tj < tw_n; tj++) wtiles += (tw_n - tj + 1) / 2;
+    }
+    // By default only full products take it: in the c2 sweep (batch of 8) the products with
+    // triangular operands (unequal tiles) and the lower-stored ones ran 2-18 % slower on it
+    // than on twice as many 128 x 128 workgroups, the full product Z = S W 4 % faster.
+    const bool full = d.flags == 0;
+    if ((full || g_wide_min_wgs == 0) && wtiles * entries >= g_wide_min_wgs &&
+        4 * std::max(mp, np) * ldk < (1LL << 31)) {
+        d.tiles_m = (int)tw_m;
+        d.tiles_n = (int)tw_n;
+        set_tile_order(d, wtri);   // round-robin dealing as the square tiles' (lower-stored launches)
+        d.zbase = 0;
+        args.d[0] = d;
+        args.nd = 1;
+        const int ldkc = (d.m + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+        void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
+        hipLaunchKernelGGL(gemm_bf16x3w_kernel, dim3((unsigned)wtiles, entries), dim3(256), 0, s, args, As, Bs,
+                           2 * mp * ldk, 2 * np * ldk, mp * ldk, np * ldk, ldk, ldk, (int)mp, (int)np, Cs,
+                           2 * np * ldkc, np * ldkc, ldkc);
+        hipError_t e = hipGetLastError();
+        timing_end(tok, s);
+        return (int)e;
+    }
     d.tiles_m = (d.m + BT - 1) / BT;
     d.tiles_n = (d.n + BT - 1) / BT;
     set_tile_order(d, (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n);

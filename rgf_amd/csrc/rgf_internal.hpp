@@ -96,7 +96,7 @@ inline void set_tile_order(GemmDesc& d, bool tri)
 int gemm_bf16x3(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s);
 
 // Split-BF16 product with the operands split once, before the product, into bf16
-// hi/lo copies in `scratch` (k-contiguous, pitch roundup(k, 64)).  One descriptor
+// hi/lo copies in `scratch` (k padded to roundup(k, 64), 16 x 32 blocks).  One descriptor
 // (count x count2 entries); scratch must hold split_scratch_elems(m, n, k, entries)
 // bf16 values.  Same result as gemm_bf16x3 to FP32 accumulation order.
 size_t split_scratch_elems(int m, int n, int k, int entries);
@@ -116,6 +116,7 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, cons
                          __bf16* Cs = nullptr);
 extern int g_presplit_kb;   // k per LDS stage of gemm_bf16x3_presplit (32 or 64)
 extern int g_presplit_swizzle;   // 1: its LDS rows unpadded with XOR-swizzled chunks
+extern int g_wide_min_wgs;       // its 256 x 128 kernel for launches of at least this many workgroups
 
 // Kernel-timing hooks shared by the GEMM launchers (op class: 0..3 FP64 NN/NT/TN/TT,
 // 4..7 the same transposes on split-BF16).

@@ -98,3 +98,6 @@ def test_tuning_switches_validate_without_device():
     cur = lib.rgf_debug_set_tuning(13, 0)   # threshold of the 32 x 32 FP64 tiles: any value >= 0
     assert cur >= 0 and lib.rgf_debug_set_tuning(13, -1) == -1
     assert lib.rgf_debug_set_tuning(13, cur) == 0
+    cur = lib.rgf_debug_set_tuning(14, 0)   # threshold of the 256 x 128 split-BF16 tiles: any value >= 0
+    assert cur >= 0 and lib.rgf_debug_set_tuning(14, -1) == -1
+    assert lib.rgf_debug_set_tuning(14, cur) == 0

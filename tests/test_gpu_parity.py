@@ -583,6 +583,76 @@ def _presplit_vs_staged(ta, tb, flags, m, n, k):
     assert np.array_equal(out[0], out[1])
 
 
+def _op_zero_triangles(X, ld, rows, k, trans, lower):
+    """Zero op(X)[i][kk] for kk > i (lower) or kk < i (upper); op(X) = X (rows x k) or X^T."""
+    for i in range(rows):
+        for kk in range(k):
+            if (kk > i) if lower else (kk < i):
+                X[(kk + i * ld) if trans else (i + kk * ld)] = 0.0
+
+
+@pytest.mark.parametrize("ta,tb,flags", [(0, 1, 1), (0, 0, 0), (0, 0, 8), (0, 1, 16), (1, 0, 1), (1, 0, 1 | 4 | 8),
+                                         (0, 0, 2), (1, 1, 0)])
+@pytest.mark.parametrize("m,n,k", [(333, 333, 333), (70, 70, 70), (300, 520, 260), (600, 200, 600)])
+def test_gemm_bf16x3_wide_matches_staged(gpu, ta, tb, flags, m, n, k):
+    """The 256 x 128 pre-split kernel (buffer loads into LDS, forced here for every launch
+    size) sums every output over the same k-blocks in the same MFMA order as the staged
+    kernel, so both give the same bits; the operands hold the zeros their triangle flags
+    promise (the two kernels skip different k-blocks of them)."""
+    say(f"gemm bf16x3 wide vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k}")
+    lib = _lib.load()
+    rng = np.random.default_rng(flags + m + 7 * n)
+    lda, ldb, ldc = (k if ta else m), (n if tb else k), m + 1
+    A = rng.standard_normal(lda * (m if ta else k))
+    B = rng.standard_normal(ldb * (k if tb else n))
+    C = rng.standard_normal(ldc * n)
+    # op(A) is m x k: ALOWER zeroes kk > i, AUPPER kk < i; op(B)^T is n x k: BLOWER
+    # (op(B)[kk][j] == 0 for kk < j) zeroes kk < j of row j, BUPPER kk > j
+    if flags & 2:
+        _op_zero_triangles(A, lda, m, k, ta, True)
+    if flags & 4:
+        _op_zero_triangles(A, lda, m, k, ta, False)
+    if flags & 8:
+        _op_zero_triangles(B, ldb, n, k, not tb, False)
+    if flags & 16:
+        _op_zero_triangles(B, ldb, n, k, not tb, True)
+    out = []
+    for kind, wide in ((256, None), (512, 0), (512, 1 << 30)):
+        old = lib.rgf_debug_set_tuning(14, wide) if wide is not None else None
+        try:
+            Cf = C.copy()
+            rc = lib.rgf_debug_gemm(ta, tb, m, n, k, 1.0, dptr(A.copy()), lda, dptr(B.copy()), ldb, 1.0, dptr(Cf), ldc,
+                                    kind | flags)
+        finally:
+            if old is not None:
+                lib.rgf_debug_set_tuning(14, old)
+        assert rc == 0
+        out.append(Cf)
+    assert np.array_equal(out[0], out[2])   # 128 / 64 tiles on the blocked split copies
+    assert np.array_equal(out[0], out[1])   # 256 x 128 tiles
+
+
+@pytest.mark.parametrize("count", [1, 3])
+def test_mixed_sweep_wide_kernel_same_bits(gpu, count):
+    """Whole mixed sweeps (Schur, trsm, Takahashi products with their split-copy epilogues,
+    lower-stored and triangular launches) with the 256 x 128 product kernel forced for every
+    pre-split launch give the bits of the sweeps without it."""
+    say(f"mixed sweep wide kernel forced vs off, count={count}")
+    lib = _lib.load()
+    opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3)
+    res = []
+    for wide in (0, 1 << 30):
+        old = lib.rgf_debug_set_tuning(14, wide)
+        try:
+            with rgf_amd.RGF.generated(300, 4, 5, seed=12, count=count, theta_mode=1 if count > 1 else 0,
+                                       options=opts) as h:
+                res.append((h.logdet_batch(), h.selinv_diag_batch()))
+        finally:
+            lib.rgf_debug_set_tuning(14, old)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
 @pytest.mark.parametrize("ns,nt,nd", [(200, 6, 6), (65, 4, 5), (300, 3, 0)])
 def test_mixed_precision_vs_fp64(gpu, ns, nt, nd):
     """Mixed path (split-BF16 Schur / solve / Takahashi products, FP64 potrf, logdet and
