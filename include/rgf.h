@@ -51,6 +51,10 @@ typedef struct {
     double refine_tol;  /* stop refining when ||b - A x|| / ||b|| <= refine_tol */
     int graphs;         /* 1: run the factorize and selected-inversion sweeps as HIP graphs,
                            captured on first use and replayed (repeated sweeps of one handle) */
+    int keep_splits;    /* mixed precision: 1 keeps the split-BF16 copy of every coupling panel
+                           [B_i; D_i] that the factorization's products write, for the selected
+                           inversion's W = [B_i; D_i] X_i (~4 bytes per panel entry, about a
+                           quarter more device memory at ns = 2865); 0 splits them again there */
 } rgf_options;
 
 typedef struct {

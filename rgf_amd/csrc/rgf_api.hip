@@ -107,6 +107,8 @@ struct rgf_handle_s {
     __bf16* split = nullptr;    // bf16 hi/lo operand copies of the split-BF16 products
     size_t split_elems = 0;
     __bf16* split_x = nullptr;  // selected inversion: split copy of X_i kept for three products
+    __bf16* lsplit = nullptr;   // split copies of the coupling panels [B_i; D_i] (as op(A) rows):
+                                // one per block with keep_splits, else the last one written
     bool factorized = false;
     bool broken = false;        // in-place factorization failed: A is gone
     bool generated = false;
@@ -226,6 +228,11 @@ size_t split_need(int ns, int nd, int count)
     return split_scratch_elems(ns + nd, ns + nd, ns + nd, count) + (size_t)count * split_entry_elems(ns + nd, ns + nd);
 }
 size_t split_x_need(int ns, int count) { return (size_t)count * split_entry_elems(ns, ns); }
+// split copies of the coupling panels [B_i; D_i] (ns + nd rows, k = ns): every block's, or one
+size_t lsplit_need(int ns, int nt, int nd, int count, int keep_splits)
+{
+    return (size_t)(keep_splits ? nt : 1) * count * split_entry_elems(ns + nd, ns);
+}
 
 // doubles of the skinny products' split-k partials: the solves and the residual (one
 // matrix, m <= max(n, 2ns+nd)) and the factor sweep's arrow-row update (count matrices
@@ -237,7 +244,7 @@ size_t solve_work_elems(int ns, int nt, int nd, int count)
     return std::max(skinny_work_elems(mx, mx, 16), skinny_batch_work_elems(Op::N, ns, ns, count));
 }
 
-size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision)
+size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision, int keep_splits)
 {
     const size_t ldp = (size_t)r8(2LL * ns + nd), ldf = (size_t)r8((long long)ns + nd);
     const size_t T = (size_t)(ns + 63) / 64;
@@ -253,7 +260,8 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     // solve buffers (Y, X, Bv, R, Dc: n x 16 each) and the skinny products' partials
     size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
     size_t bytes = (per * count + once) * sizeof(double);
-    if (precision == RGF_MIXED_BF16X3) bytes += (split_need(ns, nd, count) + split_x_need(ns, count)) * 2;
+    if (precision == RGF_MIXED_BF16X3)
+        bytes += (split_need(ns, nd, count) + split_x_need(ns, count) + lsplit_need(ns, nt, nd, count, keep_splits)) * 2;
     return bytes;
 }
 
@@ -294,6 +302,7 @@ int alloc_core(rgf_handle_s* h)
     if (h->opt.precision == RGF_MIXED_BF16X3) {
         h->split_elems = split_need(ns, nd, B);
         if ((rc = dalloc(h, &h->split, h->split_elems))) return rc;
+        if ((rc = dalloc(h, &h->lsplit, lsplit_need(ns, nt, nd, B, h->opt.keep_splits)))) return rc;
     }
     return RGF_OK;
 }
@@ -426,6 +435,21 @@ int prod_same(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d)
     return rc ? rc : gemm_bf16x3_presplit(ta, tb, d, h->split, h->split, h->stream);
 }
 
+// The coupling panel [B_j; D_j] (rows ns .. 2ns + nd of panel j) is written by a pre-split
+// product on the mixed path when it has at least 128 rows (and ns >= 128); that product
+// also writes its split copy as op(A) rows (k = ns) into panel_copy(h, j): read by the
+// Schur update of block j + 1 (B_j B_j^T) and, with keep_splits, by the selected
+// inversion's W = [B_j; D_j] X_j.
+int panel_rows(const rgf_handle_s* h, int j) { return (j < h->nt - 1 ? h->ns : 0) + h->nd; }
+bool panel_presplit(const rgf_handle_s* h, int j)
+{
+    return h->opt.precision == RGF_MIXED_BF16X3 && panel_rows(h, j) >= 128 && h->ns >= 128;
+}
+__bf16* panel_copy(rgf_handle_s* h, int j)
+{
+    return h->lsplit + (size_t)(h->opt.keep_splits ? j : 0) * h->count * split_entry_elems(h->ns + h->nd, h->ns);
+}
+
 // ------------------------------------------------------------------ factorize
 // FirstStageFactor (RGF.H:233-340) with every block resident, over all matrices
 // of the handle at once (each launch covers the whole batch):
@@ -452,10 +476,7 @@ int factor_sweep(rgf_handle_s* h)
     // Ftmp, except where that product splits its operands first (the mixed path's pre-split
     // products): then F is assembled in its own rows of the panel, which the product
     // overwrites only after the split -- no copy of the coupling block
-    auto f_in_panel = [&](int i) {
-        const int mF = (i < nt - 1 ? ns : 0) + nd;
-        return h->opt.precision == RGF_MIXED_BF16X3 && mF >= 128 && ns >= 128;
-    };
+    auto f_in_panel = [&](int i) { return panel_presplit(h, i); };
     for (int i = 0; i < nt; i++) {
         double* P = l_panel(h, i);
         double* A = a_panel(h, i);
@@ -482,7 +503,13 @@ int factor_sweep(rgf_handle_s* h)
                                  ms, ms, ms, sF);
                 if (rc) return rc;
             }
-            if ((rc = prod_same(h, Op::N, Op::T, &d))) return rc;
+            if (panel_presplit(h, i - 1)) {   // B_{i-1}: the first ns rows of its panel's copy
+                const int r = panel_rows(h, i - 1);
+                __bf16* const c = panel_copy(h, i - 1);
+                if ((rc = gemm_bf16x3_presplit(Op::N, Op::T, &d, c, c, s, nullptr, nullptr, r, r))) return rc;
+            } else if ((rc = prod_same(h, Op::N, Op::T, &d))) {
+                return rc;
+            }
         }
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, false, s)))
@@ -499,7 +526,17 @@ int factor_sweep(rgf_handle_s* h)
             double* dst = P + (has_next ? ns : 2 * ns);
             GemmDesc d = gdesc(src, ldF, P, ldp, dst, ldp, mF, ns, ns, 1.0, 0.0, G_BUPPER);
             batch2(d, B, sF, ms, ms);
-            if ((rc = prod(h, Op::N, Op::T, &d, 1))) return rc;
+            if (fp) {   // F and X_i^T split (F before the product overwrites it); the product
+                        // writes the split copy of [B_i; D_i]
+                __bf16* const As = h->split;
+                __bf16* const Bs = h->split + (size_t)B * split_entry_elems(mF, ns);
+                if ((rc = split_a(h, Op::N, d, As)) ||
+                    (rc = split_operand(d.B, d.ldb, d.n, d.k, false, d.count, d.sB, d.count2, d.sB2, Bs, s)) ||
+                    (rc = gemm_bf16x3_presplit(Op::N, Op::T, &d, As, Bs, s, nullptr, panel_copy(h, i))))
+                    return rc;
+            } else if ((rc = prod(h, Op::N, Op::T, &d, 1))) {
+                return rc;
+            }
         }
     }
     if (nd > 0) {
@@ -858,9 +895,12 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
             batch2(d, B, ms, ms, fs);
             const bool pw = px && presplit_ok(h, d);
-            if (pw) {   // op(B) = X: the split of X^T X's op(A); the product writes the split of W
-                if ((rc = split_a(h, Op::N, d, rT)) ||
-                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, h->split_x, s, rW)))
+            if (pw) {   // op(A) = [B_i; D_i]: the factorization's copy (keep_splits) or split now;
+                        // op(B) = X: the split of X^T X's op(A); the product writes the split of W
+                const bool kept = h->opt.keep_splits && panel_presplit(h, i);
+                const __bf16* const Fs = kept ? panel_copy(h, i) : rT;
+                if ((!kept && (rc = split_a(h, Op::N, d, rT))) ||
+                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, Fs, h->split_x, s, rW)))
                     return rc;
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
@@ -1209,6 +1249,7 @@ void rgf_default_options(rgf_options* o)
     o->refine_iters = 10;
     o->refine_tol = 1e-14;
     o->graphs = 0;
+    o->keep_splits = 1;
 }
 
 void rgf_default_gen_params(rgf_gen_params* p)
@@ -1306,7 +1347,7 @@ size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, const 
     rgf_options o;
     rgf_default_options(&o);
     if (opts) o = *opts;
-    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, o.keep_matrix, o.precision);
+    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, o.keep_matrix, o.precision, o.keep_splits);
 }
 
 int rgf_destroy(rgf_handle_t h)

@@ -340,22 +340,66 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
     }
 }
 
+// Split copies a pre-split product may write of its stored C, from the stored values (what
+// split_operand would write for C, so they replace that pass), entry e at base + e * stride,
+// lo plane at + lo: cs holds op(C)^T (row j = column j of C, k = C's rows, pitch ldkc; zero
+// for rows M .. ldkc-1), for C as op(B) (N) or op(A) (T) of a later product; cr holds op(C)
+// (row i = row i of C, k = C's columns, pitch ldkr; zero for columns N .. ldkr-1), for C
+// as op(A) (N) or op(B) (T).  Rows of a copy beyond C's stay unwritten.
+struct CopyOut {
+    __bf16* cs;
+    long long cs_stride, cs_lo;
+    int ldkc;
+    int ldkr;
+    __bf16* cr;
+    long long cr_stride, cr_lo;
+};
+
+// epilogue of a product tile: C entries (gi, gj .. gj + 3) (gj a multiple of 4) with values
+// v (0 where C is not stored) into the split copies; the op(C) copy takes them as one 8-byte
+// group of each plane (four consecutive k of one chunk)
+__device__ __forceinline__ void copy_out(const CopyOut& co, long long entry, int gi, int gj, int M, int N,
+                                         const double* v)
+{
+    if (co.cs && gi < co.ldkc) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (gj + r < N) {
+                __bf16 h, l;
+                split_bf16(v[r], h, l);   // rows beyond M: zero (the k padding of the copy)
+                const long long o = entry * co.cs_stride + split_index(gj + r, gi, co.ldkc >> 5);
+                co.cs[o] = h;
+                co.cs[co.cs_lo + o] = l;
+            }
+        }
+    }
+    if (co.cr && gi < M && gj < co.ldkr) {   // ldkr is a multiple of 64: gj + 3 < ldkr too
+        bf16x4 h4, l4;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            __bf16 h, l;
+            split_bf16(v[r], h, l);   // columns beyond N: zero (the k padding of the copy)
+            h4[r] = h;
+            l4[r] = l;
+        }
+        const long long o = entry * co.cr_stride + split_index(gi, gj, co.ldkr >> 5);
+        *(bf16x4*)(co.cr + o) = h4;
+        *(bf16x4*)(co.cr + co.cr_lo + o) = l4;
+    }
+}
+
 // KB: k per LDS stage (32 or 64; the split copies are padded to a multiple of 64).
 // SW: LDS rows unpadded with the 16-byte chunk index XOR-swizzled by the row
 // (chunk ^ (row / (16 / CPR)) mod CPR): any 16 lanes that store 16 chunks of 16 / CPR
 // consecutive rows, or read one chunk of 16 consecutive rows, touch 16 distinct
 // 16-byte bank groups.  Without SW rows are padded by 16 bytes.
-// Cs (may be null): the epilogue also writes the split of the stored C as op(C)^T -- row j
-// of the copy is column j of C, its k C's rows, pitch ldkc, zero for rows M .. ldkc-1 -- at
-// Cs + entry * sEC (lo at + loC), in the blocked layout: what split_operand would write for C used
-// as op(B) (N) or op(A) (T) of a later product, from the same values, so it replaces that pass.
+// co: the split copies of the stored C the epilogue writes (CopyOut).
 template <int BT, int KB = 32, bool SW = false>
 __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
                                                                               const __bf16* Bs, long long sEA,
                                                                               long long sEB, long long loA,
                                                                               long long loB, int ldka, int ldkb,
-                                                                              __bf16* Cs, long long sEC,
-                                                                              long long loC, int ldkc)
+                                                                              CopyOut co)
 {
     constexpr int WT = BT / 2;
     constexpr int NF = WT / 16;
@@ -506,29 +550,25 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
-    __bf16* const Ce = Cs ? Cs + (long long)(D.zbase + bz) * sEC : nullptr;
+    const long long entry = D.zbase + bz;
 #pragma unroll
     for (int a = 0; a < NF; a++) {
         const int gi = i0 + wm * WT + a * 16 + l16;
 #pragma unroll
         for (int b = 0; b < NF; b++) {
+            const int gj0 = j0 + wn * WT + b * 16 + 4 * lq;
+            double v[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gj = j0 + wn * WT + b * 16 + 4 * lq + r;
-                double v = 0.0;
+                const int gj = gj0 + r;
+                v[r] = 0.0;
                 if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
-                    v = alpha * (double)acc[a][b][r];
-                    if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
-                    C[(long long)gj * ldc + gi] = v;
-                }
-                if (Ce && gi < ldkc && gj < N) {
-                    __bf16 h, l;
-                    split_bf16(v, h, l);   // rows beyond M: zero (the k padding of the copy)
-                    const long long o = split_index(gj, gi, ldkc >> 5);
-                    Ce[o] = h;
-                    Ce[loC + o] = l;
+                    v[r] = alpha * (double)acc[a][b][r];
+                    if (beta != 0.0) v[r] += beta * Cin[(long long)gj * ldcin + gi];
+                    C[(long long)gj * ldc + gi] = v[r];
                 }
             }
+            copy_out(co, entry, gi, gj0, M, N, v);
         }
     }
 }
@@ -552,8 +592,7 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 
 __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, const __bf16* As, const __bf16* Bs,
                                                               long long sEA, long long sEB, long long loA, long long loB,
-                                                              int ldka, int ldkb, int rowsA, int rowsB, __bf16* Cs,
-                                                              long long sEC, long long loC, int ldkc)
+                                                              int ldka, int ldkb, int rowsA, int rowsB, CopyOut co)
 {
     constexpr int BM = 256, BN = 128, KB = 32, NS = 3;
     constexpr int SA = BM * KB, SB = BN * KB, STAGE = 2 * SA + 2 * SB;   // bf16 per array / stage
@@ -712,29 +751,25 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, con
 
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
-    __bf16* const Ce = Cs ? Cs + (long long)(D.zbase + bz) * sEC : nullptr;
+    const long long entry = D.zbase + bz;
 #pragma unroll
     for (int a = 0; a < 8; a++) {
         const int gi = i0 + wm * 128 + a * 16 + l16;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
+            const int gj0 = j0 + wn * 64 + b * 16 + 4 * lq;
+            double v[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gj = j0 + wn * 64 + b * 16 + 4 * lq + r;
-                double v = 0.0;
+                const int gj = gj0 + r;
+                v[r] = 0.0;
                 if (gi < M && gj < N && (!(flags & G_CLOWER) || gi >= gj)) {
-                    v = alpha * (double)acc[a][b][r];
-                    if (beta != 0.0) v += beta * Cin[(long long)gj * ldcin + gi];
-                    C[(long long)gj * ldc + gi] = v;
-                }
-                if (Ce && gi < ldkc && gj < N) {
-                    __bf16 h, l;
-                    split_bf16(v, h, l);   // rows beyond M: zero (the k padding of the copy)
-                    const long long o = split_index(gj, gi, ldkc >> 5);
-                    Ce[o] = h;
-                    Ce[loC + o] = l;
+                    v[r] = alpha * (double)acc[a][b][r];
+                    if (beta != 0.0) v[r] += beta * Cin[(long long)gj * ldcin + gi];
+                    C[(long long)gj * ldc + gi] = v[r];
                 }
             }
+            copy_out(co, entry, gi, gj0, M, N, v);
         }
     }
 }
@@ -776,7 +811,7 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
 }
 
 int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, const __bf16* Bs, hipStream_t s,
-                         __bf16* Cs)
+                         __bf16* Cs, __bf16* Cr, int rowsA, int rowsB)
 {
     GemmDesc d = *d0;
     if (d.count2 < 1) d.count2 = 1;
@@ -786,7 +821,18 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     long long work128 = (long long)((d.m + 127) / 128) * ((d.n + 127) / 128) * entries;
     const int BT = work128 >= 768 ? 128 : 64;
     GemmArgs args{};
-    const long long mp = (d.m + 127) / 128 * 128, np = (d.n + 127) / 128 * 128;
+    // rows of the operand copies (rowsA / rowsB when they were written for more rows)
+    const long long mp = (std::max(d.m, rowsA) + 127) / 128 * 128, np = (std::max(d.n, rowsB) + 127) / 128 * 128;
+    // the copies of C: op(C)^T (n rows, k = m) and op(C) (m rows, k = n)
+    CopyOut co{};
+    co.cs = Cs;
+    co.ldkc = (d.m + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+    co.cs_stride = 2 * ((d.n + 127) / 128 * 128) * co.ldkc;
+    co.cs_lo = co.cs_stride / 2;
+    co.cr = Cr;
+    co.ldkr = (d.n + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+    co.cr_stride = 2 * ((d.m + 127) / 128 * 128) * co.ldkr;
+    co.cr_lo = co.cr_stride / 2;
     // the 256 x 128 kernel: launches of many tiles (one workgroup per CU), buffer ranges below 2 GB
     const long long tw_m = (d.m + 255) / 256, tw_n = (d.n + 127) / 128;
     // lower-stored square C: per 128-wide column, ceil(rows on or below its diagonal / 256) tiles
@@ -808,11 +854,9 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
         d.zbase = 0;
         args.d[0] = d;
         args.nd = 1;
-        const int ldkc = (d.m + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
         void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
         hipLaunchKernelGGL(gemm_bf16x3w_kernel, dim3((unsigned)wtiles, entries), dim3(256), 0, s, args, As, Bs,
-                           2 * mp * ldk, 2 * np * ldk, mp * ldk, np * ldk, ldk, ldk, (int)mp, (int)np, Cs,
-                           2 * np * ldkc, np * ldkc, ldkc);
+                           2 * mp * ldk, 2 * np * ldk, mp * ldk, np * ldk, ldk, ldk, (int)mp, (int)np, co);
         hipError_t e = hipGetLastError();
         timing_end(tok, s);
         return (int)e;
@@ -827,13 +871,10 @@ int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, con
     dim3 grid(nt, entries);
     void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
     const long long a1 = 2 * mp * ldk, b1 = 2 * np * ldk, a2 = mp * ldk, b2 = np * ldk;
-    // the split of C as op(B) (N) / op(A) (T) of a later product: n rows of k = m
-    const int ldkc = (d.m + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
-    const long long c1 = 2 * np * ldkc, c2 = np * ldkc;
     const bool k64 = g_presplit_kb == 64, sw = g_presplit_swizzle != 0;
 #define RGF_PRESPLIT(bt, kb, swz) \
     hipLaunchKernelGGL((gemm_bf16x3s_kernel<bt, kb, swz>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, \
-                       ldk, Cs, c1, c2, ldkc)
+                       ldk, co)
     if (BT == 128) {
         if (k64) { if (sw) RGF_PRESPLIT(128, 64, true); else RGF_PRESPLIT(128, 64, false); }
         else     { if (sw) RGF_PRESPLIT(128, 32, true); else RGF_PRESPLIT(128, 32, false); }

@@ -111,9 +111,13 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
                   long long s2, __bf16* dst, hipStream_t s);
 // Cs (may be null): the product also writes the bf16 hi/lo split of its stored C into Cs, as
 // split_operand(C, ldc, n, m, kcontig = true, ...) would (one entry per matrix of the
-// descriptor, split_entry_elems(n, m) each): C used as op(B) (N) or op(A) (T) of a later product.
+// descriptor, split_entry_elems(n, m) each): C used as op(B) (N) or op(A) (T) of a later
+// product.  Cr (may be null): the same for C as op(A) (N) or op(B) (T), as
+// split_operand(C, ldc, m, n, kcontig = false, ...) would (split_entry_elems(m, n) each).
+// rowsA / rowsB: rows the operand copies were split for, when more than m / n (the copy of
+// [B; D] read for B alone); 0 otherwise.
 int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s,
-                         __bf16* Cs = nullptr);
+                         __bf16* Cs = nullptr, __bf16* Cr = nullptr, int rowsA = 0, int rowsB = 0);
 extern int g_presplit_kb;   // k per LDS stage of gemm_bf16x3_presplit (32 or 64)
 extern int g_presplit_swizzle;   // 1: its LDS rows unpadded with XOR-swizzled chunks
 extern int g_wide_min_wgs;       // its 256 x 128 kernel for launches of at least this many workgroups
