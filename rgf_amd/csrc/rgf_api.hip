@@ -107,6 +107,7 @@ struct rgf_handle_s {
     __bf16* split = nullptr;    // bf16 hi/lo operand copies of the split-BF16 products
     size_t split_elems = 0;
     __bf16* split_x = nullptr;  // selected inversion: split copy of X_i kept for three products
+    __bf16* split_s = nullptr;  // selected inversion: split copy of the Sigma block S_c (op(A) of Z)
     __bf16* lsplit = nullptr;   // split copies of the coupling panels [B_i; D_i] (as op(A) rows):
                                 // one per block with keep_splits, else the last one written
     bool factorized = false;
@@ -227,7 +228,11 @@ size_t split_need(int ns, int nd, int count)
     // inversion's product writes for W^T Z
     return split_scratch_elems(ns + nd, ns + nd, ns + nd, count) + (size_t)count * split_entry_elems(ns + nd, ns + nd);
 }
-size_t split_x_need(int ns, int count) { return (size_t)count * split_entry_elems(ns, ns); }
+// the selected inversion's copies: X_i (ns x ns) and the Sigma block S_c (ns + nd square)
+size_t split_x_need(int ns, int nd, int count)
+{
+    return (size_t)count * (split_entry_elems(ns, ns) + split_entry_elems(ns + nd, ns + nd));
+}
 // split copies of the coupling panels [B_i; D_i] (ns + nd rows, k = ns): every block's, or one
 size_t lsplit_need(int ns, int nt, int nd, int count, int keep_splits)
 {
@@ -261,7 +266,7 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
     size_t bytes = (per * count + once) * sizeof(double);
     if (precision == RGF_MIXED_BF16X3)
-        bytes += (split_need(ns, nd, count) + split_x_need(ns, count) + lsplit_need(ns, nt, nd, count, keep_splits)) * 2;
+        bytes += (split_need(ns, nd, count) + split_x_need(ns, nd, count) + lsplit_need(ns, nt, nd, count, keep_splits)) * 2;
     return bytes;
 }
 
@@ -346,7 +351,10 @@ int ensure_selinv_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->W, (size_t)h->fstride * B))) return rc;
     if ((rc = dalloc(h, &h->Z, (size_t)h->fstride * B))) return rc;
     if ((rc = dalloc(h, &h->diag, (size_t)h->n * B))) return rc;
-    if (h->opt.precision == RGF_MIXED_BF16X3 && (rc = dalloc(h, &h->split_x, split_x_need(h->ns, B)))) return rc;
+    if (h->opt.precision == RGF_MIXED_BF16X3) {
+        if ((rc = dalloc(h, &h->split_x, split_x_need(h->ns, h->nd, B)))) return rc;
+        h->split_s = h->split_x + (size_t)B * split_entry_elems(h->ns, h->ns);
+    }
     return RGF_OK;
 }
 
@@ -440,6 +448,7 @@ int prod_same(rgf_handle_s* h, Op ta, Op tb, GemmDesc* d)
 // also writes its split copy as op(A) rows (k = ns) into panel_copy(h, j): read by the
 // Schur update of block j + 1 (B_j B_j^T) and, with keep_splits, by the selected
 // inversion's W = [B_j; D_j] X_j.
+int g_trtri_mixed = 1;   // mixed path: L_ii^-1 merges as split-BF16 products (1) or FP64 (0)
 int panel_rows(const rgf_handle_s* h, int j) { return (j < h->nt - 1 ? h->ns : 0) + h->nd; }
 bool panel_presplit(const rgf_handle_s* h, int j)
 {
@@ -515,7 +524,7 @@ int factor_sweep(rgf_handle_s* h)
                             h->xdiag, h->xstride, false, s)))
             return rc;
         if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s,
-                                    h->opt.precision == RGF_MIXED_BF16X3 ? &pf : nullptr)))
+                                    h->opt.precision == RGF_MIXED_BF16X3 && g_trtri_mixed ? &pf : nullptr)))
             return rc;
         const bool has_next = i < nt - 1;
         double* const Fc = fp ? P + ns : h->Ftmp;   // coupling rows of F
@@ -829,6 +838,20 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
         }
         if ((rc = extract_diag(h->Sb[0] + (long long)ns * lds + ns, lds, nd, diag_out + (long long)nt * ns, s, B, ss, n)))
             return rc;
+    }
+    // Mixed path: the split copy of S_c, op(A) of Z = S_c W, is written by the finishing
+    // kernels of the step before (mirror, off-diagonal), its corner and k padding once here
+    // (from Sb[0]: zero but for the corner S_dd, which both S buffers hold throughout)
+    const bool sc_copy = h->opt.precision == RGF_MIXED_BF16X3 && m >= 128 && ns >= 128;
+    SplitOut scp{};
+    if (sc_copy) {
+        scp.p = h->split_s;
+        scp.stride = (long long)split_entry_elems(m, m);
+        scp.lo = scp.stride / 2;
+        scp.ldk = (m + 63) / 64 * 64;
+        if ((rc = split_operand(h->Sb[0], lds, m, m, false, B, ss, 1, 0, h->split_s, s))) return rc;
+    }
+    if (nd > 0) {
         if (so && so->corner &&
             (rc = copy2d(h->Sb[0] + so->mb * ss + (long long)ns * lds + ns, lds, so->corner, nd, nd, nd, 1.0, s)))
             return rc;
@@ -907,8 +930,11 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             }
             d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
             batch2(d, B, ss, fs, fs);
-            if (pw) {   // op(B) = W from the W product; the product writes the split of Z
-                if ((rc = split_a(h, Op::N, d, rT)) || (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, rT, rW, s, rZ)))
+            if (pw) {   // op(A) = S_c: its copy from the step before (or split now); op(B) = W from
+                        // the W product; the product writes the split of Z
+                const bool kept = sc_copy && r0 == 0 && mF == m;
+                if ((!kept && (rc = split_a(h, Op::N, d, rT))) ||
+                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, kept ? h->split_s : rT, rW, s, rZ)))
                     return rc;
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
@@ -941,7 +967,7 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             return rc;
         }
         if ((rc = selinv_finish(Sn, lds, ss, ns, nd, h->Z, lds, fs, mF - nd, exact && mF > 0,
-                                diag_out + (long long)i * ns, n, B, s, xsum, lds, ss)))
+                                diag_out + (long long)i * ns, n, B, s, xsum, lds, ss, sc_copy ? &scp : nullptr)))
             return rc;
         if (so) {
             const double* Sm = Sn + so->mb * ss;
@@ -1766,6 +1792,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
     if (key == 13 && value >= 0) {
         const int old = g_small_tile_wgs;
         g_small_tile_wgs = value;
+        return old;
+    }
+    if (key == 15 && (value == 0 || value == 1)) {
+        const int old = g_trtri_mixed;
+        g_trtri_mixed = value;
         return old;
     }
     if (key == 14 && value >= 0) {

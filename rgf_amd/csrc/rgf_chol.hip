@@ -638,7 +638,8 @@ int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s, in
 // With X: the lower triangle of A first receives A + X (the workgroup owning a lower tile
 // reads, sums and writes it back before mirroring it; its diagonal comes from the sum).
 __global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, long long sA, int n, int T, double* diag,
-                                                     long long sDg, const double* X, int ldx, long long sX)
+                                                     long long sDg, const double* X, int ldx, long long sX,
+                                                     SplitOut cp)
 {
     A += (long long)blockIdx.y * sA;
     if (diag) diag += (long long)blockIdx.y * sDg;
@@ -663,14 +664,27 @@ __global__ __launch_bounds__(256) void mirror_kernel(double* A, int lda, long lo
         if (r < n && c < n && r < c) A[(long long)c * lda + r] = t[tx][q];
         if (diag && r < n && r == c) diag[r] = t[tx][q];
     }
+    if (cp.p) {
+        // S[bi*32 + a][bj*32 + b] = t[a][b] (bi < bj), t[min][max] on a diagonal tile.  Copy
+        // rows bj*32 + q (k = bi*32 + tx) and bi*32 + q (k = bj*32 + tx): lanes along k.
+        const long long e = blockIdx.y;
+        for (int q = ty; q < 32; q += 8) {
+            const int ra = bj * 32 + q, ka = bi * 32 + tx;   // S[ra][ka] = S[ka][ra]
+            if (ra < n && ka < n) split_store(cp, e, ra, ka, bi < bj ? t[tx][q] : t[min(tx, q)][max(tx, q)]);
+            const int rb = bi * 32 + q, kb = bj * 32 + tx;
+            if (bi < bj && rb < n && kb < n) split_store(cp, e, rb, kb, t[q][tx]);
+        }
+    }
 }
 
 int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s,
-                      const double* X, int ldx, long long sX)
+                      const double* X, int ldx, long long sX, const SplitOut* cp)
 {
     if (n <= 0 || count <= 0) return 0;
     const int T = (n + 31) / 32;
-    hipLaunchKernelGGL(mirror_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, diag, sDg, X, ldx, sX);
+    const SplitOut none{};
+    hipLaunchKernelGGL(mirror_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, diag, sDg, X, ldx, sX,
+                       cp ? *cp : none);
     return (int)hipGetLastError();
 }
 
@@ -693,7 +707,7 @@ int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s, in
 
 // S_{d,i} = -Z[zrow0 + d, c] into S[ns + d, c] and S[c, ns + d]
 __global__ void selinv_offdiag_kernel(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz,
-                                      long long sZ, int zrow0)
+                                      long long sZ, int zrow0, SplitOut cp)
 {
     S += (long long)blockIdx.y * sS;
     Z += (long long)blockIdx.y * sZ;
@@ -703,18 +717,23 @@ __global__ void selinv_offdiag_kernel(double* S, int lds, long long sS, int ns, 
         double v = -Z[(long long)c * ldz + zrow0 + d];
         S[(long long)c * lds + ns + d] = v;
         S[(long long)(ns + d) * lds + c] = v;
+        if (cp.p) {
+            split_store(cp, blockIdx.y, ns + d, c, v);
+            split_store(cp, blockIdx.y, c, ns + d, v);
+        }
     }
 }
 
 int selinv_finish(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz, long long sZ,
                   int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s, const double* X,
-                  int ldx, long long sX)
+                  int ldx, long long sX, const SplitOut* cp)
 {
-    int rc = mirror_lower_diag(S, lds, sS, ns, diag_out, sDg, count, s, X, ldx, sX);
+    int rc = mirror_lower_diag(S, lds, sS, ns, diag_out, sDg, count, s, X, ldx, sX, cp);
     if (rc) return rc;
     if (exact && nd > 0) {
+        const SplitOut none{};
         hipLaunchKernelGGL(selinv_offdiag_kernel, dim3((ns + 255) / 256, count), dim3(256), 0, s, S, lds, sS, ns,
-                           nd, Z, ldz, sZ, zrow0);
+                           nd, Z, ldz, sZ, zrow0, cp ? *cp : none);
         return (int)hipGetLastError();
     }
     return 0;

@@ -28,12 +28,6 @@ constexpr int MBK = 32;     // k per step
 constexpr int KPAD = 8;     // bf16 padding per LDS row (16 bytes)
 constexpr int LROW = MBK + KPAD;
 
-__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo)
-{
-    const float f = (float)x;
-    hi = (__bf16)f;
-    lo = (__bf16)(f - (float)hi);
-}
 
 __device__ __forceinline__ void select_desc_m(const GemmArgs& args, int z, GemmDesc& D)
 {
@@ -282,13 +276,7 @@ constexpr int SPLIT_PITCH = 64;
 // multiple of 8) of row r.  One wave's 16-byte loads of a block are then one contiguous
 // kilobyte, and a block copied chunk for chunk into LDS is a 16-row x 32-k tile whose rows
 // (64 bytes) have their chunks XOR-swizzled by (row / 4) % 4: the sixteen rows of one chunk
-// column fall in sixteen distinct 16-byte bank groups.
-__host__ __device__ __forceinline__ long long split_index(int r, int k, int nkb)
-{
-    return ((long long)(r >> 4) * nkb + (k >> 5)) * 512 +
-           ((((r & 15) << 2) | (((k >> 3) & 3) ^ ((r >> 2) & 3))) << 3) + (k & 7);
-}
-
+// column fall in sixteen distinct 16-byte bank groups (split_index, rgf_internal.hpp).
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // dst[e] plane at split_index(r, k) = split(op(X)[r][k]) for r < R, k < ldk (0 beyond

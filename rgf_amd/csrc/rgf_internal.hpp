@@ -217,19 +217,55 @@ double gen_uniform_host(uint64_t seed, uint32_t mid, uint32_t idx);
 int scatter_csc(const uint64_t* ia, const uint64_t* ja, const double* a, long long nnz, int ns,
                 int nt, int nd, double* P, long long pstride, int ldp, double* Cdd, hipStream_t s);
 int mirror_lower(double* A, int lda, int n, hipStream_t s);
+
+// ---------------------------------------------------------------- split-BF16 copies
+// Element (r, k) of one plane of a split copy whose k pitch is 32 * nkb: 16-row x 32-k blocks
+// of 1 KB (block (r / 16, k / 32) at element ((r / 16) * nkb + k / 32) * 512), inside a block
+// 64 chunks of 8 k, chunk 4 (r % 16) + ((k / 8) % 4 XOR (r / 4) % 4).
+__host__ __device__ __forceinline__ long long split_index(int r, int k, int nkb)
+{
+    return ((long long)(r >> 4) * nkb + (k >> 5)) * 512 +
+           ((((r & 15) << 2) | (((k >> 3) & 3) ^ ((r >> 2) & 3))) << 3) + (k & 7);
+}
+// x -> f = (float)x, hi = bf16(f), lo = bf16(f - hi): x = hi + lo to about 2^-16 relative
+__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo)
+{
+    const float f = (float)x;
+    hi = (__bf16)f;
+    lo = (__bf16)(f - (float)hi);
+}
+// A split copy written by a kernel beside its FP64 result: entry e at p + e * stride, lo
+// plane at + lo, k pitch ldk (p null: none)
+struct SplitOut {
+    __bf16* p;
+    long long stride, lo;
+    int ldk;
+};
+__device__ __forceinline__ void split_store(const SplitOut& o, long long e, int r, int k, double v)
+{
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    const long long i = e * o.stride + split_index(r, k, o.ldk >> 5);
+    o.p[i] = h;
+    o.p[o.lo + i] = l;
+}
+
 // Mirror the lower triangle of A into its upper one (and gather the diagonal); with X the
-// lower triangle first receives A + X.
+// lower triangle first receives A + X; with cp (mixed path) the split copy of the whole
+// symmetric n x n result as op(A) rows (row r, k c) is written too.
 int mirror_lower_diag(double* A, int lda, long long sA, int n, double* diag, long long sDg, int count, hipStream_t s,
-                      const double* X = nullptr, int ldx = 0, long long sX = 0);
+                      const double* X = nullptr, int ldx = 0, long long sX = 0, const SplitOut* cp = nullptr);
 
 // ---------------------------------------------------------------- selinv helpers
 // Finish one Takahashi step: (with X) add the lower triangle of X into that of S, mirror
 // the lower triangle of S (ns x ns, ld lds) into its upper triangle, write diag(S) to
 // diag_out, and (exact mode) store S_{d,i} = -Z[zrow0:zrow0+nd, :] into S[ns:ns+nd, 0:ns]
 // and its transpose.
+// With cp, the split copy of S's rows 0 .. ns + nd - 1 (as op(A) rows) receives what these
+// two steps write (the corner and the k padding of the copy are the caller's).
 int selinv_finish(double* S, int lds, long long sS, int ns, int nd, const double* Z, int ldz, long long sZ,
                   int zrow0, int exact, double* diag_out, long long sDg, int count, hipStream_t s,
-                  const double* X = nullptr, int ldx = 0, long long sX = 0);
+                  const double* X = nullptr, int ldx = 0, long long sX = 0, const SplitOut* cp = nullptr);
 int extract_diag(const double* A, int lda, int n, double* out, hipStream_t s, int count = 1, long long sA = 0,
                  long long sO = 0);
 
