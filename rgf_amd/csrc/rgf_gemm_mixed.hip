@@ -328,40 +328,55 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const double* X, int ld
     }
 }
 
-// Split copies a pre-split product may write of its stored C, from the stored values (what
-// split_operand would write for C, so they replace that pass), entry e at base + e * stride,
-// lo plane at + lo: cs holds op(C)^T (row j = column j of C, k = C's rows, pitch ldkc; zero
-// for rows M .. ldkc-1), for C as op(B) (N) or op(A) (T) of a later product; cr holds op(C)
-// (row i = row i of C, k = C's columns, pitch ldkr; zero for columns N .. ldkr-1), for C
-// as op(A) (N) or op(B) (T).  Rows of a copy beyond C's stay unwritten.
-struct CopyOut {
-    __bf16* cs;
-    long long cs_stride, cs_lo;
-    int ldkc;
-    int ldkr;
-    __bf16* cr;
-    long long cr_stride, cr_lo;
+// Operands and destinations of one descriptor of a pre-split launch: op(A) rows and op(B)^T
+// rows (SplitView); cs, cr (p null: none) receive the stored C from the epilogue as the split
+// copies split_operand would write for C (same values, so they replace that pass): cs as
+// op(C)^T (row j = column j of C, k = C's rows), for C as op(B) (N) or op(A) (T) of a later
+// product; cr as op(C) (row i = row i of C, k = C's columns), for op(A) (N) or op(B) (T).
+// Rows of a copy beyond C's stay unwritten; with pad the k beyond C up to the pitch is zero.
+struct PresplitOperands {
+    SplitView a, b, cs, cr;
 };
-
-// epilogue of a product tile: C entries (gi, gj .. gj + 3) (gj a multiple of 4) with values
-// v (0 where C is not stored) into the split copies; the op(C) copy takes them as one 8-byte
-// group of each plane (four consecutive k of one chunk)
-__device__ __forceinline__ void copy_out(const CopyOut& co, long long entry, int gi, int gj, int M, int N,
-                                         const double* v)
+constexpr int PRESPLIT_MAXD = 2;
+struct PresplitArgs {
+    PresplitOperands op[PRESPLIT_MAXD];
+};
+// descriptor of launch entry z and its operands (uniform selects, no indexed kernel argument)
+__device__ __forceinline__ void select_desc_ops(const GemmArgs& args, const PresplitArgs& pa, int z, GemmDesc& D,
+                                                PresplitOperands& O)
 {
-    if (co.cs && gi < co.ldkc) {
+    D = args.d[0];
+    O = pa.op[0];
+#pragma unroll
+    for (int t = 1; t < PRESPLIT_MAXD; t++)
+        if (t < args.nd && z >= args.d[t].zbase) {
+            D = args.d[t];
+            O = pa.op[t];
+        }
+}
+
+// epilogue of a product tile: C entries (gi, gj .. gj + 3) (gj a multiple of 4) of entry
+// (b1, b2) with values v (0 where C is not stored) into the split copies; the op(C) copy
+// takes them as one 8-byte group of each plane (four consecutive k of one chunk)
+__device__ __forceinline__ void copy_out(const PresplitOperands& O, long long b1, long long b2, int gi, int gj,
+                                         int M, int N, const double* v)
+{
+    const SplitView& cs = O.cs;
+    if (cs.p && gi < (cs.pad ? cs.ldk : M)) {   // k = gi (beyond M: zero, the k padding)
+        __bf16* const e = cs.p + b1 * cs.s1 + b2 * cs.s2;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             if (gj + r < N) {
                 __bf16 h, l;
-                split_bf16(v[r], h, l);   // rows beyond M: zero (the k padding of the copy)
-                const long long o = entry * co.cs_stride + split_index(gj + r, gi, co.ldkc >> 5);
-                co.cs[o] = h;
-                co.cs[co.cs_lo + o] = l;
+                split_bf16(v[r], h, l);
+                const long long o = split_index(gj + r, gi, cs.ldk >> 5);
+                e[o] = h;
+                e[cs.lo + o] = l;
             }
         }
     }
-    if (co.cr && gi < M && gj < co.ldkr) {   // ldkr is a multiple of 64: gj + 3 < ldkr too
+    const SplitView& cr = O.cr;
+    if (cr.p && gi < M && gj < (cr.pad ? cr.ldk : N)) {   // k = gj; the pitch and N are multiples of 4
         bf16x4 h4, l4;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -370,9 +385,10 @@ __device__ __forceinline__ void copy_out(const CopyOut& co, long long entry, int
             h4[r] = h;
             l4[r] = l;
         }
-        const long long o = entry * co.cr_stride + split_index(gi, gj, co.ldkr >> 5);
-        *(bf16x4*)(co.cr + o) = h4;
-        *(bf16x4*)(co.cr + co.cr_lo + o) = l4;
+        __bf16* const e = cr.p + b1 * cr.s1 + b2 * cr.s2;
+        const long long o = split_index(gi, gj, cr.ldk >> 5);
+        *(bf16x4*)(e + o) = h4;
+        *(bf16x4*)(e + cr.lo + o) = l4;
     }
 }
 
@@ -383,11 +399,7 @@ __device__ __forceinline__ void copy_out(const CopyOut& co, long long entry, int
 // 16-byte bank groups.  Without SW rows are padded by 16 bytes.
 // co: the split copies of the stored C the epilogue writes (CopyOut).
 template <int BT, int KB = 32, bool SW = false>
-__global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, const __bf16* As,
-                                                                              const __bf16* Bs, long long sEA,
-                                                                              long long sEB, long long loA,
-                                                                              long long loB, int ldka, int ldkb,
-                                                                              CopyOut co)
+__global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(GemmArgs args, PresplitArgs pa)
 {
     constexpr int WT = BT / 2;
     constexpr int NF = WT / 16;
@@ -402,7 +414,10 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 
     const int z = blockIdx.y;
     GemmDesc D;
-    select_desc_m(args, z, D);
+    PresplitOperands O;
+    select_desc_ops(args, pa, z, D, O);
+    const long long loA = O.a.lo, loB = O.b.lo;
+    const int ldka = O.a.ldk, ldkb = O.b.ldk;
     const int flags = D.flags;
     int g = blockIdx.x;
     if (!D.deal_rr) {
@@ -425,8 +440,8 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 
     const int bz = z - D.zbase;   // entry index inside this descriptor; split copies are per entry
     const long long b1 = bz % D.count, b2 = bz / D.count;
-    const __bf16* Ahg = As + (long long)(D.zbase + bz) * sEA;
-    const __bf16* Bhg = Bs + (long long)(D.zbase + bz) * sEB;
+    const __bf16* Ahg = O.a.p + b1 * O.a.s1 + b2 * O.a.s2;   // this entry's operand copies
+    const __bf16* Bhg = O.b.p + b1 * O.b.s1 + b2 * O.b.s2;
     double* C = D.C + b1 * D.sC + b2 * D.sC2;
     const double* Cin = D.Cin + b1 * D.sCin + b2 * D.sCin2;
     const int M = D.m, N = D.n, K = D.k;
@@ -538,7 +553,6 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
-    const long long entry = D.zbase + bz;
 #pragma unroll
     for (int a = 0; a < NF; a++) {
         const int gi = i0 + wm * WT + a * 16 + l16;
@@ -556,7 +570,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
                     C[(long long)gj * ldc + gi] = v[r];
                 }
             }
-            copy_out(co, entry, gi, gj0, M, N, v);
+            copy_out(O, b1, b2, gi, gj0, M, N, v);
         }
     }
 }
@@ -578,9 +592,7 @@ __global__ __launch_bounds__(256, (BT == 128 ? 2 : 4)) void gemm_bf16x3s_kernel(
 // batch of 8: 1.21 ms against 1.50 ms on the 128 x 128 kernel, same bits.
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-__global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, const __bf16* As, const __bf16* Bs,
-                                                              long long sEA, long long sEB, long long loA, long long loB,
-                                                              int ldka, int ldkb, int rowsA, int rowsB, CopyOut co)
+__global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, PresplitArgs pa)
 {
     constexpr int BM = 256, BN = 128, KB = 32, NS = 3;
     constexpr int SA = BM * KB, SB = BN * KB, STAGE = 2 * SA + 2 * SB;   // bf16 per array / stage
@@ -588,7 +600,10 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, con
 
     const int z = blockIdx.y;
     GemmDesc D;
-    select_desc_m(args, z, D);
+    PresplitOperands O;
+    select_desc_ops(args, pa, z, D, O);
+    const long long loA = O.a.lo, loB = O.b.lo;
+    const int ldka = O.a.ldk, ldkb = O.b.ldk, rowsA = O.a.rows, rowsB = O.b.rows;
     const int flags = D.flags;
     int g = blockIdx.x;
     if (!D.deal_rr) {
@@ -616,8 +631,8 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, con
 
     const int bz = z - D.zbase;
     const long long b1 = bz % D.count, b2 = bz / D.count;
-    const __bf16* Ahg = As + (long long)(D.zbase + bz) * sEA;
-    const __bf16* Bhg = Bs + (long long)(D.zbase + bz) * sEB;
+    const __bf16* Ahg = O.a.p + b1 * O.a.s1 + b2 * O.a.s2;   // this entry's operand copies
+    const __bf16* Bhg = O.b.p + b1 * O.b.s1 + b2 * O.b.s2;
     double* C = D.C + b1 * D.sC + b2 * D.sC2;
     const double* Cin = D.Cin + b1 * D.sCin + b2 * D.sCin2;
     const int M = D.m, N = D.n, K = D.k;
@@ -641,8 +656,11 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, con
 #pragma unroll
     for (int q = 0; q < 2; q++)
         offb[q] = (min((j0 >> 4) + 4 * q + wave, (rowsB >> 4) - 1) * (ldkb >> 5)) * 1024 + 16 * lane;
-    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc((void*)Ahg, (short)0, (int)(2 * sEA), 0x00020000);
-    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc((void*)Bhg, (short)0, (int)(2 * sEB), 0x00020000);
+    // ranges: both planes of the rows readable from the entry's origin (bytes)
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)Ahg, (short)0, (int)(2 * (loA + (long long)rowsA * ldka)), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)Bhg, (short)0, (int)(2 * (loB + (long long)rowsB * ldkb)), 0x00020000);
     const int loAb = (int)(2 * loA), loBb = (int)(2 * loB);
     auto issue = [&](int s, int buf) {
         const int kb = ((klo >> 5) + s) * 1024;
@@ -739,7 +757,6 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, con
 
     const double alpha = D.alpha, beta = D.beta;
     const int ldc = D.ldc, ldcin = D.ldcin;
-    const long long entry = D.zbase + bz;
 #pragma unroll
     for (int a = 0; a < 8; a++) {
         const int gi = i0 + wm * 128 + a * 16 + l16;
@@ -757,7 +774,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3w_kernel(GemmArgs args, con
                     C[(long long)gj * ldc + gi] = v[r];
                 }
             }
-            copy_out(co, entry, gi, gj0, M, N, v);
+            copy_out(O, b1, b2, gi, gj0, M, N, v);
         }
     }
 }
@@ -798,82 +815,117 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
     return (int)hipGetLastError();
 }
 
-int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d0, const __bf16* As, const __bf16* Bs, hipStream_t s,
-                         __bf16* Cs, __bf16* Cr, int rowsA, int rowsB)
+SplitView split_view(__bf16* p, int rows, int k, int count)
 {
-    GemmDesc d = *d0;
-    if (d.count2 < 1) d.count2 = 1;
-    if (d.m <= 0 || d.n <= 0 || d.count <= 0) return 0;
-    const int entries = d.count * d.count2;
-    const int ldk = (d.k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
-    long long work128 = (long long)((d.m + 127) / 128) * ((d.n + 127) / 128) * entries;
-    const int BT = work128 >= 768 ? 128 : 64;
+    SplitView v{};
+    v.p = p;
+    v.ldk = (k + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
+    v.rows = (rows + 127) / 128 * 128;
+    v.lo = (long long)v.rows * v.ldk;
+    v.s1 = 2 * v.lo;
+    v.s2 = v.s1 * count;
+    v.pad = 1;
+    return v;
+}
+
+int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, hipStream_t s)
+{
+    if (ncalls <= 0) return 0;
+    if (ncalls > PRESPLIT_MAXD) return (int)hipErrorInvalidValue;
     GemmArgs args{};
-    // rows of the operand copies (rowsA / rowsB when they were written for more rows)
-    const long long mp = (std::max(d.m, rowsA) + 127) / 128 * 128, np = (std::max(d.n, rowsB) + 127) / 128 * 128;
-    // the copies of C: op(C)^T (n rows, k = m) and op(C) (m rows, k = n)
-    CopyOut co{};
-    co.cs = Cs;
-    co.ldkc = (d.m + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
-    co.cs_stride = 2 * ((d.n + 127) / 128 * 128) * co.ldkc;
-    co.cs_lo = co.cs_stride / 2;
-    co.cr = Cr;
-    co.ldkr = (d.n + SPLIT_PITCH - 1) / SPLIT_PITCH * SPLIT_PITCH;
-    co.cr_stride = 2 * ((d.m + 127) / 128 * 128) * co.ldkr;
-    co.cr_lo = co.cr_stride / 2;
-    // the 256 x 128 kernel: launches of many tiles (one workgroup per CU), buffer ranges below 2 GB
-    const long long tw_m = (d.m + 255) / 256, tw_n = (d.n + 127) / 128;
-    // lower-stored square C: per 128-wide column, ceil(rows on or below its diagonal / 256) tiles
-    const bool wtri = (d.flags & G_CLOWER) && d.m == d.n;
-    long long wtiles = tw_m * tw_n;
-    if (wtri) {
-        wtiles = 0;
-        for (long long tj = 0; tj < tw_n; tj++) wtiles += (tw_n - tj + 1) / 2;
+    PresplitArgs pa{};
+    long long work128 = 0, wide_wgs = 0;
+    bool all_full = true;
+    int nd = 0;
+    for (int q = 0; q < ncalls; q++) {
+        GemmDesc d = calls[q].d;
+        if (d.count2 < 1) d.count2 = 1;
+        if (d.m <= 0 || d.n <= 0 || d.count <= 0) continue;
+        const long long entries = (long long)d.count * d.count2;
+        PresplitOperands& O = pa.op[nd];
+        O.a = calls[q].a;
+        O.b = calls[q].b;
+        O.cs = calls[q].cs;
+        O.cr = calls[q].cr;
+        work128 += (long long)((d.m + 127) / 128) * ((d.n + 127) / 128) * entries;
+        // the 256 x 128 kernel: lower-stored square C takes, per 128-wide column, ceil(rows on
+        // or below its diagonal / 256) tiles
+        const long long tw_m = (d.m + 255) / 256, tw_n = (d.n + 127) / 128;
+        long long wt = tw_m * tw_n;
+        if ((d.flags & G_CLOWER) && d.m == d.n) {
+            wt = 0;
+            for (long long tj = 0; tj < tw_n; tj++) wt += (tw_n - tj + 1) / 2;
+        }
+        wide_wgs += wt * entries;
+        // buffer-load ranges: both planes of an entry's copy below 2 GB from its origin
+        const long long span = 2 * std::max(O.a.lo + (long long)O.a.rows * O.a.ldk, O.b.lo + (long long)O.b.rows * O.b.ldk);
+        all_full = all_full && d.flags == 0 && span < (1LL << 31);
+        args.d[nd++] = d;
     }
-    // By default only full products take it: in the c2 sweep (batch of 8) the products with
-    // triangular operands (unequal tiles) and the lower-stored ones ran 2-18 % slower on it
-    // than on twice as many 128 x 128 workgroups, the full product Z = S W 4 % faster.
-    const bool full = d.flags == 0;
-    if ((full || g_wide_min_wgs == 0) && wtiles * entries >= g_wide_min_wgs &&
-        4 * std::max(mp, np) * ldk < (1LL << 31)) {
-        d.tiles_m = (int)tw_m;
-        d.tiles_n = (int)tw_n;
-        set_tile_order(d, wtri);   // round-robin dealing as the square tiles' (lower-stored launches)
-        d.zbase = 0;
-        args.d[0] = d;
-        args.nd = 1;
-        void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
-        hipLaunchKernelGGL(gemm_bf16x3w_kernel, dim3((unsigned)wtiles, entries), dim3(256), 0, s, args, As, Bs,
-                           2 * mp * ldk, 2 * np * ldk, mp * ldk, np * ldk, ldk, ldk, (int)mp, (int)np, co);
-        hipError_t e = hipGetLastError();
-        timing_end(tok, s);
-        return (int)e;
+    if (nd == 0) return 0;
+    args.nd = nd;
+    // By default only full products take the 256 x 128 kernel: in the c2 sweep (batch of 8) the
+    // products with triangular operands (unequal tiles) and the lower-stored ones ran 2-18 %
+    // slower on it than on twice as many 128 x 128 workgroups, the full product Z = S W 4 % faster.
+    const bool wide = (all_full || g_wide_min_wgs == 0) && wide_wgs >= g_wide_min_wgs;
+    const int BT = work128 >= 768 ? 128 : 64;
+    int z = 0, maxtiles = 0;
+    for (int q = 0; q < nd; q++) {
+        GemmDesc& d = args.d[q];
+        const bool sq_lower = (d.flags & G_CLOWER) && d.m == d.n;
+        int tiles;
+        if (wide) {
+            d.tiles_m = (d.m + 255) / 256;
+            d.tiles_n = (d.n + 127) / 128;
+            set_tile_order(d, sq_lower);   // round-robin dealing as the square tiles' (lower-stored launches)
+            tiles = d.tiles_m * d.tiles_n;
+            if (sq_lower) {
+                tiles = 0;
+                for (int tj = 0; tj < d.tiles_n; tj++) tiles += (d.tiles_n - tj + 1) / 2;
+            }
+        } else {
+            d.tiles_m = (d.m + BT - 1) / BT;
+            d.tiles_n = (d.n + BT - 1) / BT;
+            set_tile_order(d, (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n);
+            tiles = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
+        }
+        d.zbase = z;
+        z += d.count * d.count2;
+        maxtiles = std::max(maxtiles, tiles);
     }
-    d.tiles_m = (d.m + BT - 1) / BT;
-    d.tiles_n = (d.n + BT - 1) / BT;
-    set_tile_order(d, (d.flags & G_CLOWER) && d.tiles_m == d.tiles_n);
-    d.zbase = 0;
-    const int nt = d.tri_tiles ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
-    args.d[0] = d;
-    args.nd = 1;
-    dim3 grid(nt, entries);
-    void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, 1, s);
-    const long long a1 = 2 * mp * ldk, b1 = 2 * np * ldk, a2 = mp * ldk, b2 = np * ldk;
-    const bool k64 = g_presplit_kb == 64, sw = g_presplit_swizzle != 0;
-#define RGF_PRESPLIT(bt, kb, swz) \
-    hipLaunchKernelGGL((gemm_bf16x3s_kernel<bt, kb, swz>), grid, dim3(256), 0, s, args, As, Bs, a1, b1, a2, b2, ldk, \
-                       ldk, co)
-    if (BT == 128) {
-        if (k64) { if (sw) RGF_PRESPLIT(128, 64, true); else RGF_PRESPLIT(128, 64, false); }
-        else     { if (sw) RGF_PRESPLIT(128, 32, true); else RGF_PRESPLIT(128, 32, false); }
+    const dim3 grid(maxtiles, z);
+    void* tok = timing_begin(4 + (ta == Op::T ? 2 : 0) + (tb == Op::T ? 1 : 0), args.d, nd, s);
+    if (wide) {
+        hipLaunchKernelGGL(gemm_bf16x3w_kernel, grid, dim3(256), 0, s, args, pa);
     } else {
-        if (k64) { if (sw) RGF_PRESPLIT(64, 64, true); else RGF_PRESPLIT(64, 64, false); }
-        else     { if (sw) RGF_PRESPLIT(64, 32, true); else RGF_PRESPLIT(64, 32, false); }
-    }
+        const bool k64 = g_presplit_kb == 64, sw = g_presplit_swizzle != 0;
+#define RGF_PRESPLIT(bt, kb, swz) \
+        hipLaunchKernelGGL((gemm_bf16x3s_kernel<bt, kb, swz>), grid, dim3(256), 0, s, args, pa)
+        if (BT == 128) {
+            if (k64) { if (sw) RGF_PRESPLIT(128, 64, true); else RGF_PRESPLIT(128, 64, false); }
+            else     { if (sw) RGF_PRESPLIT(128, 32, true); else RGF_PRESPLIT(128, 32, false); }
+        } else {
+            if (k64) { if (sw) RGF_PRESPLIT(64, 64, true); else RGF_PRESPLIT(64, 64, false); }
+            else     { if (sw) RGF_PRESPLIT(64, 32, true); else RGF_PRESPLIT(64, 32, false); }
+        }
 #undef RGF_PRESPLIT
+    }
     hipError_t e = hipGetLastError();
     timing_end(tok, s);
     return (int)e;
+}
+
+int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s,
+                         __bf16* Cs, __bf16* Cr, int rowsA, int rowsB)
+{
+    const int count = std::max(d->count, 1);
+    PresplitCall c{};
+    c.d = *d;
+    c.a = split_view((__bf16*)As, std::max(d->m, rowsA), d->k, count);
+    c.b = split_view((__bf16*)Bs, std::max(d->n, rowsB), d->k, count);
+    if (Cs) c.cs = split_view(Cs, d->n, d->m, count);
+    if (Cr) c.cr = split_view(Cr, d->m, d->n, count);
+    return gemm_bf16x3_presplit_n(ta, tb, &c, 1, s);
 }
 
 // One product on split-BF16 MFMA with operands split first into `scratch` (bf16

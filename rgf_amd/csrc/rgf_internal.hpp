@@ -118,6 +118,25 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
 // [B; D] read for B alone); 0 otherwise.
 int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s,
                          __bf16* Cs = nullptr, __bf16* Cr = nullptr, int rowsA = 0, int rowsB = 0);
+// A split copy, or a block of one, as an operand or destination of a pre-split product: its
+// origin at p (a multiple of 16 rows and 32 k inside the copy), lo plane at p + lo, k pitch
+// ldk (a multiple of 64), rows readable from p; entry (b1, b2) of the product's descriptor
+// (b1 < count, b2 < count2) at p + b1 * s1 + b2 * s2.  pad (destinations): write zeros in
+// the copy's k beyond C (a whole copy, not a block of a larger one).
+struct SplitView {
+    __bf16* p;
+    long long lo, s1, s2;
+    int ldk, rows, pad;
+};
+// the layout split_operand writes: rows x k per entry, count x count2 entries one after another
+SplitView split_view(__bf16* p, int rows, int k, int count);
+// Up to two products of the same transposes in one launch (the tiles of both in one grid).
+// cs / cr (p null: none): the product's stored C as op(C)^T / op(C) copies (see Cs / Cr).
+struct PresplitCall {
+    GemmDesc d;
+    SplitView a, b, cs, cr;
+};
+int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, hipStream_t s);
 extern int g_presplit_kb;   // k per LDS stage of gemm_bf16x3_presplit (32 or 64)
 extern int g_presplit_swizzle;   // 1: its LDS rows unpadded with XOR-swizzled chunks
 extern int g_wide_min_wgs;       // its 256 x 128 kernel for launches of at least this many workgroups
