@@ -48,8 +48,8 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * in the launch of this block's Z = S W (1) or in a launch of its own (0); key 13 = FP64 launches of
  * fewer 64 x 64 workgroups than the value take 32 x 32 tiles (0: never; any value >= 0); key 14 =
  * pre-split split-BF16 products without triangle flags and with at least this many 256 x 128
- * workgroups take the 256 x 128 kernel (0: every product, flags included; any value >= 0,
- * 2^30 in effect never); key 15 = the mixed path's merges of L_ii^-1 on split-BF16 (1) or FP64 (0)
+ * workgroups take the 256 x 128 kernel (0: every product without triangular operands, lower-
+ * stored ones included; any value >= 0, 2^30 in effect never); key 15 = the mixed path's merges of L_ii^-1 on split-BF16 (1) or FP64 (0)
  * products.  Returns the previous value,
  * or -1 for an unknown key or value (nothing changed).  Not for concurrent use with
  * running handles. */

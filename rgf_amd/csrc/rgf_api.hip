@@ -110,6 +110,8 @@ struct rgf_handle_s {
     __bf16* split_s = nullptr;  // selected inversion: split copy of the Sigma block S_c (op(A) of Z)
     __bf16* lsplit = nullptr;   // split copies of the coupling panels [B_i; D_i] (as op(A) rows):
                                 // one per block with keep_splits, else the last one written
+    __bf16* xr = nullptr;       // split copy of X_i's rows (the current block's)
+    __bf16* xc = nullptr;       // split copies of X_i's columns: one per block with keep_splits
     bool factorized = false;
     bool broken = false;        // in-place factorization failed: A is gone
     bool generated = false;
@@ -238,6 +240,19 @@ size_t lsplit_need(int ns, int nt, int nd, int count, int keep_splits)
 {
     return (size_t)(keep_splits ? nt : 1) * count * split_entry_elems(ns + nd, ns);
 }
+// the copies of X_i = L_ii^-1 the mixed factorization keeps (ns >= 128): rows (one block's)
+// and columns (every block's with keep_splits)
+size_t xcopy_entry(int ns) { return 2 * (size_t)xcopy_rows(ns) * ((ns + 63) / 64 * 64); }
+size_t xcopy_need(int ns, int nt, int count, int keep_splits)
+{
+    return ns >= 128 ? (size_t)(1 + (keep_splits ? nt : 1)) * count * xcopy_entry(ns) : 0;
+}
+// the h->split scratch: the largest product's operands (and the copy of Z), or a level of
+// the inverse's L21 and T copies
+size_t split_scratch_need(int ns, int nd, int count)
+{
+    return std::max(split_need(ns, nd, count), trtri_scratch_elems(ns, count));
+}
 
 // doubles of the skinny products' split-k partials: the solves and the residual (one
 // matrix, m <= max(n, 2ns+nd)) and the factor sweep's arrow-row update (count matrices
@@ -266,7 +281,8 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
     size_t bytes = (per * count + once) * sizeof(double);
     if (precision == RGF_MIXED_BF16X3)
-        bytes += (split_need(ns, nd, count) + split_x_need(ns, nd, count) + lsplit_need(ns, nt, nd, count, keep_splits)) * 2;
+        bytes += (split_scratch_need(ns, nd, count) + split_x_need(ns, nd, count) +
+                  lsplit_need(ns, nt, nd, count, keep_splits) + xcopy_need(ns, nt, count, keep_splits)) * 2;
     return bytes;
 }
 
@@ -305,9 +321,13 @@ int alloc_core(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->dinfo, (size_t)B))) return rc;
     if ((rc = dalloc(h, &h->scal, 64))) return rc;
     if (h->opt.precision == RGF_MIXED_BF16X3) {
-        h->split_elems = split_need(ns, nd, B);
+        h->split_elems = split_scratch_need(ns, nd, B);
         if ((rc = dalloc(h, &h->split, h->split_elems))) return rc;
         if ((rc = dalloc(h, &h->lsplit, lsplit_need(ns, nt, nd, B, h->opt.keep_splits)))) return rc;
+        if (ns >= 128) {
+            if ((rc = dalloc(h, &h->xr, (size_t)B * xcopy_entry(ns)))) return rc;
+            if ((rc = dalloc(h, &h->xc, (size_t)(h->opt.keep_splits ? nt : 1) * B * xcopy_entry(ns)))) return rc;
+        }
     }
     return RGF_OK;
 }
@@ -458,6 +478,23 @@ __bf16* panel_copy(rgf_handle_s* h, int j)
 {
     return h->lsplit + (size_t)(h->opt.keep_splits ? j : 0) * h->count * split_entry_elems(h->ns + h->nd, h->ns);
 }
+// whole-copy views of X_i's rows / columns (entry = matrix of the handle)
+SplitView xcopy_view(const rgf_handle_s* h, __bf16* p)
+{
+    SplitView v{};
+    v.p = p;
+    v.ldk = (h->ns + 63) / 64 * 64;
+    v.rows = xcopy_rows(h->ns);
+    v.lo = (long long)v.rows * v.ldk;
+    v.s1 = v.s2 = 2 * v.lo;   // one entry per matrix, whichever level of the descriptor counts them
+    v.pad = 1;
+    return v;
+}
+SplitView xr_view(const rgf_handle_s* h) { return xcopy_view(h, h->xr); }
+SplitView xc_view(const rgf_handle_s* h, int j)
+{
+    return xcopy_view(h, h->xc + (size_t)(h->opt.keep_splits ? j : 0) * h->count * xcopy_entry(h->ns));
+}
 
 // ------------------------------------------------------------------ factorize
 // FirstStageFactor (RGF.H:233-340) with every block resident, over all matrices
@@ -523,8 +560,13 @@ int factor_sweep(rgf_handle_s* h)
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, false, s)))
             return rc;
+        // mixed path: the inverse keeps split copies of X_i's rows (the product below) and
+        // columns (the selected inversion's)
+        TrtriCopies xcp{xr_view(h), h->xc ? xc_view(h, i) : SplitView{}, h->split, h->split_elems};
+        const bool xcopies = h->xr && g_trtri_mixed;
         if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s,
-                                    h->opt.precision == RGF_MIXED_BF16X3 && g_trtri_mixed ? &pf : nullptr)))
+                                    h->opt.precision == RGF_MIXED_BF16X3 && g_trtri_mixed ? &pf : nullptr, 0,
+                                    xcopies ? &xcp : nullptr)))
             return rc;
         const bool has_next = i < nt - 1;
         double* const Fc = fp ? P + ns : h->Ftmp;   // coupling rows of F
@@ -539,10 +581,15 @@ int factor_sweep(rgf_handle_s* h)
                         // writes the split copy of [B_i; D_i]
                 __bf16* const As = h->split;
                 __bf16* const Bs = h->split + (size_t)B * split_entry_elems(mF, ns);
-                if ((rc = split_a(h, Op::N, d, As)) ||
-                    (rc = split_operand(d.B, d.ldb, d.n, d.k, false, d.count, d.sB, d.count2, d.sB2, Bs, s)) ||
-                    (rc = gemm_bf16x3_presplit(Op::N, Op::T, &d, As, Bs, s, nullptr, panel_copy(h, i))))
+                if ((rc = split_a(h, Op::N, d, As))) return rc;
+                PresplitCall c{d, split_view(As, mF, ns, d.count), split_view(Bs, ns, ns, d.count), SplitView{},
+                               split_view(panel_copy(h, i), mF, ns, d.count)};
+                if (xcopies) {   // op(B)^T = X_i's rows: the inverse's copy
+                    c.b = xr_view(h);
+                } else if ((rc = split_operand(d.B, d.ldb, d.n, d.k, false, d.count, d.sB, d.count2, d.sB2, Bs, s))) {
                     return rc;
+                }
+                if ((rc = gemm_bf16x3_presplit_n(Op::N, Op::T, &c, 1, s))) return rc;
             } else if ((rc = prod(h, Op::N, Op::T, &d, 1))) {
                 return rc;
             }
@@ -892,7 +939,14 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
         GemmDesc dx = gdesc(P, ldp, P, ldp, Sn, lds, ns, ns, ns, 1.0, 0.0, G_CLOWER | G_AUPPER | G_BLOWER);
         batch2(dx, B, ms, ms, ss);
         const bool px = presplit_ok(h, dx);
-        if (px && (rc = split_a(h, Op::T, dx, h->split_x))) return rc;
+        // X_i's columns: the factorization's copy (keep_splits), else split here
+        const bool kept_x = px && h->opt.keep_splits && h->xc && g_trtri_mixed;
+        const SplitView xv = kept_x ? xc_view(h, i) : split_view(h->split_x, ns, ns, dx.count);
+        if (px && !kept_x && (rc = split_a(h, Op::T, dx, h->split_x))) return rc;
+        auto xtx = [&]() {   // X^T X from the copies of X
+            const PresplitCall c{dx, xv, xv, SplitView{}, SplitView{}};
+            return gemm_bf16x3_presplit_n(Op::T, Op::N, &c, 1, s);
+        };
         if (pipe) {
             double* const Wi = wb[i & 1];
             GemmDesc zz[2];
@@ -921,9 +975,11 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             if (pw) {   // op(A) = [B_i; D_i]: the factorization's copy (keep_splits) or split now;
                         // op(B) = X: the split of X^T X's op(A); the product writes the split of W
                 const bool kept = h->opt.keep_splits && panel_presplit(h, i);
-                const __bf16* const Fs = kept ? panel_copy(h, i) : rT;
+                __bf16* const Fs = kept ? panel_copy(h, i) : rT;
+                const PresplitCall ww{d, split_view(Fs, mF, ns, d.count), xv, split_view(rW, ns, mF, d.count),
+                                      SplitView{}};
                 if ((!kept && (rc = split_a(h, Op::N, d, rT))) ||
-                    (rc = gemm_bf16x3_presplit(Op::N, Op::N, &d, Fs, h->split_x, s, rW)))
+                    (rc = gemm_bf16x3_presplit_n(Op::N, Op::N, &ww, 1, s)))
                     return rc;
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
@@ -944,7 +1000,7 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             // finishing pass adds it into Sn (the same sum as accumulating it, same bits)
             const bool fuse = h->opt.precision == RGF_FP64 && g_selinv_fuse;
             if (px) {
-                if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &dx, h->split_x, h->split_x, s))) return rc;
+                if ((rc = xtx())) return rc;
             } else if (!fuse && (rc = prod(h, Op::T, Op::N, &dx, 1))) {
                 return rc;
             }
@@ -962,7 +1018,7 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
                 return rc;
             }
         } else if (px) {
-            if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &dx, h->split_x, h->split_x, s))) return rc;
+            if ((rc = xtx())) return rc;
         } else if ((rc = prod(h, Op::T, Op::N, &dx, 1))) {
             return rc;
         }

@@ -535,8 +535,48 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(double* A, int lda, lon
     }
 }
 
+// The 128-row diagonal blocks of X (complete once the 64-row merges are done) into both split
+// copies, with their zeros above the diagonal and beyond n, and the k padding [n, ldk) of
+// their rows.  grid: (ceil(n / 128), count); lanes along k.
+__global__ __launch_bounds__(256) void trtri_copy_kernel(const double* A, int lda, long long sA, int n, SplitView xr,
+                                                         SplitView xc)
+{
+    const long long e = blockIdx.y;
+    A += e * sA;
+    const int r0 = 128 * blockIdx.x;
+    for (int t = threadIdx.x; t < 128 * 128; t += 256) {
+        const int a = t & 127, b = t >> 7;
+        const int r = r0 + b, k = r0 + a;   // xr: X[r][k]; xc: X[k][r]
+        if (r < n && k < xr.ldk) {
+            split_store_v(xr, e, 0, r, k, (k < n && r >= k) ? A[(long long)k * lda + r] : 0.0);
+            split_store_v(xc, e, 0, r, k, (k < n && k >= r) ? A[(long long)r * lda + k] : 0.0);
+        }
+    }
+    const int pad = xr.ldk - n;   // both copies have the pitch roundup(n, 64)
+    for (int t = threadIdx.x; t < 128 * pad; t += 256) {
+        const int r = r0 + t / pad, k = n + t % pad;
+        if (r < n && k >= r0 + 128) {   // (inside the block: written above)
+            split_store_v(xr, e, 0, r, k, 0.0);
+            split_store_v(xc, e, 0, r, k, 0.0);
+        }
+    }
+}
+
+size_t trtri_scratch_elems(int n, int count)
+{
+    // per level: the split copies of L21 (n2 x n1) and of T (n1 x n2) for every group
+    const int T = (n + NB - 1) / NB;
+    size_t need = 0;
+    for (int sz = 2; sz < T; sz *= 2) {
+        const int n1 = sz * NB, G = (T - sz + 2 * sz - 1) / (2 * sz);
+        need = std::max(need, 2 * (size_t)G * count * split_entry_elems(n1, n1));
+    }
+    return need;
+}
+
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
-                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod, long long plane)
+                      const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod, long long plane,
+                      const TrtriCopies* cp)
 {
     auto run = [&](GemmDesc* d, int nd) {
         return prod ? prod->fn(prod->ctx, Op::N, Op::N, d, nd, s) : gemm_f64(Op::N, Op::N, d, nd, s);
@@ -549,16 +589,31 @@ int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
     }
+    bool copied = false;
+    auto copy_blocks = [&]() {   // the 128-row diagonal blocks into the copies (once)
+        if (!cp || copied) return 0;
+        copied = true;
+        hipLaunchKernelGGL(trtri_copy_kernel, dim3((n + 127) / 128, count), dim3(256), 0, s, A, lda, sA, n, cp->xr,
+                           cp->xc);
+        return (int)hipGetLastError();
+    };
     for (int sz = 1; sz < T; sz *= 2) {
         const int G = (T - sz + 2 * sz - 1) / (2 * sz);   // groups with a non-empty right half
         const int n1 = sz * NB;
         const long long gstride = (long long)(2 * sz * NB) * (1 + (long long)lda);
         const long long tstride = (long long)n1 * n1;
+        const bool split = cp && n1 >= 128;
+        if (split) {
+            int rc = copy_blocks();
+            if (rc) return rc;
+        }
         // groups whose right half is complete share one descriptor
         int gfull = 0;
         for (int g = 0; g < G; g++)
             if ((g * 2 * sz + 2 * sz) * NB <= n) gfull = g + 1;
         GemmDesc d1[2], d2[2];
+        PresplitCall c1[2], c2[2];
+        __bf16* scr = split ? cp->scratch : nullptr;
         int nd1 = 0;
         for (int part = 0; part < 2; part++) {
             int g0 = part == 0 ? 0 : gfull;
@@ -578,14 +633,36 @@ int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double
             GemmDesc b = gdesc(X22, lda, Tg, n1, L21, lda, n2, n1, n2, -1.0, 0.0, G_ALOWER);
             b.count = cnt; b.sA = gstride; b.sB = tstride; b.sC = gstride; b.sCin = gstride;
             batch2(b, count, sA, sT, sA);
-            d1[nd1] = a; d2[nd1] = b; nd1++;
+            d1[nd1] = a; d2[nd1] = b;
+            if (split) {
+                // L21 split here; X11 (columns) and X22 (rows) read from the copies; T's copy
+                // written by its product; X21 written into both copies by its product
+                const size_t le = (size_t)cnt * count * split_entry_elems(n2, n1);
+                const size_t te = (size_t)cnt * count * split_entry_elems(n1, n2);
+                if ((size_t)(scr - cp->scratch) + le + te > cp->scratch_elems) return (int)hipErrorInvalidValue;
+                __bf16* const Ls = scr;
+                __bf16* const Ts = scr + le;
+                scr += le + te;
+                int rc = split_operand(L21, lda, n2, n1, false, cnt, gstride, count, sA, Ls, s);
+                if (rc) return rc;
+                const int step = 2 * sz * NB;
+                const long long st_r = split_block_offset(cp->xr, step, step);
+                const long long st_c = split_block_offset(cp->xc, step, step);
+                c1[nd1] = PresplitCall{a, split_view(Ls, n2, n1, cnt),
+                                       split_subview(cp->xc, c0, c0, st_c, cp->xc.s1), split_view(Ts, n1, n2, cnt),
+                                       SplitView{}};
+                c2[nd1] = PresplitCall{b, split_subview(cp->xr, r0, r0, st_r, cp->xr.s1), split_view(Ts, n1, n2, cnt),
+                                       split_subview(cp->xc, c0, r0, st_c, cp->xc.s1),
+                                       split_subview(cp->xr, r0, c0, st_r, cp->xr.s1)};
+            }
+            nd1++;
         }
-        int rc = run(d1, nd1);
+        int rc = split ? gemm_bf16x3_presplit_n(Op::N, Op::N, c1, nd1, s) : run(d1, nd1);
         if (rc) return rc;
-        rc = run(d2, nd1);
+        rc = split ? gemm_bf16x3_presplit_n(Op::N, Op::N, c2, nd1, s) : run(d2, nd1);
         if (rc) return rc;
     }
-    return 0;
+    return copy_blocks();   // (no merge of 128 rows or more: the copies are the diagonal blocks)
 }
 
 // ------------------------------------------------------------------ small kernels

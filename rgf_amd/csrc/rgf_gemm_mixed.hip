@@ -835,7 +835,7 @@ int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, 
     GemmArgs args{};
     PresplitArgs pa{};
     long long work128 = 0, wide_wgs = 0;
-    bool all_full = true;
+    bool all_full = true, no_tri = true;
     int nd = 0;
     for (int q = 0; q < ncalls; q++) {
         GemmDesc d = calls[q].d;
@@ -859,7 +859,11 @@ int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, 
         wide_wgs += wt * entries;
         // buffer-load ranges: both planes of an entry's copy below 2 GB from its origin
         const long long span = 2 * std::max(O.a.lo + (long long)O.a.rows * O.a.ldk, O.b.lo + (long long)O.b.rows * O.b.ldk);
+        // products with triangular operands never take it: its 256-row tiles would read the
+        // copies' k-blocks above the 128-row diagonal blocks, which the copies of L^-1 leave unwritten
+        const bool tri_op = d.flags & (G_ALOWER | G_AUPPER | G_BLOWER | G_BUPPER);
         all_full = all_full && d.flags == 0 && span < (1LL << 31);
+        no_tri = no_tri && !tri_op && span < (1LL << 31);
         args.d[nd++] = d;
     }
     if (nd == 0) return 0;
@@ -867,7 +871,7 @@ int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, 
     // By default only full products take the 256 x 128 kernel: in the c2 sweep (batch of 8) the
     // products with triangular operands (unequal tiles) and the lower-stored ones ran 2-18 %
     // slower on it than on twice as many 128 x 128 workgroups, the full product Z = S W 4 % faster.
-    const bool wide = (all_full || g_wide_min_wgs == 0) && wide_wgs >= g_wide_min_wgs;
+    const bool wide = (all_full || (g_wide_min_wgs == 0 && no_tri)) && wide_wgs >= g_wide_min_wgs;
     const int BT = work128 >= 768 ? 128 : 64;
     int z = 0, maxtiles = 0;
     for (int q = 0; q < nd; q++) {

@@ -118,6 +118,46 @@ int split_operand(const double* X, int ld, int rows, int k, bool kcontig, int co
 // [B; D] read for B alone); 0 otherwise.
 int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s,
                          __bf16* Cs = nullptr, __bf16* Cr = nullptr, int rowsA = 0, int rowsB = 0);
+// ---------------------------------------------------------------- split-BF16 copies
+// Element (r, k) of one plane of a split copy whose k pitch is 32 * nkb: 16-row x 32-k blocks
+// of 1 KB (block (r / 16, k / 32) at element ((r / 16) * nkb + k / 32) * 512), inside a block
+// 64 chunks of 8 k, chunk 4 (r % 16) + ((k / 8) % 4 XOR (r / 4) % 4).
+__host__ __device__ __forceinline__ long long split_index(int r, int k, int nkb)
+{
+    return ((long long)(r >> 4) * nkb + (k >> 5)) * 512 +
+           ((((r & 15) << 2) | (((k >> 3) & 3) ^ ((r >> 2) & 3))) << 3) + (k & 7);
+}
+// x -> hi = x rounded once to bf16 (nearest, ties to even), f = (float)x, lo = bf16(f - hi):
+// x = hi + lo to about 2^-16 relative.  hi goes through x rounded to float with round-to-odd
+// (truncate, then set the last bit when inexact), which a second rounding to the shorter
+// bf16 leaves equal to the single rounding of x; written out so that every kernel produces
+// the same bits (left to the compiler, (bf16)(float)x is folded into one rounding in some
+// kernels and rounded twice in others, which differ where (float)x lies on a bf16 tie).
+__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo)
+{
+    const float f = (float)x;
+    unsigned u = __float_as_uint(f);
+    const double fd = (double)f;
+    if (fd != x && !(u & 1u)) u += fabs(x) > fabs(fd) ? 1u : 0xffffffffu;
+    hi = (__bf16)__uint_as_float(u);
+    lo = (__bf16)(f - (float)hi);
+}
+// A split copy written by a kernel beside its FP64 result: entry e at p + e * stride, lo
+// plane at + lo, k pitch ldk (p null: none)
+struct SplitOut {
+    __bf16* p;
+    long long stride, lo;
+    int ldk;
+};
+__device__ __forceinline__ void split_store(const SplitOut& o, long long e, int r, int k, double v)
+{
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    const long long i = e * o.stride + split_index(r, k, o.ldk >> 5);
+    o.p[i] = h;
+    o.p[o.lo + i] = l;
+}
+
 // A split copy, or a block of one, as an operand or destination of a pre-split product: its
 // origin at p (a multiple of 16 rows and 32 k inside the copy), lo plane at p + lo, k pitch
 // ldk (a multiple of 64), rows readable from p; entry (b1, b2) of the product's descriptor
@@ -129,7 +169,33 @@ struct SplitView {
     int ldk, rows, pad;
 };
 // the layout split_operand writes: rows x k per entry, count x count2 entries one after another
+// (count: the inner entry count of the product's descriptor, GemmDesc::count)
 SplitView split_view(__bf16* p, int rows, int k, int count);
+// element (r, k) of entry (b1, b2) of a view, both planes
+__device__ __forceinline__ void split_store_v(const SplitView& v, long long b1, long long b2, int r, int k, double x)
+{
+    __bf16 h, l;
+    split_bf16(x, h, l);
+    const long long i = b1 * v.s1 + b2 * v.s2 + split_index(r, k, v.ldk >> 5);
+    v.p[i] = h;
+    v.p[v.lo + i] = l;
+}
+// the block of a view at row r, k c (multiples of 16 and 32), entry (b1, b2) steps s1, s2
+inline SplitView split_subview(const SplitView& v, int r, int c, long long s1, long long s2)
+{
+    SplitView o = v;
+    o.p = v.p + ((long long)(r >> 4) * (v.ldk >> 5) + (c >> 5)) * 512;
+    o.rows = v.rows - r;
+    o.s1 = s1;
+    o.s2 = s2;
+    o.pad = 0;
+    return o;
+}
+// element offset of the block at row r, k c of a view (for entry steps of sub-blocks)
+inline long long split_block_offset(const SplitView& v, int r, int c)
+{
+    return ((long long)(r >> 4) * (v.ldk >> 5) + (c >> 5)) * 512;
+}
 // Up to two products of the same transposes in one launch (the tiles of both in one grid).
 // cs / cr (p null: none): the product's stored C as op(C)^T / op(C) copies (see Cs / Cr).
 struct PresplitCall {
@@ -179,13 +245,29 @@ struct ProdFn {
     int (*fn)(void* ctx, Op ta, Op tb, GemmDesc* d, int nd, hipStream_t s);
     void* ctx;
 };
+// Split copies of X = L^-1 that the mixed path's inverse keeps up to date (see trtri below):
+// xr holds X's rows (element (r, k) = X[r][k]), xc X's columns (element (j, k) = X[k][j]),
+// each a whole copy per matrix (entry b at p + b * s1) with xcopy_rows(n) rows; scratch holds
+// the split copies of one level's L21 and T blocks (trtri_scratch_elems(n, count) bf16).
+struct TrtriCopies {
+    SplitView xr, xc;
+    __bf16* scratch;
+    size_t scratch_elems;
+};
+inline int xcopy_rows(int n) { return (n + 127) / 128 * 128 + 128; }
+size_t trtri_scratch_elems(int n, int count);
 // In-place L^-1; xdiag (may be null) holds the diagonal tiles already inverted.  The
 // merge products run through `prod` (null: FP64 GEMM).  plane != 0: complex L stored as
 // two planes (imaginary part `plane` elements after the real part, for A, tmp and xdiag
 // alike); the diagonal tiles of both planes are placed and `prod` multiplies complex.
+// cp (mixed path): the merges of blocks of at least 128 rows run as pre-split products that
+// read X11 and X22 from the copies and write each X21 into both of them (only L21 and T are
+// split); the 128-row diagonal blocks, their zeros above the diagonal and the k padding
+// enter the copies from a pass after the smaller merges.  Regions above the diagonal outside
+// those blocks are never written (a product with triangle flags never reads them).
 int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double* tmp, long long sT,
                       const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod = nullptr,
-                      long long plane = 0);
+                      long long plane = 0, const TrtriCopies* cp = nullptr);
 int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
            hipStream_t s, int count = 1, long long sS = 0, long long sD = 0);
@@ -236,38 +318,6 @@ double gen_uniform_host(uint64_t seed, uint32_t mid, uint32_t idx);
 int scatter_csc(const uint64_t* ia, const uint64_t* ja, const double* a, long long nnz, int ns,
                 int nt, int nd, double* P, long long pstride, int ldp, double* Cdd, hipStream_t s);
 int mirror_lower(double* A, int lda, int n, hipStream_t s);
-
-// ---------------------------------------------------------------- split-BF16 copies
-// Element (r, k) of one plane of a split copy whose k pitch is 32 * nkb: 16-row x 32-k blocks
-// of 1 KB (block (r / 16, k / 32) at element ((r / 16) * nkb + k / 32) * 512), inside a block
-// 64 chunks of 8 k, chunk 4 (r % 16) + ((k / 8) % 4 XOR (r / 4) % 4).
-__host__ __device__ __forceinline__ long long split_index(int r, int k, int nkb)
-{
-    return ((long long)(r >> 4) * nkb + (k >> 5)) * 512 +
-           ((((r & 15) << 2) | (((k >> 3) & 3) ^ ((r >> 2) & 3))) << 3) + (k & 7);
-}
-// x -> f = (float)x, hi = bf16(f), lo = bf16(f - hi): x = hi + lo to about 2^-16 relative
-__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo)
-{
-    const float f = (float)x;
-    hi = (__bf16)f;
-    lo = (__bf16)(f - (float)hi);
-}
-// A split copy written by a kernel beside its FP64 result: entry e at p + e * stride, lo
-// plane at + lo, k pitch ldk (p null: none)
-struct SplitOut {
-    __bf16* p;
-    long long stride, lo;
-    int ldk;
-};
-__device__ __forceinline__ void split_store(const SplitOut& o, long long e, int r, int k, double v)
-{
-    __bf16 h, l;
-    split_bf16(v, h, l);
-    const long long i = e * o.stride + split_index(r, k, o.ldk >> 5);
-    o.p[i] = h;
-    o.p[o.lo + i] = l;
-}
 
 // Mirror the lower triangle of A into its upper one (and gather the diagonal); with X the
 // lower triangle first receives A + X; with cp (mixed path) the split copy of the whole

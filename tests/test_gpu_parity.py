@@ -596,9 +596,10 @@ def _op_zero_triangles(X, ld, rows, k, trans, lower):
 @pytest.mark.parametrize("m,n,k", [(333, 333, 333), (70, 70, 70), (300, 520, 260), (600, 200, 600)])
 def test_gemm_bf16x3_wide_matches_staged(gpu, ta, tb, flags, m, n, k):
     """The 256 x 128 pre-split kernel (buffer loads into LDS, forced here for every launch
-    size) sums every output over the same k-blocks in the same MFMA order as the staged
+    size of the products without triangular operands; those with them keep the 128 x 128
+    kernel) sums every output over the same k-blocks in the same MFMA order as the staged
     kernel, so both give the same bits; the operands hold the zeros their triangle flags
-    promise (the two kernels skip different k-blocks of them)."""
+    promise."""
     say(f"gemm bf16x3 wide vs staged ta={ta} tb={tb} flags={flags} {m}x{n}x{k}")
     lib = _lib.load()
     rng = np.random.default_rng(flags + m + 7 * n)
@@ -649,6 +650,23 @@ def test_mixed_sweep_wide_kernel_same_bits(gpu, count):
                 res.append((h.logdet_batch(), h.selinv_diag_batch()))
         finally:
             lib.rgf_debug_set_tuning(14, old)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
+def test_mixed_kept_splits_same_bits(gpu):
+    """Sweeps that keep split copies (written by product epilogues, the mirror and the
+    L_ii^-1 merges, read again by later products) give the bits of sweeps that split every
+    operand where it is read.  At this size millions of split values are written into
+    copies, so some land on a bf16 tie after the FP64 -> FP32 step: every split rounds the
+    FP64 value once, wherever it is made (splits that rounded twice in the L_ii^-1 copies
+    changed 12 diagonal entries of this case)."""
+    say("mixed sweep keep_splits 1 vs 0, ns=1500 nt=4 nd=6 count=2")
+    res = []
+    for keep in (1, 0):
+        opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3, keep_splits=keep)
+        with rgf_amd.RGF.generated(1500, 4, 6, seed=31, count=2, theta_mode=1, options=opts) as h:
+            res.append((h.logdet_batch(), h.selinv_diag_batch()))
     assert np.array_equal(res[0][0], res[1][0])
     assert np.array_equal(res[0][1], res[1][1])
 
