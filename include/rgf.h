@@ -145,6 +145,24 @@ int rgf_set_device_cache_limit(size_t bytes, size_t *previous);
 /* Extension: release every wholly free block of the arena of `device` (-1: all devices). */
 int rgf_trim_device_cache(int device);
 
+/* Extension (no counterpart in the reference, whose products are MAGMA calls): tile shapes
+ * of the product launches, chosen per launch signature -- the transposes and, for each
+ * product of the launch, m, n, k, its triangle flags and batch entries.  mode 0: fixed size
+ * thresholds (default); 1: the cached choice of a signature where there is one, thresholds
+ * otherwise; 2: as 1, and a launch whose signature is not cached first runs every candidate
+ * tile shape on scratch outputs (its real outputs untouched; not under HIP graph capture,
+ * where thresholds apply), times them with HIP events and caches the fastest.  Every tile
+ * shape sums each output in the same order, so the choice never changes a result bit.
+ * Process-wide; tune while one thread drives the device, so the timings are its own.
+ * *previous (may be NULL) receives the old mode. */
+int rgf_set_autotune(int mode, int *previous);
+/* Number of cached signatures; clear the cache; save it as text, or add a saved one
+ * (RGF_EINVAL for an unreadable or malformed file, the cache then unchanged). */
+int rgf_autotune_entries(void);
+int rgf_autotune_clear(void);
+int rgf_autotune_save(const char *path);
+int rgf_autotune_load(const char *path);
+
 /* Message of the last failed call on h (or of the last failed create when h is NULL). */
 const char *rgf_last_error(rgf_handle_t h);
 

@@ -9,6 +9,7 @@ tests read like the reference's own driver (main.C:86-121).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -365,6 +366,34 @@ def set_device_cache_limit(nbytes: int) -> int:
 def trim_device_cache(device: int = -1) -> None:
     """Give every wholly free block of the arena back to the device (-1: all devices)."""
     _check(_lib.load().rgf_trim_device_cache(int(device)), None)
+
+
+def set_autotune(mode: int) -> int:
+    """Tile-shape choice of the product launches: 0 size thresholds, 1 cached per-signature
+    choices, 2 the same and time every candidate for signatures not cached yet (results
+    are bitwise the same whichever shape runs).  Returns the previous mode."""
+    prev = C.c_int()
+    _check(_lib.load().rgf_set_autotune(int(mode), C.byref(prev)), None)
+    return prev.value
+
+
+def autotune_entries() -> int:
+    """Launch signatures with a cached tile-shape choice."""
+    return int(_lib.load().rgf_autotune_entries())
+
+
+def autotune_clear() -> None:
+    _check(_lib.load().rgf_autotune_clear(), None)
+
+
+def autotune_save(path: str) -> None:
+    """Write the tile-shape cache to a text file."""
+    _check(_lib.load().rgf_autotune_save(os.fsencode(path)), None)
+
+
+def autotune_load(path: str) -> None:
+    """Add the choices of a file written by autotune_save to the cache."""
+    _check(_lib.load().rgf_autotune_load(os.fsencode(path)), None)
 
 
 def kernel_timing_enable(op_class: int) -> None:

@@ -28,7 +28,8 @@ EXPORTS = [
     "rgf_export_factor_mf", "rgf_kernel_timing_enable", "rgf_kernel_timing_read",
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats", "rgf_set_device_cache_limit",
-    "rgf_trim_device_cache", "rgf_solve_device_noalloc",
+    "rgf_trim_device_cache", "rgf_solve_device_noalloc", "rgf_set_autotune", "rgf_autotune_entries",
+    "rgf_autotune_clear", "rgf_autotune_save", "rgf_autotune_load",
     "rgf_z_create_csc", "rgf_z_destroy", "rgf_z_factorize", "rgf_z_solve", "rgf_z_logdet", "rgf_z_selinv_diag",
     "rgf_z_residual_norm", "rgf_z_residual_norm_normalized", "rgf_z_last_error",
 ]
@@ -122,6 +123,11 @@ def load():
         lib.rgf_z_last_error.argtypes = [H]
         lib.rgf_z_last_error.restype = C.c_char_p
         lib.rgf_trim_device_cache.argtypes = [C.c_int]
+        lib.rgf_set_autotune.argtypes = [C.c_int, C.POINTER(C.c_int)]
+        lib.rgf_autotune_entries.argtypes = []
+        lib.rgf_autotune_clear.argtypes = []
+        lib.rgf_autotune_save.argtypes = [C.c_char_p]
+        lib.rgf_autotune_load.argtypes = [C.c_char_p]
         lib.rgf_debug_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int,
                                        dp, C.c_int, C.c_double, dp, C.c_int, C.c_int]
         lib.rgf_debug_gemm.restype = C.c_int

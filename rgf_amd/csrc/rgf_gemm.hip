@@ -302,7 +302,16 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
 {
     if (ndesc <= 0) return 0;
     if (ndesc > GEMM_MAXD) return (int)hipErrorInvalidValue;
-    if (cfg < 0) cfg = wgs64(descs, ndesc) < g_small_tile_wgs ? 3 : 0;
+    if (cfg < 0) {
+        const int def = wgs64(descs, ndesc) < g_small_tile_wgs ? 3 : 0;
+        const int cands[4] = {def, def == 0 ? 3 : 0, 1, 2};
+        cfg = tune_choose(0, ta, tb, descs, ndesc, cands, 4, s, [&](int c, const GemmDesc* dd) {
+            GemmDesc t[GEMM_MAXD];
+            std::copy(dd, dd + ndesc, t);
+            return gemm_f64(ta, tb, t, ndesc, s, c);
+        });
+        if (cfg < 0 || cfg > 3) cfg = def;
+    }
     const int TM = cfg == 3 ? 32 : (cfg >= 1 ? 128 : 64), TN = cfg == 3 ? 32 : (cfg == 2 ? 128 : 64);
     GemmArgs args{};
     int z = 0, maxtiles = 0, nd = 0;
@@ -337,8 +346,8 @@ int gemm_f64(Op ta, Op tb, GemmDesc* descs, int ndesc, hipStream_t s, int cfg)
 void* timing_begin(int op_class, const GemmDesc* d, int nd, hipStream_t s)
 {
     // class 8 selects every split-BF16 class (4..7), class 9 every FP64 class (0..3)
-    const bool on = op_class == g_timing_class || (g_timing_class == 8 && op_class >= 4 && op_class <= 7) ||
-                    (g_timing_class == 9 && op_class >= 0 && op_class <= 3);
+    const bool on = !tune_in_trial() && (op_class == g_timing_class || (g_timing_class == 8 && op_class >= 4 && op_class <= 7) ||
+                    (g_timing_class == 9 && op_class >= 0 && op_class <= 3));
     if (!on) return nullptr;
     auto* tl = new TimedLaunch{};
     hipEventCreate(&tl->a);

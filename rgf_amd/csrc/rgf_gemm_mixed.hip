@@ -828,7 +828,11 @@ SplitView split_view(__bf16* p, int rows, int k, int count)
     return v;
 }
 
-int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, hipStream_t s)
+// cfg: 0 = 64 x 64 tiles, 1 = 128 x 128, 2 = the 256 x 128 kernel (launches without triangular
+// operands only), -1 = by size; with allowed (may be null) the configurations this launch
+// may take (bit c for cfg c)
+static int presplit_launch(Op ta, Op tb, const PresplitCall* calls, int ncalls, hipStream_t s, int cfg,
+                           int* allowed = nullptr, int* automatic = nullptr)
 {
     if (ncalls <= 0) return 0;
     if (ncalls > PRESPLIT_MAXD) return (int)hipErrorInvalidValue;
@@ -871,8 +875,14 @@ int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, 
     // By default only full products take the 256 x 128 kernel: in the c2 sweep (batch of 8) the
     // products with triangular operands (unequal tiles) and the lower-stored ones ran 2-18 %
     // slower on it than on twice as many 128 x 128 workgroups, the full product Z = S W 4 % faster.
-    const bool wide = (all_full || (g_wide_min_wgs == 0 && no_tri)) && wide_wgs >= g_wide_min_wgs;
-    const int BT = work128 >= 768 ? 128 : 64;
+    const bool wide_auto = (all_full || (g_wide_min_wgs == 0 && no_tri)) && wide_wgs >= g_wide_min_wgs;
+    const int BT_auto = work128 >= 768 ? 128 : 64;
+    if (allowed) *allowed = 3 | (no_tri ? 4 : 0);
+    if (automatic) *automatic = wide_auto ? 2 : (BT_auto == 128 ? 1 : 0);
+    if (allowed || automatic) return 0;   // (a query: nothing launched)
+    if (cfg == 2 && !no_tri) return (int)hipErrorInvalidValue;
+    const bool wide = cfg < 0 ? wide_auto : cfg == 2;
+    const int BT = cfg < 0 ? BT_auto : (cfg == 1 ? 128 : 64);
     int z = 0, maxtiles = 0;
     for (int q = 0; q < nd; q++) {
         GemmDesc& d = args.d[q];
@@ -917,6 +927,34 @@ int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, 
     hipError_t e = hipGetLastError();
     timing_end(tok, s);
     return (int)e;
+}
+
+int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, hipStream_t s)
+{
+    if (ncalls <= 0) return 0;
+    if (ncalls > PRESPLIT_MAXD) return (int)hipErrorInvalidValue;
+    int allowed = 0, def = 0;
+    int rc = presplit_launch(ta, tb, calls, ncalls, s, -1, &allowed, &def);
+    if (rc) return rc;
+    int cands[3], nc = 0;
+    cands[nc++] = def;
+    for (int c = 0; c < 3; c++)
+        if (c != def && (allowed >> c & 1)) cands[nc++] = c;
+    GemmDesc d[PRESPLIT_MAXD];
+    for (int q = 0; q < ncalls; q++) d[q] = calls[q].d;
+    int cfg = tune_choose(1, ta, tb, d, ncalls, cands, nc, s, [&](int c, const GemmDesc* dd) {
+        // trial launches: outputs redirected, no copies written
+        PresplitCall t[PRESPLIT_MAXD];
+        for (int q = 0; q < ncalls; q++) {
+            t[q] = calls[q];
+            t[q].d = dd[q];
+            t[q].cs = SplitView{};
+            t[q].cr = SplitView{};
+        }
+        return presplit_launch(ta, tb, t, ncalls, s, c);
+    });
+    if (cfg < 0 || !(allowed >> cfg & 1)) cfg = def;
+    return presplit_launch(ta, tb, calls, ncalls, s, cfg);
 }
 
 int gemm_bf16x3_presplit(Op ta, Op tb, const GemmDesc* d, const __bf16* As, const __bf16* Bs, hipStream_t s,

@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <functional>
+
 #define RGF_HIP_CHECK(expr)                                                        \
     do {                                                                           \
         hipError_t _e = (expr);                                                    \
@@ -206,6 +208,15 @@ int gemm_bf16x3_presplit_n(Op ta, Op tb, const PresplitCall* calls, int ncalls, 
 extern int g_presplit_kb;   // k per LDS stage of gemm_bf16x3_presplit (32 or 64)
 extern int g_presplit_swizzle;   // 1: its LDS rows unpadded with XOR-swizzled chunks
 extern int g_wide_min_wgs;       // its 256 x 128 kernel for launches of at least this many workgroups
+
+// Tile-shape choice of a product launch (kind 0: FP64 GEMM, 1: pre-split split-BF16): the
+// cached configuration of its signature, or -1 (then the launcher's size thresholds apply).
+// With tuning on and the signature not cached, every candidate (cands[0]: the threshold
+// choice) is first run through `run` on descriptors whose outputs are redirected to
+// scratch, timed, and the fastest cached.  Not inside HIP graph capture.
+int tune_choose(int kind, Op ta, Op tb, const GemmDesc* d, int nd, const int* cands, int ncand, hipStream_t s,
+                const std::function<int(int, const GemmDesc*)>& run);
+bool tune_in_trial();   // a trial launch of tune_choose is being issued (no kernel timing)
 
 // Kernel-timing hooks shared by the GEMM launchers (op class: 0..3 FP64 NN/NT/TN/TT,
 // 4..7 the same transposes on split-BF16).

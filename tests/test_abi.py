@@ -101,3 +101,43 @@ def test_tuning_switches_validate_without_device():
     cur = lib.rgf_debug_set_tuning(14, 0)   # threshold of the 256 x 128 split-BF16 tiles: any value >= 0
     assert cur >= 0 and lib.rgf_debug_set_tuning(14, -1) == -1
     assert lib.rgf_debug_set_tuning(14, cur) == 0
+
+
+def test_autotune_cache_without_device(tmp_path):
+    """The tile-shape cache: mode validation, save / load round trip, malformed files
+    rejected with the cache unchanged (no device call)."""
+    prev = rgf_amd.set_autotune(0)
+    try:
+        with pytest.raises(rgf_amd.RGFError):
+            rgf_amd.set_autotune(3)
+        assert rgf_amd.set_autotune(1) == 0
+        rgf_amd.autotune_clear()
+        assert rgf_amd.autotune_entries() == 0
+        p = tmp_path / "tune.txt"
+        # signature: kind, ta, tb, launches' products, then m, n, k, flags, entries per product
+        p.write_text("rgf-autotune 1 gfx950\n"
+                     "9 0 0 1 1 2871 2865 2871 0 1 3 0.512\n"
+                     "14 1 1 0 2 2871 2871 2865 1 8 2871 2865 2865 16 8 2 1.5\n")
+        rgf_amd.autotune_load(str(p))
+        assert rgf_amd.autotune_entries() == 2
+        q = tmp_path / "again.txt"
+        rgf_amd.autotune_save(str(q))
+        rgf_amd.autotune_clear()
+        rgf_amd.autotune_load(str(q))
+        assert rgf_amd.autotune_entries() == 2
+        assert sorted(q.read_text().splitlines()[1:]) == sorted(
+            line.replace("0.512", "0.512000").replace("1.5", "1.500000") for line in p.read_text().splitlines()[1:])
+        for bad in ("rgf-autotune 1 gfx950\n9 0 0 1 1 10 10 10 0 1 7 0.5\n",      # no FP64 shape 7
+                    "rgf-autotune 1 gfx950\n9 1 0 1 1 10 10 10 0 1 3 0.5\n",      # no split-BF16 shape 3
+                    "rgf-autotune 1 gfx950\n9 0 0 1 2 10 10 10 0 1 0 0.5\n",      # length != 4 + 5 products
+                    "rgf-autotune 1 gfx950\n9 0 0 1 1 10 10\n",                    # truncated
+                    "something else\n"):
+            p.write_text(bad)
+            with pytest.raises(rgf_amd.RGFError):
+                rgf_amd.autotune_load(str(p))
+            assert rgf_amd.autotune_entries() == 2
+        with pytest.raises(rgf_amd.RGFError):
+            rgf_amd.autotune_load(str(tmp_path / "missing.txt"))
+    finally:
+        rgf_amd.autotune_clear()
+        rgf_amd.set_autotune(prev)

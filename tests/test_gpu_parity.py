@@ -671,6 +671,42 @@ def test_mixed_kept_splits_same_bits(gpu):
     assert np.array_equal(res[0][1], res[1][1])
 
 
+@pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
+def test_autotune_keeps_bits(gpu, prec, tmp_path):
+    """Tile shapes tuned per launch signature (every candidate timed on scratch outputs the
+    first time a signature is seen) leave logdet and diag(A^-1) bitwise as with the size
+    thresholds; a saved cache, loaded again, gives the same bits too."""
+    say(f"autotune bits prec={prec}")
+
+    def run():
+        with rgf_amd.RGF.generated(300, 5, 6, seed=5, count=3, theta_mode=1,
+                                   options=rgf_amd.default_options(precision=prec)) as h:
+            return h.logdet_batch(), h.selinv_diag_batch()
+
+    prev = rgf_amd.set_autotune(0)
+    try:
+        rgf_amd.autotune_clear()
+        base = run()
+        rgf_amd.set_autotune(2)
+        tuned = run()
+        n = rgf_amd.autotune_entries()
+        assert n > 0
+        path = str(tmp_path / "tune.txt")
+        rgf_amd.autotune_save(path)
+        rgf_amd.autotune_clear()
+        rgf_amd.autotune_load(path)
+        assert rgf_amd.autotune_entries() == n
+        rgf_amd.set_autotune(1)
+        cached = run()
+        assert rgf_amd.autotune_entries() == n      # nothing tuned again
+    finally:
+        rgf_amd.autotune_clear()
+        rgf_amd.set_autotune(prev)
+    for got in (tuned, cached):
+        assert np.array_equal(got[0], base[0])
+        assert np.array_equal(got[1], base[1])
+
+
 @pytest.mark.parametrize("ns,nt,nd", [(200, 6, 6), (65, 4, 5), (300, 3, 0)])
 def test_mixed_precision_vs_fp64(gpu, ns, nt, nd):
     """Mixed path (split-BF16 Schur / solve / Takahashi products, FP64 potrf, logdet and
