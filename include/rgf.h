@@ -192,6 +192,27 @@ int rgf_selinv_diag_device(rgf_handle_t h, double *diag_dev);
 int rgf_selinv_blocks_device(rgf_handle_t h, double *diag_blocks, double *sub_blocks, double *arrow_blocks,
                              double *corner);
 int rgf_synchronize(rgf_handle_t h);
+
+/* ---- handle groups (extension; the reference handles one matrix per object, RGF.H:510-546) ----
+ * Handles of uneven shapes (any ns, nt, nd, batch count and precision, one device) whose
+ * factorizations or selected inversions run as one HIP graph launch: each handle's sweep is
+ * a branch on its own stream, so the sweeps overlap on the device.  The graph is captured on
+ * the first call and replayed by later ones (e.g. the re-factorizations of an INLA
+ * hyperparameter loop after rgf_regenerate or new values); results are bitwise those of the
+ * handles run alone.  The handles must outlive the group, and must not be used by another
+ * thread during a group call.  Group calls are asynchronous like the handles' own; each
+ * handle's stream continues after the group's work. */
+typedef struct rgf_group_s *rgf_group_t;
+int rgf_group_create(const rgf_handle_t *handles, size_t n, rgf_group_t *out);
+int rgf_group_destroy(rgf_group_t g);
+/* Factorize every handle (waits for the sweeps to check the pivots).  An in-place handle
+ * (keep_matrix = 0) must hold a matrix, not its factor (RGF_ESTATE).  RGF_ENOTSPD if a
+ * matrix is not positive definite: rgf_last_error of its handle names the column. */
+int rgf_group_factorize(rgf_group_t g);
+/* diag(A^-1) of every matrix of handle q into diag_devs[q] (device, count_q x n_q);
+ * factorizes handles that are not factorized yet, first, one by one. */
+int rgf_group_selinv_diag_device(rgf_group_t g, double *const *diag_devs);
+int rgf_group_synchronize(rgf_group_t g);
 /* Refinement of the last mixed-precision solve: steps taken and the final
  * max_c ||b_c - A x_c|| / ||b_c||. */
 int rgf_refine_stats(rgf_handle_t h, int *iters, double *rel_residual);

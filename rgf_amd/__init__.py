@@ -18,7 +18,7 @@ from ._lib import (RGF_EDEVICE, RGF_EINVAL, RGF_ENOMEM, RGF_ENOTSPD, RGF_ESTATE,
                    RGF_MIXED_BF16X3, RGF_OK, RGF_SELINV_EXACT, RGF_SELINV_REFERENCE,
                    DeviceBuffer, GenParams, Info, Options, gpu_count)
 
-__all__ = ["RGF", "ZRGF", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
+__all__ = ["RGF", "ZRGF", "Group", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
            "default_gen_params", "gpu_count", "RGF_FP64", "RGF_MIXED_BF16X3",
            "RGF_SELINV_EXACT", "RGF_SELINV_REFERENCE", "RGF_OK", "RGF_EINVAL", "RGF_ENOTSPD",
            "RGF_ENOMEM", "RGF_EDEVICE", "RGF_ESTATE", "mf_size", "canonical_flops"]
@@ -243,6 +243,50 @@ class RGF:
         if self._h:
             _lib.load().rgf_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Group:
+    """Handles of uneven shapes whose factorizations and selected inversions run as one HIP
+    graph launch (one branch per handle, overlapping on the device; captured on the first
+    call, replayed afterwards).  Results equal those of the handles run alone, bit for bit.
+    The handles must outlive the group."""
+
+    def __init__(self, handles: list["RGF"]):
+        self.handles = list(handles)
+        arr = (C.c_void_p * len(self.handles))(*[h._h.value for h in self.handles])
+        self._g = C.c_void_p()
+        _check(_lib.load().rgf_group_create(arr, len(self.handles), C.byref(self._g)), None)
+
+    def _checkg(self, rc: int) -> None:
+        _check(rc, None)   # a group call's message is the process-wide one
+
+    def factorize(self) -> None:
+        self._checkg(_lib.load().rgf_group_factorize(self._g))
+
+    def selinv_diag_device(self, diags: list[DeviceBuffer]) -> None:
+        """diag(A^-1) of every matrix of handle q into diags[q] (count_q * n_q values)."""
+        arr = (C.c_void_p * len(diags))(*[d.ptr.value for d in diags])
+        self._checkg(_lib.load().rgf_group_selinv_diag_device(self._g, arr))
+
+    def synchronize(self) -> None:
+        self._checkg(_lib.load().rgf_group_synchronize(self._g))
+
+    def close(self) -> None:
+        if self._g:
+            _lib.load().rgf_group_destroy(self._g)
+            self._g = C.c_void_p()
 
     def __del__(self):
         try:

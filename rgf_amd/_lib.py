@@ -29,7 +29,8 @@ EXPORTS = [
     "rgf_create_generated_batch", "rgf_batch_count", "rgf_select", "rgf_regenerate", "rgf_footprint_bytes",
     "rgf_logdet_batch", "rgf_selinv_diag_batch", "rgf_refine_stats", "rgf_set_device_cache_limit",
     "rgf_trim_device_cache", "rgf_solve_device_noalloc", "rgf_set_autotune", "rgf_autotune_entries",
-    "rgf_autotune_clear", "rgf_autotune_save", "rgf_autotune_load",
+    "rgf_autotune_clear", "rgf_autotune_save", "rgf_autotune_load", "rgf_group_create", "rgf_group_destroy",
+    "rgf_group_factorize", "rgf_group_selinv_diag_device", "rgf_group_synchronize",
     "rgf_z_create_csc", "rgf_z_destroy", "rgf_z_factorize", "rgf_z_solve", "rgf_z_logdet", "rgf_z_selinv_diag",
     "rgf_z_residual_norm", "rgf_z_residual_norm_normalized", "rgf_z_last_error",
 ]
@@ -128,6 +129,11 @@ def load():
         lib.rgf_autotune_clear.argtypes = []
         lib.rgf_autotune_save.argtypes = [C.c_char_p]
         lib.rgf_autotune_load.argtypes = [C.c_char_p]
+        lib.rgf_group_create.argtypes = [C.POINTER(H), C.c_size_t, C.POINTER(H)]
+        lib.rgf_group_destroy.argtypes = [H]
+        lib.rgf_group_factorize.argtypes = [H]
+        lib.rgf_group_selinv_diag_device.argtypes = [H, C.POINTER(C.c_void_p)]
+        lib.rgf_group_synchronize.argtypes = [H]
         lib.rgf_debug_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int,
                                        dp, C.c_int, C.c_double, dp, C.c_int, C.c_int]
         lib.rgf_debug_gemm.restype = C.c_int

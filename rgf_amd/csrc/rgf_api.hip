@@ -661,6 +661,7 @@ int run_piece(rgf_handle_s* h, int kind, int mb, int nr, const void* x, const vo
     return run_captured(h, v[i].gx, v[i].epoch, std::forward<Fn>(fn));
 }
 
+int check_factor_info(rgf_handle_s* h);
 int do_factorize(rgf_handle_s* h)
 {
     if (!h->Ap && h->factorized) return RGF_OK;   // in place: A is gone, the factor is current
@@ -670,6 +671,13 @@ int do_factorize(rgf_handle_s* h)
     if (h->nd > 0 && h->nt > 1 && (rc = ensure_solve_buffers(h))) return rc;   // arrow products use `work`
     rc = run_captured(h, h->gx_factor, h->gx_epoch_factor, [h] { return factor_sweep(h); });
     if (rc) return dev_status(h, rc, "factorize");
+    return check_factor_info(h);
+}
+
+// After a factor sweep on h->stream: the first non-positive pivot of each matrix (waits
+// for the sweep); marks the handle factorized or failed.
+int check_factor_info(rgf_handle_s* h)
+{
     std::vector<int> info(h->count);
     hipError_t e = hipMemcpyAsync(info.data(), h->dinfo, sizeof(int) * h->count, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1931,4 +1939,185 @@ extern "C" int rgf_debug_bench_potrf(int n, int reps, double* ms_potrf, double* 
     if (ms_potrf) *ms_potrf = tot / reps;
     if (ms_tile) *ms_tile = mst / (reps * 8);
     return RGF_OK;
+}
+
+// ------------------------------------------------------------------ handle groups
+// Handles of any shapes, batch counts and precisions on one device whose sweeps run as one
+// HIP graph: each handle's sweep is a branch on its own stream, forked from and joined to
+// the group's stream, so the sweeps of uneven matrices overlap on the device and the host
+// issues one launch per group.  Captured on the first call (and again when the kernel-timing
+// configuration or the selected inversion's outputs change), replayed afterwards: the
+// handles' device buffers never move.
+struct rgf_group_s {
+    std::vector<rgf_handle_s*> hs;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, after = nullptr;
+    std::vector<hipEvent_t> before, joins;   // one per handle
+    hipGraphExec_t gx_factor = nullptr, gx_selinv = nullptr;
+    long long ep_factor = -1, ep_selinv = -1;
+    std::vector<double*> sel_out;            // outputs the captured selected inversion writes
+};
+
+namespace {
+// a failed group call: its message (set on the handle concerned) also becomes rgf_last_error(NULL)
+int group_fail(rgf_handle_s* h, int rc)
+{
+    if (rc != RGF_OK && h) g_create_error = h->err;
+    return rc;
+}
+
+template <class Fn>
+int group_run(rgf_group_s* g, hipGraphExec_t& gx, long long& epoch, Fn&& branch)
+{
+    const long long ep = timing_epoch();
+    if (gx && epoch != ep) {
+        hipGraphExecDestroy(gx);
+        gx = nullptr;
+    }
+    const size_t n = g->hs.size();
+    hipError_t e = hipSuccess;
+    // the group starts after the work already queued on every handle's stream
+    for (size_t q = 0; q < n && e == hipSuccess; q++) {
+        e = hipEventRecord(g->before[q], g->hs[q]->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, g->before[q], 0);
+    }
+    if (e != hipSuccess) return (int)e;
+    if (!gx) {
+        hipGraph_t graph = nullptr;
+        if ((e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return (int)e;
+        int rc = (int)hipEventRecord(g->fork, g->stream);
+        for (size_t q = 0; q < n && rc == 0; q++) {
+            rgf_handle_s* h = g->hs[q];
+            rc = (int)hipStreamWaitEvent(h->stream, g->fork, 0);
+            if (rc == 0) rc = branch(h, q);
+            // every stream that joined the capture is joined back, also after a failure
+            const hipError_t j = hipEventRecord(g->joins[q], h->stream);
+            const hipError_t w = j == hipSuccess ? hipStreamWaitEvent(g->stream, g->joins[q], 0) : j;
+            if (rc == 0) rc = (int)w;
+        }
+        e = hipStreamEndCapture(g->stream, &graph);
+        if (rc || e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return rc ? rc : (int)e;
+        }
+        e = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            gx = nullptr;
+            return (int)e;
+        }
+        epoch = ep;
+    }
+    if ((e = hipGraphLaunch(gx, g->stream)) != hipSuccess) return (int)e;
+    // every handle's stream continues after the group
+    if ((e = hipEventRecord(g->after, g->stream)) != hipSuccess) return (int)e;
+    for (size_t q = 0; q < n && e == hipSuccess; q++) e = hipStreamWaitEvent(g->hs[q]->stream, g->after, 0);
+    return (int)e;
+}
+}  // namespace
+
+extern "C" int rgf_group_create(const rgf_handle_t* handles, size_t n, rgf_group_t* out)
+{
+    if (!out) return set_err(nullptr, RGF_EINVAL, "group: null output");
+    *out = nullptr;
+    if (!handles || n == 0) return set_err(nullptr, RGF_EINVAL, "group: no handles");
+    std::vector<rgf_handle_s*> hs(handles, handles + n);
+    for (size_t q = 0; q < n; q++) {
+        if (!hs[q]) return set_err(nullptr, RGF_EINVAL, "group: handle %zu is null", q);
+        if (hs[q]->device != hs[0]->device) return set_err(nullptr, RGF_EINVAL, "group: handles on different devices");
+        for (size_t p = 0; p < q; p++)
+            if (hs[p] == hs[q]) return set_err(nullptr, RGF_EINVAL, "group: handle %zu given twice", q);
+    }
+    auto* g = new rgf_group_s();
+    g->hs = hs;
+    g->device = hs[0]->device;
+    g->before.assign(n, nullptr);
+    g->joins.assign(n, nullptr);
+    hipSetDevice(g->device);
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->after, hipEventDisableTiming);
+    for (size_t q = 0; q < n && e == hipSuccess; q++) {
+        e = hipEventCreateWithFlags(&g->before[q], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->joins[q], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        rgf_group_destroy(g);
+        return set_err(nullptr, RGF_EDEVICE, "group: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return RGF_OK;
+}
+
+extern "C" int rgf_group_destroy(rgf_group_t g)
+{
+    if (!g) return RGF_OK;
+    hipSetDevice(g->device);
+    if (g->stream) hipStreamSynchronize(g->stream);
+    if (g->gx_factor) hipGraphExecDestroy(g->gx_factor);
+    if (g->gx_selinv) hipGraphExecDestroy(g->gx_selinv);
+    for (hipEvent_t ev : g->before)
+        if (ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : g->joins)
+        if (ev) hipEventDestroy(ev);
+    if (g->fork) hipEventDestroy(g->fork);
+    if (g->after) hipEventDestroy(g->after);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return RGF_OK;
+}
+
+extern "C" int rgf_group_factorize(rgf_group_t g)
+{
+    if (!g) return set_err(nullptr, RGF_EINVAL, "group: null group");
+    hipSetDevice(g->device);
+    for (rgf_handle_s* h : g->hs) {
+        if (h->broken)
+            return group_fail(h, set_err(h, RGF_ESTATE, "factorization failed earlier and the matrix was not kept"));
+        if (!h->Ap && h->factorized)
+            return group_fail(h, set_err(h, RGF_ESTATE,
+                                         "factorized in place already: regenerate the matrix before a group factorize"));
+        int rc;
+        if (h->nd > 0 && h->nt > 1 && (rc = ensure_solve_buffers(h))) return group_fail(h, rc);   // arrow products use `work`
+    }
+    int rc = group_run(g, g->gx_factor, g->ep_factor, [](rgf_handle_s* h, size_t) { return factor_sweep(h); });
+    if (rc) return group_fail(g->hs[0], dev_status(g->hs[0], rc, "group factorize"));
+    int first = RGF_OK;
+    for (rgf_handle_s* h : g->hs) {
+        const int r = check_factor_info(h);
+        if (r != RGF_OK && first == RGF_OK) first = group_fail(h, r);
+    }
+    return first;
+}
+
+extern "C" int rgf_group_selinv_diag_device(rgf_group_t g, double* const* diag_devs)
+{
+    if (!g || !diag_devs) return set_err(nullptr, RGF_EINVAL, "group: null argument");
+    hipSetDevice(g->device);
+    for (size_t q = 0; q < g->hs.size(); q++) {
+        rgf_handle_s* h = g->hs[q];
+        if (!diag_devs[q]) return set_err(nullptr, RGF_EINVAL, "group: null output for handle %zu", q);
+        int rc = ensure_factorized(h);
+        if (!rc) rc = ensure_selinv_buffers(h);
+        if (rc) return group_fail(h, rc);
+    }
+    std::vector<double*> out(diag_devs, diag_devs + g->hs.size());
+    if (g->gx_selinv && out != g->sel_out) {
+        hipGraphExecDestroy(g->gx_selinv);
+        g->gx_selinv = nullptr;
+    }
+    g->sel_out = out;
+    int rc = group_run(g, g->gx_selinv, g->ep_selinv,
+                       [&out](rgf_handle_s* h, size_t q) { return selinv_sweep(h, out[q]); });
+    if (rc) return group_fail(g->hs[0], dev_status(g->hs[0], rc, "group selinv"));
+    return RGF_OK;
+}
+
+extern "C" int rgf_group_synchronize(rgf_group_t g)
+{
+    if (!g) return set_err(nullptr, RGF_EINVAL, "group: null group");
+    hipSetDevice(g->device);
+    const hipError_t e = hipStreamSynchronize(g->stream);
+    return e == hipSuccess ? RGF_OK : group_fail(g->hs[0], dev_status(g->hs[0], (int)e, "group synchronize"));
 }

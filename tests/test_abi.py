@@ -141,3 +141,15 @@ def test_autotune_cache_without_device(tmp_path):
     finally:
         rgf_amd.autotune_clear()
         rgf_amd.set_autotune(prev)
+
+
+def test_group_rejects_bad_arguments_without_device():
+    lib = _lib.load()
+    import ctypes as C
+    g = C.c_void_p()
+    assert lib.rgf_group_create(None, 0, C.byref(g)) == rgf_amd.RGF_EINVAL
+    arr = (C.c_void_p * 2)(None, None)
+    assert lib.rgf_group_create(arr, 2, C.byref(g)) == rgf_amd.RGF_EINVAL
+    assert b"null" in lib.rgf_last_error(None)
+    assert lib.rgf_group_factorize(None) == rgf_amd.RGF_EINVAL
+    assert lib.rgf_group_destroy(None) == rgf_amd.RGF_OK

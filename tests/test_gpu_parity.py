@@ -707,6 +707,53 @@ def test_autotune_keeps_bits(gpu, prec, tmp_path):
         assert np.array_equal(got[1], base[1])
 
 
+def test_group_uneven_shapes_match_alone(gpu):
+    """Handles of uneven shapes, batch counts and precisions factorized and selected-inverted
+    as one group graph (captured, then replayed after new matrix values) give the bits of
+    the same handles run alone; an in-place handle must hold a matrix again first."""
+    say("group of uneven handles vs alone")
+    shapes = [(300, 4, 5, 2, rgf_amd.RGF_FP64, 1), (200, 6, 0, 1, rgf_amd.RGF_MIXED_BF16X3, 1),
+              (130, 3, 7, 3, rgf_amd.RGF_FP64, 0), (64, 5, 3, 1, rgf_amd.RGF_MIXED_BF16X3, 1)]
+
+    def make(q, seed):
+        ns, nt, nd, cnt, prec, keep = shapes[q]
+        return rgf_amd.RGF.generated(ns, nt, nd, seed=seed + q, count=cnt, theta_mode=1,
+                                     options=rgf_amd.default_options(precision=prec, keep_matrix=keep))
+
+    def alone(seed):
+        out = []
+        for q in range(len(shapes)):
+            with make(q, seed) as h:
+                out.append((h.logdet_batch(), h.selinv_diag_batch()))
+        return out
+
+    hs = [make(q, 40) for q in range(len(shapes))]
+    diags = [rgf_amd.DeviceBuffer(h.n * h.count) for h in hs]
+    ref = alone(40)
+    try:
+        with rgf_amd.Group(hs) as g:
+            for rnd in range(2):
+                if rnd == 1:   # the captured graphs replayed
+                    with pytest.raises(rgf_amd.RGFError) as e:
+                        g.factorize()          # the in-place handle holds its factor
+                    assert e.value.code == rgf_amd.RGF_ESTATE
+                    for h in hs:
+                        h.regenerate()         # the in-place handle's factor is its matrix again
+                    for d in diags:
+                        d.upload(np.full(d.count, np.nan))
+                g.factorize()
+                g.selinv_diag_device(diags)
+                g.synchronize()
+                for q, h in enumerate(hs):
+                    assert np.array_equal(h.logdet_batch(), ref[q][0]), (rnd, q)
+                    assert np.array_equal(diags[q].download(), ref[q][1].reshape(-1)), (rnd, q)
+    finally:
+        for d in diags:
+            d.free()
+        for h in hs:
+            h.close()
+
+
 @pytest.mark.parametrize("ns,nt,nd", [(200, 6, 6), (65, 4, 5), (300, 3, 0)])
 def test_mixed_precision_vs_fp64(gpu, ns, nt, nd):
     """Mixed path (split-BF16 Schur / solve / Takahashi products, FP64 potrf, logdet and
