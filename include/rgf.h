@@ -42,6 +42,14 @@ typedef enum {
     RGF_SELINV_REFERENCE = 1  /* RGF.H:476-503 as written: drops the arrow cross terms */
 } rgf_selinv_mode;
 
+typedef enum {
+    RGF_CHOLESKY_AUTO = 0,        /* task graph for handles of at most 4 matrices, tile launches beyond */
+    RGF_CHOLESKY_TASK_GRAPH = 1,  /* each diagonal block's Cholesky and L_ii^-1 as one launch of a tile
+                                     task graph (latency: the pivot chain overlaps the other tiles' work) */
+    RGF_CHOLESKY_TILE_LAUNCHES = 2 /* a launch per 64-wide column block plus GEMM launches for the
+                                     trailing updates and L_ii^-1 (throughput over many matrices) */
+} rgf_cholesky_schedule;
+
 typedef struct {
     int precision;      /* rgf_precision */
     int selinv_mode;    /* rgf_selinv_mode */
@@ -55,6 +63,9 @@ typedef struct {
                            [B_i; D_i] that the factorization's products write, for the selected
                            inversion's W = [B_i; D_i] X_i (~4 bytes per panel entry, about a
                            quarter more device memory at ns = 2865); 0 splits them again there */
+    int cholesky;       /* rgf_cholesky_schedule.  The two schedules sum the same terms in different
+                           orders: results agree to rounding, and bitwise between handles that run
+                           the same schedule (a matrix of a batch and the same matrix alone) */
 } rgf_options;
 
 typedef struct {

@@ -35,6 +35,11 @@ int rgf_debug_cache_stats(long long *hits, long long *mallocs, long long *frees,
  * Cholesky of an n x n block and of its 64 x 64 tile kernel alone. */
 int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int flags, int reps, double *ms_per_launch);
 int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
+/* One task-graph Cholesky of the bench_potrf block with its tasks traced: up to cap records
+ * of 8 values (workgroup, kind 0 chain step / 1 TRSM / 2 UPDATE / 3 the chain's next-tile
+ * update, i, j, generation, wall-clock stamps at claim, dependencies met, done) into out;
+ * *nrec receives the number of tasks traced and *ticks_per_us the clock rate. */
+int rgf_debug_trace_potrf(int n, long long *out, long long cap, long long *nrec, double *ticks_per_us);
 
 /* Process-wide kernel variant switches, for measurements of whole sweeps: key 1 = k per
  * LDS stage of the pre-split split-BF16 product (32 or 64); key 3 = its LDS row layout

@@ -14,13 +14,14 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import (RGF_EDEVICE, RGF_EINVAL, RGF_ENOMEM, RGF_ENOTSPD, RGF_ESTATE, RGF_FP64,
-                   RGF_MIXED_BF16X3, RGF_OK, RGF_SELINV_EXACT, RGF_SELINV_REFERENCE,
-                   DeviceBuffer, GenParams, Info, Options, gpu_count)
+from ._lib import (RGF_CHOLESKY_AUTO, RGF_CHOLESKY_TASK_GRAPH, RGF_CHOLESKY_TILE_LAUNCHES, RGF_EDEVICE,
+                   RGF_EINVAL, RGF_ENOMEM, RGF_ENOTSPD, RGF_ESTATE, RGF_FP64, RGF_MIXED_BF16X3, RGF_OK,
+                   RGF_SELINV_EXACT, RGF_SELINV_REFERENCE, DeviceBuffer, GenParams, Info, Options, gpu_count)
 
 __all__ = ["RGF", "ZRGF", "Group", "RGFError", "Options", "GenParams", "Info", "DeviceBuffer", "default_options",
            "default_gen_params", "gpu_count", "RGF_FP64", "RGF_MIXED_BF16X3",
-           "RGF_SELINV_EXACT", "RGF_SELINV_REFERENCE", "RGF_OK", "RGF_EINVAL", "RGF_ENOTSPD",
+           "RGF_SELINV_EXACT", "RGF_SELINV_REFERENCE", "RGF_CHOLESKY_AUTO", "RGF_CHOLESKY_TASK_GRAPH",
+           "RGF_CHOLESKY_TILE_LAUNCHES", "RGF_OK", "RGF_EINVAL", "RGF_ENOTSPD",
            "RGF_ENOMEM", "RGF_EDEVICE", "RGF_ESTATE", "mf_size", "canonical_flops"]
 
 

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "librgf_hip.so")
 RGF_OK, RGF_EINVAL, RGF_ENOTSPD, RGF_ENOMEM, RGF_EDEVICE, RGF_ESTATE = range(6)
 RGF_FP64, RGF_MIXED_BF16X3 = 0, 2
 RGF_SELINV_EXACT, RGF_SELINV_REFERENCE = range(2)
+RGF_CHOLESKY_AUTO, RGF_CHOLESKY_TASK_GRAPH, RGF_CHOLESKY_TILE_LAUNCHES = range(3)
 
 STATUS_NAMES = {0: "RGF_OK", 1: "RGF_EINVAL", 2: "RGF_ENOTSPD", 3: "RGF_ENOMEM", 4: "RGF_EDEVICE",
                 5: "RGF_ESTATE"}
@@ -36,13 +37,14 @@ EXPORTS = [
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
+                "rgf_debug_trace_potrf",
                 "rgf_debug_cache_stats", "rgf_debug_set_tuning"]
 
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("selinv_mode", C.c_int), ("keep_matrix", C.c_int),
                 ("device", C.c_int), ("refine_iters", C.c_int), ("refine_tol", C.c_double),
-                ("graphs", C.c_int), ("keep_splits", C.c_int)]
+                ("graphs", C.c_int), ("keep_splits", C.c_int), ("cholesky", C.c_int)]
 
 
 class GenParams(C.Structure):
@@ -144,6 +146,8 @@ def load():
         lib.rgf_debug_bench_potrf.argtypes = [C.c_int, C.c_int, dp, dp]
         lib.rgf_debug_bench_potrf.restype = C.c_int
         lp = C.POINTER(C.c_longlong)
+        lib.rgf_debug_trace_potrf.argtypes = [C.c_int, lp, C.c_longlong, lp, dp]
+        lib.rgf_debug_trace_potrf.restype = C.c_int
         lib.rgf_debug_cache_stats.argtypes = [lp, lp, lp, dp]
         lib.rgf_debug_cache_stats.restype = C.c_int
         lib.rgf_debug_set_tuning.argtypes = [C.c_int, C.c_int]

@@ -88,6 +88,8 @@ struct rgf_handle_s {
     double* parts = nullptr;    // per-tile log-diagonal sums: nt*T + 1
     double* cpart = nullptr;    // corner partials nt * nd * nd
     int* dinfo = nullptr;       // first non-positive pivot per matrix (INT_MAX = none)
+    int* cflags = nullptr;      // task-graph Cholesky: dependency flags per matrix (zero between launches)
+    double* xbuf = nullptr;     // task-graph Cholesky: off-diagonal tiles of L_ii^-1 as they form
     // selected inversion
     double* Sb[2] = {nullptr, nullptr};
     double* W = nullptr;
@@ -264,7 +266,7 @@ size_t solve_work_elems(int ns, int nt, int nd, int count)
     return std::max(skinny_work_elems(mx, mx, 16), skinny_batch_work_elems(Op::N, ns, ns, count));
 }
 
-size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision, int keep_splits)
+size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision, int keep_splits, int cholesky)
 {
     const size_t ldp = (size_t)r8(2LL * ns + nd), ldf = (size_t)r8((long long)ns + nd);
     const size_t T = (size_t)(ns + 63) / 64;
@@ -279,7 +281,8 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
     // solve buffers (Y, X, Bv, R, Dc: n x 16 each) and the skinny products' partials
     size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
-    size_t bytes = (per * count + once) * sizeof(double);
+    size_t bytes = (per * count + once) * sizeof(double) + potrf_dag_flag_elems(ns) * count * sizeof(int);
+    if (potrf_dag_schedule(cholesky, count)) bytes += potrf_dag_xbuf_elems(ns) * count * sizeof(double);
     if (precision == RGF_MIXED_BF16X3)
         bytes += (split_scratch_need(ns, nd, count) + split_x_need(ns, nd, count) +
                   lsplit_need(ns, nt, nd, count, keep_splits) + xcopy_need(ns, nt, count, keep_splits)) * 2;
@@ -319,6 +322,10 @@ int alloc_core(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->parts, (size_t)h->partstride * B))) return rc;
     if ((rc = dalloc(h, &h->cpart, (size_t)std::max(nt * nd * nd, 1) * B))) return rc;
     if ((rc = dalloc(h, &h->dinfo, (size_t)B))) return rc;
+    if ((rc = dalloc(h, &h->cflags, potrf_dag_flag_elems(ns) * B))) return rc;
+    if ((rc = (int)hipMemsetAsync(h->cflags, 0, potrf_dag_flag_elems(ns) * B * sizeof(int), h->stream)))
+        return dev_status(h, rc, "create");
+    if (potrf_dag_schedule(h->opt.cholesky, B) && (rc = dalloc(h, &h->xbuf, potrf_dag_xbuf_elems(ns) * B))) return rc;
     if ((rc = dalloc(h, &h->scal, 64))) return rc;
     if (h->opt.precision == RGF_MIXED_BF16X3) {
         h->split_elems = split_scratch_need(ns, nd, B);
@@ -395,6 +402,12 @@ rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, size_t count, const rg
     h->device = h->opt.device;
     if (h->opt.precision != RGF_FP64 && h->opt.precision != RGF_MIXED_BF16X3) {
         *rc = set_err(nullptr, RGF_EINVAL, "precision mode %d is not available in this build", h->opt.precision);
+        delete h;
+        return nullptr;
+    }
+    if (h->opt.cholesky < 0 || h->opt.cholesky > 2) {
+        *rc = set_err(nullptr, RGF_EINVAL, "cholesky schedule %d is not one of 0 (auto), 1 (task graph), 2 (tile launches)",
+                      h->opt.cholesky);
         delete h;
         return nullptr;
     }
@@ -557,17 +570,23 @@ int factor_sweep(rgf_handle_s* h)
                 return rc;
             }
         }
+        // the task-graph Cholesky forms X_i = L_ii^-1 in the same launch (x_done)
+        bool x_done = false;
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
-                            h->xdiag, h->xstride, false, s)))
+                            h->xdiag, h->xstride, false, s, h->cflags, h->xbuf, (long long)potrf_dag_xbuf_elems(ns),
+                            &x_done, h->opt.cholesky)))
             return rc;
         // mixed path: the inverse keeps split copies of X_i's rows (the product below) and
         // columns (the selected inversion's)
         TrtriCopies xcp{xr_view(h), h->xc ? xc_view(h, i) : SplitView{}, h->split, h->split_elems};
         const bool xcopies = h->xr && g_trtri_mixed;
-        if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s,
-                                    h->opt.precision == RGF_MIXED_BF16X3 && g_trtri_mixed ? &pf : nullptr, 0,
-                                    xcopies ? &xcp : nullptr)))
+        if (x_done) {
+            if (xcopies && (rc = xsplit_copies(P, ldp, ms, ns, B, xcp.xr, xcp.xc, s))) return rc;
+        } else if ((rc = trtri_f64_inplace(P, ldp, ms, ns, B, h->ttmp, h->tstride, h->xdiag, h->xstride, s,
+                                           h->opt.precision == RGF_MIXED_BF16X3 && g_trtri_mixed ? &pf : nullptr, 0,
+                                           xcopies ? &xcp : nullptr))) {
             return rc;
+        }
         const bool has_next = i < nt - 1;
         double* const Fc = fp ? P + ns : h->Ftmp;   // coupling rows of F
         if (has_next && Fc != A + ns && (rc = copy2d(A + ns, ldp, Fc, ldF, ns, ns, 1.0, s, B, ms, sF))) return rc;
@@ -683,7 +702,12 @@ int check_factor_info(rgf_handle_s* h)
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return dev_status(h, (int)e, "factorize");
     for (int b = 0; b < h->count; b++)
-        if (info[b] != INT_MAX) {
+        if (info[b] == INT_MIN) {   // a wait of the task-graph Cholesky expired
+            h->factorized = false;
+            if (!h->Ap) h->broken = true;
+            return set_err(h, RGF_EDEVICE, "factorize: a dependency wait of the Cholesky launch expired (matrix %d)",
+                           b);
+        } else if (info[b] != INT_MAX) {
             h->factorized = false;
             if (!h->Ap) h->broken = true;
             if (h->count == 1)
@@ -1340,6 +1364,7 @@ void rgf_default_options(rgf_options* o)
     o->refine_tol = 1e-14;
     o->graphs = 0;
     o->keep_splits = 1;
+    o->cholesky = RGF_CHOLESKY_AUTO;
 }
 
 void rgf_default_gen_params(rgf_gen_params* p)
@@ -1437,7 +1462,8 @@ size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, const 
     rgf_options o;
     rgf_default_options(&o);
     if (opts) o = *opts;
-    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, o.keep_matrix, o.precision, o.keep_splits);
+    return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, o.keep_matrix, o.precision, o.keep_splits,
+                           o.cholesky);
 }
 
 int rgf_destroy(rgf_handle_t h)
@@ -1736,8 +1762,11 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
         size_t G = (size_t)((T - sz + 2 * sz - 1) / (2 * sz));
         tt = std::max(tt, G * (size_t)(sz * 64) * (size_t)(sz * 64));
     }
-    DevBuf a, parts, info, tmp, xd;
+    DevBuf a, parts, info, tmp, xd, fl, xb;
     hipError_t e = hipMalloc(&a.p, (size_t)lda * n * 8);
+    if (e == hipSuccess) e = hipMalloc(&xb.p, potrf_dag_xbuf_elems(n) * 8);
+    if (e == hipSuccess) e = hipMalloc(&fl.p, potrf_dag_flag_elems(n) * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(fl.p, 0, potrf_dag_flag_elems(n) * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&xd.p, (size_t)T * 64 * 64 * 8);
     if (e == hipSuccess) e = hipMalloc(&parts.p, (size_t)T * 8);
     if (e == hipSuccess) e = hipMalloc(&info.p, sizeof(int));
@@ -1746,8 +1775,12 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
     int init = INT_MAX;
     if (e == hipSuccess) e = hipMemcpy(info.p, &init, sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: %s", hipGetErrorString(e));
-    int rc = potrf_f64((double*)a.p, lda, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, true, nullptr);
-    if (!rc && invert) rc = trtri_f64_inplace((double*)a.p, lda, 0, n, 1, (double*)tmp.p, 0, (const double*)xd.p, 0, nullptr);
+    // L in place (diagonal tiles included), or L^-1: the inverse takes the diagonal tiles from
+    // xdiag, so that run factors as the sweeps do (task-graph launch)
+    bool x_done = false;
+    int rc = potrf_f64((double*)a.p, lda, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, !invert,
+                       nullptr, (int*)fl.p, (double*)xb.p, 0, &x_done);
+    if (!rc && invert && !x_done) rc = trtri_f64_inplace((double*)a.p, lda, 0, n, 1, (double*)tmp.p, 0, (const double*)xd.p, 0, nullptr);
     if (rc) return dev_status(nullptr, rc, "debug_potrf");
     std::vector<double> hp(T);
     int hinfo = 0;
@@ -1759,6 +1792,7 @@ extern "C" int rgf_debug_potrf(int n, double* A, int lda, int invert, double* lo
     double sum = 0.0;
     for (double v : hp) sum += v;
     if (logdet) *logdet = 2.0 * sum;
+    if (hinfo == INT_MIN) return set_err(nullptr, RGF_EDEVICE, "debug_potrf: a dependency wait expired");
     if (hinfo != INT_MAX) return set_err(nullptr, RGF_ENOTSPD, "non-positive pivot at column %d", hinfo);
     return RGF_OK;
 }
@@ -1868,6 +1902,21 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         g_wide_min_wgs = value;
         return old;
     }
+    if (key == 16 && (value == 0 || value == 1)) {
+        const int old = g_chol_dag;
+        g_chol_dag = value;
+        return old;
+    }
+    if (key == 17 && value >= 2) {
+        const int old = g_chol_dag_wgs;
+        g_chol_dag_wgs = value;
+        return old;
+    }
+    if (key == 18 && value >= 2) {
+        const int old = g_chol_dag_min;
+        g_chol_dag_min = value;
+        return old;
+    }
     if (key == 9 && (value == 0 || value == 1)) {
         const int old = g_rr_bands;
         g_rr_bands = value;
@@ -1898,9 +1947,13 @@ extern "C" int rgf_debug_bench_potrf(int n, int reps, double* ms_potrf, double* 
 {
     if (n <= 0 || reps <= 0) return RGF_EINVAL;
     const int T = (n + 63) / 64;
-    DevBuf a, a0, parts, info, xd;
+    DevBuf a, a0, parts, info, xd, fl, xb, tmp;
     const size_t na = (size_t)n * n;
     hipError_t e = hipMalloc(&a.p, na * 8);
+    if (e == hipSuccess) e = hipMalloc(&xb.p, potrf_dag_xbuf_elems(n) * 8);
+    if (e == hipSuccess) e = hipMalloc(&tmp.p, na * 8);
+    if (e == hipSuccess) e = hipMalloc(&fl.p, potrf_dag_flag_elems(n) * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(fl.p, 0, potrf_dag_flag_elems(n) * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&a0.p, na * 8);
     if (e == hipSuccess) e = hipMalloc(&parts.p, (size_t)T * 8);
     if (e == hipSuccess) e = hipMalloc(&info.p, sizeof(int));
@@ -1917,7 +1970,10 @@ extern "C" int rgf_debug_bench_potrf(int n, int reps, double* ms_potrf, double* 
     for (int r = 0; r < reps && !rc; r++) {
         rc = copy2d((const double*)a0.p, n, (double*)a.p, n, n, n, 1.0, nullptr);
         hipEventRecord(e0, nullptr);
-        if (!rc) rc = potrf_f64((double*)a.p, n, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, false, nullptr);
+        bool x_done = false;   // Cholesky and L^-1, as a factor sweep forms them
+        if (!rc) rc = potrf_f64((double*)a.p, n, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, false, nullptr,
+                                (int*)fl.p, (double*)xb.p, 0, &x_done);
+        if (!rc && !x_done) rc = trtri_f64_inplace((double*)a.p, n, 0, n, 1, (double*)tmp.p, 0, (const double*)xd.p, 0, nullptr);
         hipEventRecord(e1, nullptr);
         hipEventSynchronize(e1);
         float ms = 0.f;
@@ -1938,6 +1994,49 @@ extern "C" int rgf_debug_bench_potrf(int n, int reps, double* ms_potrf, double* 
     if (rc) return dev_status(nullptr, rc, "bench_potrf");
     if (ms_potrf) *ms_potrf = tot / reps;
     if (ms_tile) *ms_tile = mst / (reps * 8);
+    return RGF_OK;
+}
+
+extern "C" int rgf_debug_trace_potrf(int n, long long* out, long long cap, long long* nrec, double* ticks_per_us)
+{
+    if (n <= 0 || cap < 0 || !out || !nrec) return RGF_EINVAL;
+    const int T = (n + 63) / 64;
+    DevBuf a, parts, info, xd, fl, tr, xb;
+    const size_t na = (size_t)n * n;
+    hipError_t e = hipMalloc(&a.p, na * 8);
+    if (e == hipSuccess) e = hipMalloc(&xb.p, potrf_dag_xbuf_elems(n) * 8);
+    if (e == hipSuccess) e = hipMalloc(&fl.p, potrf_dag_flag_elems(n) * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(fl.p, 0, potrf_dag_flag_elems(n) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&parts.p, (size_t)T * 8);
+    if (e == hipSuccess) e = hipMalloc(&info.p, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&xd.p, (size_t)T * 64 * 64 * 8);
+    if (e == hipSuccess) e = hipMalloc(&tr.p, (size_t)(1 + 8 * cap) * 8);
+    if (e == hipSuccess) e = hipMemset(tr.p, 0, 8);
+    if (e != hipSuccess) return set_err(nullptr, RGF_ENOMEM, "trace_potrf: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(bench_fill_kernel, dim3(1024), dim3(256), 0, nullptr, (double*)a.p, (long long)na, 0.001);
+    int rc = mirror_lower((double*)a.p, n, n, nullptr);
+    hipLaunchKernelGGL(diag_shift_kernel, dim3((n + 255) / 256), dim3(256), 0, nullptr, (double*)a.p, n, (double)n);
+    const int old = g_chol_dag;
+    g_chol_dag = 1;
+    g_chol_dag_trace = (long long*)tr.p;
+    g_chol_dag_trace_cap = cap;
+    bool x_done = false;
+    if (!rc) rc = potrf_f64((double*)a.p, n, 0, n, 1, (double*)parts.p, 0, (int*)info.p, 0, (double*)xd.p, 0, false,
+                            nullptr, (int*)fl.p, (double*)xb.p, 0, &x_done);
+    g_chol_dag_trace = nullptr;
+    g_chol_dag_trace_cap = 0;
+    g_chol_dag = old;
+    if (rc) return dev_status(nullptr, rc, "trace_potrf");
+    long long cntr = 0;
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&cntr, tr.p, 8, hipMemcpyDeviceToHost);
+    const long long got = std::min(cntr, cap);
+    if (e == hipSuccess && got > 0) e = hipMemcpy(out, (long long*)tr.p + 1, (size_t)got * 64, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_err(nullptr, RGF_EDEVICE, "trace_potrf: %s", hipGetErrorString(e));
+    *nrec = cntr;
+    int khz = 100000;
+    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0);
+    if (ticks_per_us) *ticks_per_us = khz / 1000.0;
     return RGF_OK;
 }
 

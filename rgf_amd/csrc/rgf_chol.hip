@@ -11,6 +11,7 @@
 //   small   copy / fill / mirror / diagonal gather (d_fill, d_indexed_copy_offset,
 //           CWC_utility.cu:883-901, 964-979).
 #include "rgf_internal.hpp"
+#include <algorithm>
 #include <climits>
 
 namespace rgf {
@@ -428,6 +429,567 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
     }
 }
 
+// ------------------------------------------------------------------ potrf, one launch
+// The blocked Cholesky of a diagonal block as ONE launch over its 64 x 64 tiles (T tile
+// rows), a task graph run by G workgroups per matrix (grid G x count):
+//   workgroup 0, the chain, runs step k = 0 .. T-1 on LDS operands:
+//     L_{k,k-1} = A_{k,k-1} X_{k-1}^T (stored), tile (k,k) -= L_{k,k-1} L_{k,k-1}^T,
+//     factor + inverse + log-diagonal of the tile (X_k stored), then
+//     A_{k+1,k} -= L_{k+1,k-1} L_{k,k-1}^T, kept in LDS for step k + 1;
+//   workgroups 1 .. G-1 claim the other tasks from the matrix's ticket counter in the
+//   order of generations g = 0, 1, ...:
+//     TRSM(i, g), i >= g + 2: A_ig -= L_{i,g-1} L_{g,g-1}^T, then L_ig = A_ig X_g^T;
+//     UPDATE(i, j, g - 1), i >= j >= g + 1 (nearest columns first): A_ij -= L_{i,g-1} L_{j,g-1}^T;
+//     with an X buffer, the inverse X = L^-1 beside the factorization, by columns of tiles:
+//     XACC(i, j, k = g - 2), i > k >= j: Y_ij += L_ik X_kj (Y_ij = L_ij X_jj for k = j), and
+//     for k = i - 1 then X_ij = -X_ii Y_ij (X_jj: the chain's tile inverses).
+// Every tile receives panels 0, 1, ... in increasing order (per-tile counters), so the
+// result does not depend on which workgroup runs which task.  Dependencies are per-matrix
+// flags in global memory, written after an agent-scope release and read before an
+// agent-scope acquire.  Chain step g, then generation g, for g = 0, 1, ..., is a
+// topological order of the tasks and the workers claim in it, so the launch progresses
+// with the chain and any one worker resident.  Every wait is bounded in time: when one
+// expires the matrix's info becomes INT_MIN and its workgroups leave.  The last workgroup
+// of a matrix to leave clears the matrix's flags, which the next launch expects at zero.
+constexpr int FL_TICKET = 0, FL_EXIT = 1, FL_ABORT = 2, FL_HDR = 4;
+
+// flags: header, xdone[T], then T x T arrays ldone, cnt, acnt (terms summed into Y_ij), xrdy (X_ij stored)
+__host__ __device__ inline int dag_flag_elems(int T) { return FL_HDR + T + 4 * T * T; }
+
+__host__ __device__ inline int dag_gen_tasks(int T, int g, bool with_x, int& ntr, int& nup, int& nxa)
+{
+    ntr = T - g - 2 > 0 ? T - g - 2 : 0;
+    nup = g >= 1 && T - g - 1 > 0 ? (T - g - 1) * (T - g) / 2 : 0;
+    const int k = g - 2;
+    nxa = with_x && k >= 0 && k < T - 1 ? (k + 1) * (T - 1 - k) : 0;
+    return ntr + nup + nxa;
+}
+
+__host__ __device__ inline int dag_gens(int T, bool with_x) { return with_x ? T + 1 : T - 1; }
+
+__host__ __device__ inline int dag_tasks(int T, bool with_x)
+{
+    int total = 0, a, b, c;
+    for (int g = 0; g < dag_gens(T, with_x); g++) total += dag_gen_tasks(T, g, with_x, a, b, c);
+    return total;
+}
+
+struct DagArgs {
+    double* A;
+    long long sA;
+    double* xdiag;
+    long long sX;
+    double* parts;
+    long long sP;
+    int* info;
+    int* flags;
+    long long sF;
+    long long timeout;   // wall-clock ticks one wait may take
+    int lda, n, T, ntasks, info_base;
+    long long* trace;    // null, or: [0] record count, then records of 8 (see dag_trace)
+    long long trace_cap;
+    double* xbuf;        // null, or the off-diagonal tiles of X = L^-1 (ld ldx, per matrix sXb)
+    long long sXb;
+    int ldx;
+};
+
+// (thread 0, tracing) one record: workgroup, kind (0 chain step, 1 TRSM, 2 UPDATE, 3 the
+// chain's A_{k+1,k} update), i, j, g and three wall-clock stamps
+__device__ __forceinline__ void dag_trace(const DagArgs& a, int kind, int i, int j, int g, long long t0,
+                                          long long t1, long long t2)
+{
+    if (!a.trace) return;
+    const long long r = atomicAdd((unsigned long long*)a.trace, 1ULL);
+    if (r >= a.trace_cap) return;
+    long long* q = a.trace + 1 + 8 * r;
+    q[0] = blockIdx.x + 65536LL * blockIdx.y;
+    q[1] = kind;
+    q[2] = i;
+    q[3] = j;
+    q[4] = g;
+    q[5] = t0;
+    q[6] = t1;
+    q[7] = t2;
+}
+
+__device__ __forceinline__ int dag_peek(const int* f)
+{
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// (thread 0) until *f >= v; false when the matrix aborted or the deadline passed
+__device__ __forceinline__ bool dag_wait(const int* f, int v, const int* fl, long long deadline)
+{
+    while (dag_peek(f) < v) {
+        if (dag_peek(fl + FL_ABORT) || wall_clock64() > deadline) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// Tiles shared between workgroups are read and written only by agent-scope atomic loads
+// and stores (relaxed): each such access is coherent across the XCDs by itself, so no
+// L2 write-back or invalidation is needed.  A flag is stored after the workgroup's
+// stores have completed (every wave waits for its memory counters, then the barrier),
+// and the tiles it guards are loaded only after it has been seen.
+__device__ __forceinline__ double ldc(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stc(double* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All threads: thread 0 waits for up to three flags (null: none), then the workgroup
+// proceeds.  false: the matrix aborted (a wait expired here or elsewhere).
+__device__ __forceinline__ bool dag_acquire(const DagArgs& a, int* fl, int* info, const int* f0, int v0,
+                                            const int* f1, int v1, const int* f2, int v2, int* s_ok, int tid)
+{
+    __syncthreads();
+    if (tid == 0) {
+        const long long dl = wall_clock64() + a.timeout;
+        bool ok = (!f0 || dag_wait(f0, v0, fl, dl)) && (!f1 || dag_wait(f1, v1, fl, dl)) &&
+                  (!f2 || dag_wait(f2, v2, fl, dl));
+        if (!ok) {
+            __hip_atomic_store(fl + FL_ABORT, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicMin(info, INT_MIN);
+        }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok;
+}
+
+// All threads: once every store of the workgroup has completed, set *f = v.
+__device__ __forceinline__ void dag_release(int* f, int v, int tid)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the same with a second flag
+__device__ __forceinline__ void dag_release2(int* f, int v, int* f2, int v2, int tid)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) {
+        __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(f2, v2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// rows x cols of the tile at (r0, c0) into S, zero padded (lanes along rows: coalesced)
+__device__ __forceinline__ void dag_load(double (*S)[LP], const double* A, int lda, int r0, int c0, int rows,
+                                         int cols, int tid)
+{
+    double v[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+        const bool in = r < rows && c < cols;
+        v[q] = ldc(A + (in ? (long long)(c0 + c) * lda + (r0 + r) : (long long)c0 * lda + r0));
+        v[q] = in ? v[q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+        S[r][c] = v[q];
+    }
+}
+
+// the same, transposed: S[c][r] = tile(r, c)
+__device__ __forceinline__ void dag_load_t(double (*S)[LP], const double* A, int lda, int r0, int c0, int rows,
+                                           int cols, int tid)
+{
+    double v[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+        const bool in = r < rows && c < cols;
+        v[q] = ldc(A + (in ? (long long)(c0 + c) * lda + (r0 + r) : (long long)c0 * lda + r0));
+        v[q] = in ? v[q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+        S[c][r] = v[q];
+    }
+}
+
+// the wave's 32 x 32 quarter of a 64 x 64 accumulator (mfma_nt<2> layout) from / to LDS
+__device__ __forceinline__ void acc_from(v4d (&acc)[2][2], const double (*S)[LP], int wm, int wn, int l16, int lq)
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[a][b][t] = S[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
+}
+__device__ __forceinline__ void acc_to(const v4d (&acc)[2][2], double (*S)[LP], int wm, int wn, int l16, int lq)
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) S[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
+}
+__device__ __forceinline__ void acc_to_t(const v4d (&acc)[2][2], double (*S)[LP], int wm, int wn, int l16, int lq)
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) S[32 * wn + 16 * b + 4 * t + lq][32 * wm + 16 * a + l16] = acc[a][b][t];
+}
+// the accumulator from the tile at (r0, c0) (64 x 64 readable)
+__device__ __forceinline__ void acc_load(v4d (&acc)[2][2], const double* A, int lda, int r0, int c0, int wm, int wn,
+                                         int l16, int lq)
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                acc[a][b][t] = ldc(A + (long long)(c0 + 32 * wn + 16 * b + 4 * t + lq) * lda + (r0 + 32 * wm + 16 * a + l16));
+}
+__device__ __forceinline__ void acc_zero(v4d (&acc)[2][2])
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+}
+// rows x cols of the accumulator into the tile at (r0, c0)
+__device__ __forceinline__ void acc_store(const v4d (&acc)[2][2], double* A, int lda, int r0, int c0, int rows,
+                                          int cols, int wm, int wn, int l16, int lq)
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int r = 32 * wm + 16 * a + l16, c = 32 * wn + 16 * b + 4 * t + lq;
+                if (r < rows && c < cols) stc(A + (long long)(c0 + c) * lda + (r0 + r), acc[a][b][t]);
+            }
+}
+
+__global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
+{
+    const long long mb = blockIdx.y;
+    double* const A = a.A + mb * a.sA;
+    double* const xd = a.xdiag + mb * a.sX;
+    double* const parts = a.parts + mb * a.sP;
+    int* const info = a.info + mb;
+    int* const fl = a.flags + mb * a.sF;
+    int* const xdone = fl + FL_HDR;
+    int* const ldone = xdone + a.T;   // ldone[i * T + k]: L_ik stored
+    int* const cnt = ldone + a.T * a.T;   // cnt[i * T + j]: panels applied to tile (i, j)
+    int* const acnt = cnt + a.T * a.T;    // acnt[i * T + j]: terms summed into Y_ij
+    int* const xrdy = acnt + a.T * a.T;   // xrdy[i * T + j]: X_ij stored
+    const int lda = a.lda, n = a.n, T = a.T;
+    const bool with_x = a.xbuf != nullptr;
+    double* const xb = with_x ? a.xbuf + mb * a.sXb : nullptr;
+    const int ldx = a.ldx;
+
+    constexpr int TSZ = 2 * (NB / 2) * (NB / 2 + 2);
+    __shared__ __attribute__((aligned(16))) double pool[3 * NB + 4 * NB * LP + TSZ];
+    __shared__ int s_ok, s_task;
+    double* const colb = pool;
+    double* const rinv = pool + NB;
+    double* const dgl = pool + 2 * NB;
+    double (*const B0)[LP] = reinterpret_cast<double (*)[LP]>(pool + 3 * NB);
+    double (*const B1)[LP] = B0 + NB;
+    double (*const B2)[LP] = B0 + 2 * NB;
+    double (*const B3)[LP] = B0 + 3 * NB;
+    double (*const Ts)[NB / 2][NB / 2 + 2] =
+        reinterpret_cast<double (*)[NB / 2][NB / 2 + 2]>(pool + 3 * NB + 4 * NB * LP);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int wm = wave & 1, wn = wave >> 1;
+    for (int e = tid; e < 4 * NB * LP; e += 256) pool[3 * NB + e] = 0.0;   // X's upper blocks stay zero
+    __syncthreads();
+    v4d acc[2][2];
+
+    if (blockIdx.x == 0) {
+        // ---- the chain
+        double (*Bl)[LP] = B0;   // tile (k, k); later L_{k+1,k-1}
+        double (*Bx)[LP] = B1;   // X_{k-1}, then X_k
+        double (*Ba)[LP] = B2;   // A_{k,k-1} (panels <= k-2 applied), then L_{k,k-1}
+        double (*Bn)[LP] = B3;   // A_{k+1,k} being brought to panel k-1
+        for (int k = 0; k < T; k++) {
+            const int k0 = NB * k, kb = min(NB, n - k0);
+            const long long c0 = wall_clock64();
+            if (!dag_acquire(a, fl, info, k >= 2 ? cnt + k * T + k : nullptr, k - 1, nullptr, 0, nullptr, 0, &s_ok,
+                             tid))
+                break;
+            const long long c1 = wall_clock64();
+            {   // tile (k, k): lower triangle, identity beyond kb
+                double v[NB * NB / 256];
+#pragma unroll
+                for (int q = 0; q < NB * NB / 256; q++) {
+                    const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+                    const bool in = r < kb && c < kb;
+                    v[q] = ldc(A + (in ? (long long)(k0 + c) * lda + (k0 + r) : (long long)k0 * lda + k0));
+                }
+#pragma unroll
+                for (int q = 0; q < NB * NB / 256; q++) {
+                    const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+                    Bl[r][c] = (r < kb && c < kb) ? (c <= r ? v[q] : 0.0) : (r == c ? 1.0 : 0.0);
+                }
+            }
+            __syncthreads();
+            const long long cl = wall_clock64();
+            if (k >= 1) {
+                acc_zero(acc);
+                mfma_nt<2>(acc, Ba, Bx, 1.0, wm, wn, l16, lq);   // L_{k,k-1} = A_{k,k-1} X_{k-1}^T
+                acc_store(acc, A, lda, k0, k0 - NB, kb, NB, wm, wn, l16, lq);
+                __syncthreads();
+                acc_to(acc, Ba, wm, wn, l16, lq);
+                __syncthreads();
+                tile_sub_panel(Bl, Ba, wave, l16, lq);
+            }
+            const long long cp = wall_clock64();
+            int bad = INT_MAX;
+            tile_factor(Bl, colb, rinv, dgl, kb, bad, tid);
+            const long long cf = wall_clock64();
+            tile_logdet(dgl, kb, bad, true, parts + k, info, a.info_base + k0, tid);
+            tile_inverse(Bl, Bx, Ts, rinv, tid);
+            const long long ci = wall_clock64();
+            for (int e = tid; e < NB * NB; e += 256) {
+                const int r = e & (NB - 1), c = e >> 6;
+                stc(xd + (long long)k * NB * NB + c * NB + r, Bx[r][c]);
+            }
+            if (k >= 1)
+                dag_release2(xdone + k, 1, ldone + k * T + (k - 1), 1, tid);
+            else
+                dag_release(xdone + k, 1, tid);
+            if (tid == 0) {
+                const long long ce = wall_clock64();
+                dag_trace(a, 0, k, k, k, c0, c1, ce);
+                dag_trace(a, 4, k, k, k, cl, cp, cf);
+                dag_trace(a, 5, k, k, k, ci, ce, 0);
+            }
+            if (k + 1 < T) {
+                const int kb1 = min(NB, n - k0 - NB);
+                const long long d0 = wall_clock64();
+                if (!dag_acquire(a, fl, info, k >= 1 ? ldone + (k + 1) * T + (k - 1) : nullptr, 1,
+                                 k >= 2 ? cnt + (k + 1) * T + k : nullptr, k - 1, nullptr, 0, &s_ok, tid))
+                    break;
+                const long long d1 = wall_clock64();
+                dag_load(Bn, A, lda, k0 + NB, k0, kb1, NB, tid);
+                if (k >= 1) dag_load(Bl, A, lda, k0 + NB, k0 - NB, kb1, NB, tid);   // L_{k+1,k-1}
+                __syncthreads();
+                if (k >= 1) {
+                    acc_from(acc, Bn, wm, wn, l16, lq);
+                    mfma_nt<2>(acc, Bl, Ba, -1.0, wm, wn, l16, lq);
+                    __syncthreads();
+                    acc_to(acc, Bn, wm, wn, l16, lq);
+                }
+                double (*t)[LP] = Ba;
+                Ba = Bn;
+                Bn = t;
+                if (tid == 0) dag_trace(a, 3, k + 1, k, k, d0, d1, wall_clock64());
+            }
+        }
+    } else {
+        // ---- workers
+        for (;;) {
+            __syncthreads();
+            if (tid == 0)
+                s_task = dag_peek(fl + FL_ABORT)
+                             ? -1
+                             : __hip_atomic_fetch_add(fl + FL_TICKET, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            int t = s_task;
+            if (t < 0 || t >= a.ntasks) break;
+            const long long w0 = wall_clock64();
+            int g = 0, ntr, nup, nxa;
+            while (t >= dag_gen_tasks(T, g, with_x, ntr, nup, nxa)) {
+                t -= ntr + nup + nxa;
+                g++;
+            }
+            if (t >= ntr + nup) {
+                // XACC(i, j, k = g - 2)
+                t -= ntr + nup;
+                const int k = g - 2, i = k + 1 + t / (k + 1), j = t % (k + 1);
+                const int ri = min(NB, n - NB * i);
+                if (!dag_acquire(a, fl, info, ldone + i * T + k, 1, k == j ? xdone + j : xrdy + k * T + j, 1,
+                                 k > j ? acnt + i * T + j : nullptr, k - j, &s_ok, tid))
+                    break;
+                const long long w1 = wall_clock64();
+                dag_load(B0, A, lda, NB * i, NB * k, ri, NB, tid);   // L_ik
+                if (k == j)
+                    dag_load_t(B1, xd + (long long)j * NB * NB, NB, 0, 0, NB, NB, tid);
+                else
+                    dag_load_t(B1, xb, ldx, NB * k, NB * j, NB, NB, tid);
+                if (k == i - 1) dag_load(B2, xd + (long long)i * NB * NB, NB, 0, 0, NB, NB, tid);   // X_ii
+                if (k > j)
+                    acc_load(acc, xb, ldx, NB * i, NB * j, wm, wn, l16, lq);
+                else
+                    acc_zero(acc);
+                __syncthreads();
+                mfma_nt<2>(acc, B0, B1, 1.0, wm, wn, l16, lq);   // Y_ij += L_ik X_kj
+                if (k < i - 1) {
+                    acc_store(acc, xb, ldx, NB * i, NB * j, NB, NB, wm, wn, l16, lq);
+                    dag_release(acnt + i * T + j, k - j + 1, tid);
+                } else {
+                    acc_to_t(acc, B3, wm, wn, l16, lq);
+                    __syncthreads();
+                    acc_zero(acc);
+                    mfma_nt<2>(acc, B2, B3, -1.0, wm, wn, l16, lq);   // X_ij = -X_ii Y_ij
+                    acc_store(acc, xb, ldx, NB * i, NB * j, NB, NB, wm, wn, l16, lq);
+                    dag_release(xrdy + i * T + j, 1, tid);
+                }
+                if (tid == 0) dag_trace(a, 6, i, j, g, w0, w1, wall_clock64());
+            } else if (t < ntr) {
+                // TRSM(i, g)
+                const int i = g + 2 + t, rows = min(NB, n - NB * i);
+                if (!dag_acquire(a, fl, info, xdone + g, 1, g >= 1 ? ldone + i * T + (g - 1) : nullptr, 1,
+                                 g >= 2 ? cnt + i * T + g : nullptr, g - 1, &s_ok, tid))
+                    break;
+                const long long w1 = wall_clock64();
+                dag_load(B3, A, lda, NB * i, NB * g, rows, NB, tid);
+                dag_load(B1, xd + (long long)g * NB * NB, NB, 0, 0, NB, NB, tid);
+                if (g >= 1) {
+                    dag_load(B0, A, lda, NB * i, NB * (g - 1), rows, NB, tid);
+                    dag_load(B2, A, lda, NB * g, NB * (g - 1), NB, NB, tid);
+                }
+                __syncthreads();
+                if (g >= 1) {
+                    acc_from(acc, B3, wm, wn, l16, lq);
+                    mfma_nt<2>(acc, B0, B2, -1.0, wm, wn, l16, lq);
+                    __syncthreads();
+                    acc_to(acc, B3, wm, wn, l16, lq);
+                    __syncthreads();
+                }
+                acc_zero(acc);
+                mfma_nt<2>(acc, B3, B1, 1.0, wm, wn, l16, lq);
+                acc_store(acc, A, lda, NB * i, NB * g, rows, NB, wm, wn, l16, lq);
+                dag_release(ldone + i * T + g, 1, tid);
+                if (tid == 0) dag_trace(a, 1, i, g, g, w0, w1, wall_clock64());
+            } else {
+                // UPDATE(i, j, p = g - 1)
+                t -= ntr;
+                int j = g + 1;
+                while (t >= T - j) {
+                    t -= T - j;
+                    j++;
+                }
+                const int i = j + t, p = g - 1;
+                const int ri = min(NB, n - NB * i), rj = min(NB, n - NB * j);
+                if (!dag_acquire(a, fl, info, ldone + i * T + p, 1, j != i ? ldone + j * T + p : nullptr, 1,
+                                 p >= 1 ? cnt + i * T + j : nullptr, p, &s_ok, tid))
+                    break;
+                const long long w1 = wall_clock64();
+                dag_load(B3, A, lda, NB * i, NB * j, ri, rj, tid);
+                dag_load(B0, A, lda, NB * i, NB * p, ri, NB, tid);
+                if (j != i) dag_load(B2, A, lda, NB * j, NB * p, rj, NB, tid);
+                __syncthreads();
+                acc_from(acc, B3, wm, wn, l16, lq);
+                mfma_nt<2>(acc, B0, j != i ? B2 : B0, -1.0, wm, wn, l16, lq);
+                acc_store(acc, A, lda, NB * i, NB * j, ri, rj, wm, wn, l16, lq);
+                dag_release(cnt + i * T + j, p + 1, tid);
+                if (tid == 0) dag_trace(a, 2, i, j, g, w0, w1, wall_clock64());
+            }
+        }
+    }
+    // leave; the last workgroup of the matrix clears its flags
+    __syncthreads();
+    if (tid == 0) {
+        const int old = __hip_atomic_fetch_add(fl + FL_EXIT, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = old == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_ok) {
+        const int ne = dag_flag_elems(T);
+        for (int e = tid; e < ne; e += 256) __hip_atomic_store(fl + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+int g_chol_dag = 1;         // 1: diagonal-block Cholesky as one task-graph launch (where flags are given)
+int g_chol_dag_wgs = 256;   // its workgroups per launch (all matrices together)
+int g_chol_dag_min = 64;    // ... used only when each matrix gets at least this many
+
+size_t potrf_dag_flag_elems(int n) { return (size_t)dag_flag_elems((n + NB - 1) / NB); }
+size_t potrf_dag_xbuf_elems(int n)
+{
+    const size_t T = (size_t)(n + NB - 1) / NB;
+    return T * NB * NB * std::max<size_t>(T - 1, 1);   // 64T rows x the tile columns below the last
+}
+
+// After the task-graph launch with an X buffer: X = L^-1 into the n x n block at A, tile
+// (ti, tj) of a workgroup: below the diagonal from xbuf, on it the tile inverse, above zeros.
+__global__ __launch_bounds__(256) void dag_finish_kernel(double* A, int lda, long long sA, int n, int T,
+                                                         const double* xdiag, long long sX, const double* xbuf,
+                                                         int ldx, long long sXb)
+{
+    const long long mb = blockIdx.y;
+    A += mb * sA;
+    xdiag += mb * sX;
+    xbuf += mb * sXb;
+    const int ti = blockIdx.x % T, tj = blockIdx.x / T;
+    const int r0 = ti * NB, c0 = tj * NB, rb = min(NB, n - r0), cb = min(NB, n - c0);
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int r = e & (NB - 1), c = e >> 6;
+        if (r >= rb || c >= cb) continue;
+        double v = 0.0;
+        if (ti == tj) v = xdiag[(long long)ti * NB * NB + c * NB + r];
+        else if (ti > tj) v = xbuf[(long long)(c0 + c) * ldx + (r0 + r)];
+        A[(long long)(c0 + c) * lda + (r0 + r)] = v;
+    }
+}
+
+// Split copies of X = L^-1 (the n x n block at A, zeros above the diagonal) for the mixed
+// path's products: xr element (r, k) = X[r][k], xc element (r, k) = X[k][r] (xc.p null:
+// none), k < ldk (zeros beyond n).  grid: (row blocks of 64 x k blocks of 64, count).
+__global__ __launch_bounds__(256) void xsplit_kernel(const double* A, int lda, long long sA, int n, SplitView xr,
+                                                     SplitView xc)
+{
+    const long long e = blockIdx.y;
+    A += e * sA;
+    const int nkb = xr.ldk / NB;
+    const int r0 = (blockIdx.x / nkb) * NB, k0 = (blockIdx.x % nkb) * NB;
+    for (int t = threadIdx.x; t < NB * NB; t += 256) {
+        const int a = t & (NB - 1), b = t >> 6;
+        {   // xr: lanes along r (column reads of A coalesced)
+            const int r = r0 + a, k = k0 + b;
+            if (r < n) split_store_v(xr, e, 0, r, k, (k < n && k <= r) ? A[(long long)k * lda + r] : 0.0);
+        }
+        if (xc.p) {   // xc: lanes along k
+            const int r = r0 + b, k = k0 + a;
+            if (r < n) split_store_v(xc, e, 0, r, k, (k < n && k >= r) ? A[(long long)r * lda + k] : 0.0);
+        }
+    }
+}
+
+int xsplit_copies(const double* A, int lda, long long sA, int n, int count, const SplitView& xr, const SplitView& xc,
+                  hipStream_t s)
+{
+    const int nrb = (n + NB - 1) / NB, nkb = xr.ldk / NB;
+    hipLaunchKernelGGL(xsplit_kernel, dim3(nrb * nkb, count), dim3(256), 0, s, A, lda, sA, n, xr, xc);
+    return (int)hipGetLastError();
+}
+
+static long long dag_timeout_ticks()
+{
+    static long long ticks[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (!ticks[dev]) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+        ticks[dev] = (long long)khz * 1000LL * 10;   // 10 s for one dependency
+    }
+    return ticks[dev];
+}
+
 // Blocked right-looking Cholesky of the n x n block at A (lower), 64-wide column blocks
 // with the trailing matrix updated every second block by a rank-128 GEMM: column block
 // p is one launch of the tile kernel (for odd p first the update of the block by panel
@@ -436,9 +998,37 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(double* A, int lda, lon
 // columns beyond it are updated by panels p-1 and p together (K = 128: the same traffic
 // on the trailing matrix as one rank-64 update, half the launches).  xdiag receives the
 // T inverted diagonal tiles.
-int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
-              int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s)
+long long* g_chol_dag_trace = nullptr;   // tracing of the task-graph launch (tuning)
+long long g_chol_dag_trace_cap = 0;
+
+bool potrf_uses_dag(int count) { return g_chol_dag && g_chol_dag_wgs / std::max(count, 1) >= std::max(2, g_chol_dag_min); }
+
+bool potrf_dag_schedule(int schedule, int count)
 {
+    return schedule == 1 || (schedule == 0 && potrf_uses_dag(count));
+}
+
+int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
+              int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s, int* dag_flags,
+              double* xbuf, long long sXb, bool* x_done, int schedule)
+{
+    if (x_done) *x_done = false;
+    if (dag_flags && !store_diag_l && potrf_dag_schedule(schedule, count)) {
+        const int T = (n + NB - 1) / NB;
+        const bool with_x = xbuf && x_done;
+        DagArgs a{A, sA, xdiag, sX, logdet_parts, sP, info, dag_flags, (long long)dag_flag_elems(T),
+                  dag_timeout_ticks(), lda, n, T, dag_tasks(T, with_x), info_base, g_chol_dag_trace,
+                  g_chol_dag_trace_cap, with_x ? xbuf : nullptr, sXb, T * NB};
+        const int G = std::min(std::max(2, g_chol_dag_wgs / std::max(count, 1)), 1 + a.ntasks);
+        hipLaunchKernelGGL(potrf_dag_kernel, dim3(G, count), dim3(256), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !with_x) return (int)e;
+        hipLaunchKernelGGL(dag_finish_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, xdiag, sX, xbuf,
+                           T * NB, sXb);
+        e = hipGetLastError();
+        if (e == hipSuccess) *x_done = true;
+        return (int)e;
+    }
     int p = 0;
     for (int k0 = 0; k0 < n; k0 += NB, p++) {
         const int kb = min(NB, n - k0);

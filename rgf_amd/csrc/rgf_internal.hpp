@@ -249,8 +249,24 @@ inline void batch2(GemmDesc& d, int count, long long sA, long long sB, long long
 // xdiag: T = ceil(n/64) tiles of 64 x 64 per matrix receiving the inverted diagonal tiles.
 // The diagonal tiles of A receive L only if store_diag_l (the factor sweep replaces
 // them by their inverses from xdiag and skips the store).
+// dag_flags (potrf_dag_flag_elems(n) ints per matrix, zero before the first call; each call
+// leaves them zero): the factorization runs as one task-graph launch per call (unless
+// store_diag_l or the batch is too large for it); a wait of that launch that expires sets
+// the matrix's info to INT_MIN.
+// With xbuf (potrf_dag_xbuf_elems(n) doubles per matrix at stride sXb) and x_done, the
+// task-graph launch also forms X = L^-1 and leaves it in the block (as trtri_f64_inplace
+// would; then *x_done is set).
+// schedule: rgf_cholesky_schedule (0 auto: potrf_uses_dag(count)).
 int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet_parts, long long sP, int* info,
-              int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s);
+              int info_base, double* xdiag, long long sX, bool store_diag_l, hipStream_t s, int* dag_flags = nullptr,
+              double* xbuf = nullptr, long long sXb = 0, bool* x_done = nullptr, int schedule = 0);
+bool potrf_dag_schedule(int schedule, int count);
+size_t potrf_dag_flag_elems(int n);
+size_t potrf_dag_xbuf_elems(int n);
+bool potrf_uses_dag(int count);   // a launch over `count` matrices takes the task-graph path
+extern int g_chol_dag, g_chol_dag_wgs, g_chol_dag_min;
+extern long long* g_chol_dag_trace;   // device buffer: [0] records, then 8 longs per task (tuning)
+extern long long g_chol_dag_trace_cap;
 // A product launcher supplied by the caller (precision policy of a handle).
 struct ProdFn {
     int (*fn)(void* ctx, Op ta, Op tb, GemmDesc* d, int nd, hipStream_t s);
@@ -280,6 +296,10 @@ int trtri_f64_inplace(double* A, int lda, long long sA, int n, int count, double
                       const double* xdiag, long long sX, hipStream_t s, const ProdFn* prod = nullptr,
                       long long plane = 0, const TrtriCopies* cp = nullptr);
 int potrf_tile_only(double* A, int lda, double* xdiag, double* logdet_slot, int* info, hipStream_t s);
+// split copies of X = L^-1 (n x n at A, zeros above the diagonal) as trtri_f64_inplace's cp
+// leaves them: xr (rows) and xc (columns; xc.p null: none)
+int xsplit_copies(const double* A, int lda, long long sA, int n, int count, const SplitView& xr, const SplitView& xc,
+                  hipStream_t s);
 int copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double alpha,
            hipStream_t s, int count = 1, long long sS = 0, long long sD = 0);
 int fill2d(double* dst, int ldd, int rows, int cols, double v, hipStream_t s, int count = 1, long long sD = 0);

@@ -33,6 +33,7 @@ def test_defaults():
     o = rgf_amd.default_options()
     assert (o.precision, o.selinv_mode, o.keep_matrix) == (rgf_amd.RGF_FP64, rgf_amd.RGF_SELINV_EXACT, 1)
     assert (o.refine_iters, o.refine_tol, o.graphs) == (10, 1e-14, 0)
+    assert (o.keep_splits, o.cholesky) == (1, rgf_amd.RGF_CHOLESKY_AUTO)
     g = rgf_amd.default_gen_params()
     assert (g.sigma_c, g.sigma_a, g.theta_d, g.theta_c) == (0.1, 0.05, 1.0, 1.0)
 
@@ -56,6 +57,15 @@ def test_bad_shapes_rejected():
         rgf_amd.RGF(np.zeros(5, np.uint64), np.zeros(0, np.uint64), np.zeros(0), 3, 3, 0)   # wrong ia length
 
 
+def test_bad_cholesky_schedule_rejected_without_device():
+    ia = np.arange(3 + 1, dtype=np.uint64)   # the 3 x 3 identity, ns = 3, nt = 1
+    ja = np.arange(3, dtype=np.uint64)
+    for bad in (-1, 3):
+        with pytest.raises(rgf_amd.RGFError) as e:
+            rgf_amd.RGF(ia, ja, np.ones(3), 3, 1, 0, options=rgf_amd.default_options(cholesky=bad))
+        assert e.value.code == rgf_amd.RGF_EINVAL and "cholesky schedule" in str(e.value)
+
+
 def test_canonical_flops_closed_form():
     F, S = rgf_amd.canonical_flops(2865, 100, 6)
     assert F + S == pytest.approx(1.636e13, rel=2e-3)
@@ -76,6 +86,10 @@ def test_footprint_bytes_host_only():
     assert f - rgf_amd.footprint_bytes(500, 10, 4, 1, inplace) >= 8 * 10 * ldp * 500
     assert rgf_amd.footprint_bytes(500, 10, 4, 1, mixed) > f
     assert rgf_amd.footprint_bytes(500, 10, 4, 3, keep) > 2 * f
+    # the task-graph Cholesky (default for few matrices) holds L_ii^-1 tiles as they form
+    tiles = rgf_amd.default_options(keep_matrix=1, cholesky=rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES)
+    assert f - rgf_amd.footprint_bytes(500, 10, 4, 1, tiles) == 8 * 512 * 448
+    assert rgf_amd.footprint_bytes(500, 10, 4, 8, keep) == rgf_amd.footprint_bytes(500, 10, 4, 8, tiles)
     assert rgf_amd.footprint_bytes(0, 10, 4, 1, keep) == 0
     assert rgf_amd.footprint_bytes(5, 10, 6, 1, keep) == 0     # nd > ns
 
@@ -92,6 +106,10 @@ def test_tuning_switches_validate_without_device():
         for v in bad:
             assert lib.rgf_debug_set_tuning(key, v) == -1
         assert lib.rgf_debug_set_tuning(key, good[1]) == good[0]
+        assert lib.rgf_debug_set_tuning(key, cur) == good[1]
+    for key, good, bad in ((16, (0, 1), (2, -1)),):
+        cur = lib.rgf_debug_set_tuning(key, good[1])
+        assert cur in good and lib.rgf_debug_set_tuning(key, bad[0]) == -1
         assert lib.rgf_debug_set_tuning(key, cur) == good[1]
     for key in (0, 2, 5, 99):
         assert lib.rgf_debug_set_tuning(key, 1) == -1

@@ -2,8 +2,9 @@
 
 c3: 64 matrices of (ns 1500, nt 60, nd 5) with per-instance theta in one batched handle,
 FP64 and mixed: sampled instances bitwise equal to single-matrix handles of the same
-seed and id, one instance against the oracle on the same matrix, every mixed logdet
-within 1e-8 of FP64.
+seed, id and Cholesky schedule (a batch this large takes the tile-launch schedule; a
+single matrix by default the task graph, which agrees to rounding), one instance against
+the oracle on the same matrix, every mixed logdet within 1e-8 of FP64.
 c4: requests drawn from the serving distribution (ns ~ U{300..3000}, nt ~ U{10..120},
 nd ~ U{0..8}) through the continuous-batching server, FP64 and mixed: every result
 bitwise equal to the same request run alone, every x with a normalized residual
@@ -49,15 +50,26 @@ def test_c3_batch(gpu, prec):
         x17 = hb.solve(None, b, 1)
     say(f"  batch done; logdet range {lds.min():.6e} .. {lds.max():.6e}")
     assert np.isfinite(lds).all() and np.isfinite(diags).all() and (diags > 0).all()
+    tiles = rgf_amd.default_options(precision=prec, keep_matrix=0, cholesky=rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES)
     for mid in (0, 17, count - 1):
         td, tc = _theta(C3_SEED, mid)
         with rgf_amd.RGF.generated(ns, nt, nd, seed=C3_SEED, matrix_id=mid, theta_d=td, theta_c=tc,
-                                   options=opts) as h1:
+                                   options=tiles) as h1:
             assert h1.logDet() == lds[mid], mid
             assert np.array_equal(h1.RGFdiag(), diags[mid]), mid
             if mid == 17:
                 assert np.array_equal(h1.solve(None, b, 1), x17)
-    say("  sampled instances equal single-matrix handles bitwise")
+    say("  sampled instances equal single-matrix handles of the same schedule bitwise")
+    # the same matrix alone on the default (task-graph) schedule: the same result to rounding
+    td, tc = _theta(C3_SEED, 17)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=C3_SEED, matrix_id=17, theta_d=td, theta_c=tc,
+                               options=opts) as h1:
+        ld_g, d_g = h1.logDet(), h1.RGFdiag()
+    tol_d = 1e-12 if prec == rgf_amd.RGF_FP64 else 1e-5
+    assert abs(ld_g - lds[17]) <= 1e-13 * abs(lds[17])
+    assert np.abs(d_g / diags[17] - 1).max() <= tol_d
+    say(f"  task-graph schedule vs batch: logdet rel {abs(ld_g - lds[17]) / abs(lds[17]):.1e}, "
+        f"diag rel {np.abs(d_g / diags[17] - 1).max():.1e}")
     # instance 17 against the oracle on the same matrix (host generator = device generator)
     td, tc = _theta(C3_SEED, 17)
     A_mf = oracle.generate(C3_SEED, ns, nt, nd, matrix_id=17, theta_d=td, theta_c=tc)

@@ -164,7 +164,7 @@ def test_gemm_tile_shapes_same_bits(gpu, ta, tb, flags, m, n, k):
         assert np.array_equal(o, outs[0])
 
 
-@pytest.mark.parametrize("n", [1, 5, 64, 65, 150, 200, 300, 333])
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 150, 200, 300, 333, 1000])
 def test_potrf_trtri_kernels(gpu, n):
     say(f"potrf/trtri n={n}")
     rng = np.random.default_rng(n)
@@ -189,19 +189,49 @@ def test_potrf_trtri_kernels(gpu, n):
     assert np.abs(X - Xr).max() < 1e-11 * np.abs(Xr).max()
 
 
-@pytest.mark.parametrize("bad", [70, 0, 20, 63, 99])
-def test_potrf_not_spd(gpu, bad):
-    """The first non-positive pivot is reported 1-based, wherever it falls in a tile."""
-    say(f"potrf not SPD at {bad}")
-    n = 100
+@pytest.mark.parametrize("bad", [70, 0, 20, 63, 99, 200, 311])
+@pytest.mark.parametrize("invert", [0, 1])
+def test_potrf_not_spd(gpu, bad, invert):
+    """The first non-positive pivot is reported 1-based, wherever it falls in a tile (invert=1
+    factors through the one-launch task graph, invert=0 through the tile launches)."""
+    say(f"potrf not SPD at {bad} invert={invert}")
+    n = 100 if bad < 100 else 330
     A = np.eye(n)
     A[bad, bad] = -1.0
     A[min(bad + 3, n - 1), min(bad + 3, n - 1)] = 0.0   # a later bad pivot is not reported
     flat = A.reshape(-1, order="F").copy()
     import ctypes as C
     ld = C.c_double()
-    assert _lib.load().rgf_debug_potrf(n, dptr(flat), n, 0, C.byref(ld)) == rgf_amd.RGF_ENOTSPD
+    assert _lib.load().rgf_debug_potrf(n, dptr(flat), n, invert, C.byref(ld)) == rgf_amd.RGF_ENOTSPD
     assert f"column {bad + 1}".encode() in _lib.load().rgf_last_error(None)
+
+
+@pytest.mark.parametrize("n", [130, 333, 1000, 2865])
+def test_potrf_task_graph_matches_tile_launches(gpu, n):
+    """L^-1 through the one-launch task-graph Cholesky against the tile-launch Cholesky: the
+    same factorization to rounding (the panel updates are grouped differently)."""
+    say(f"potrf task graph vs tile launches n={n}")
+    rng = np.random.default_rng(7 + n)
+    Gm = rng.standard_normal((n, n))
+    A = Gm @ Gm.T / n + np.eye(n)
+    import ctypes as C
+    lib = _lib.load()
+    outs, lds = [], []
+    for dag in (1, 0):
+        old = lib.rgf_debug_set_tuning(16, dag)
+        try:
+            flat = A.reshape(-1, order="F").copy()
+            ld = C.c_double()
+            assert lib.rgf_debug_potrf(n, dptr(flat), n, 1, C.byref(ld)) == 0
+        finally:
+            lib.rgf_debug_set_tuning(16, old)
+        outs.append(flat.reshape(n, n, order="F"))
+        lds.append(ld.value)
+    X0, X1 = outs
+    assert np.abs(np.triu(X0, 1)).max() == 0.0
+    assert np.abs(X0 - X1).max() < 1e-12 * np.abs(X1).max()
+    assert lds[0] == pytest.approx(lds[1], rel=1e-13)
+    assert lds[0] == pytest.approx(np.linalg.slogdet(A)[1], rel=1e-11)
 
 
 # ------------------------------------------------------------------ full path vs oracle
@@ -468,13 +498,17 @@ def test_cli_matches_reference_driver_format(gpu, tmp_path):
 
 
 # ------------------------------------------------------------------ batched handles
-@pytest.mark.parametrize("ns,nt,nd,count", [(100, 6, 5, 3), (65, 3, 0, 4), (8, 5, 3, 5)])
-def test_batch_matches_single_and_oracle(gpu, ns, nt, nd, count):
+@pytest.mark.parametrize("sched", [rgf_amd.RGF_CHOLESKY_TASK_GRAPH, rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES])
+@pytest.mark.parametrize("ns,nt,nd,count", [(100, 6, 5, 3), (65, 3, 0, 4), (8, 5, 3, 5), (200, 3, 4, 6)])
+def test_batch_matches_single_and_oracle(gpu, ns, nt, nd, count, sched):
     """c3-style batch (per-instance theta): every matrix of the batch gives bitwise the
-    same logdet and diag(A^-1) as that matrix alone, and matches the oracle."""
-    say(f"batch ns={ns} nt={nt} nd={nd} count={count}")
+    same logdet and diag(A^-1) as that matrix alone on the same Cholesky schedule, and
+    matches the oracle."""
+    say(f"batch ns={ns} nt={nt} nd={nd} count={count} schedule={sched}")
     seed = 99
-    with rgf_amd.RGF.generated(ns, nt, nd, seed=seed, matrix_id=10, count=count, theta_mode=1) as hb:
+    opts = rgf_amd.default_options(cholesky=sched)
+    with rgf_amd.RGF.generated(ns, nt, nd, seed=seed, matrix_id=10, count=count, theta_mode=1,
+                               options=opts) as hb:
         assert hb.count == count
         lds = hb.logdet_batch()
         diags = hb.selinv_diag_batch()
@@ -491,7 +525,8 @@ def test_batch_matches_single_and_oracle(gpu, ns, nt, nd, count):
         L = oracle.factor(mats[b], ns, nt, nd)
         assert abs(lds[b] - oracle.logdet(L, ns, nt, nd)) <= 1e-11 * abs(lds[b])
         assert np.abs(diags[b] / oracle.selinv(L, ns, nt, nd, exact=True) - 1).max() <= 1e-10
-        with rgf_amd.RGF.generated(ns, nt, nd, seed=seed, matrix_id=10 + b, theta_d=td, theta_c=tc) as h1:
+        with rgf_amd.RGF.generated(ns, nt, nd, seed=seed, matrix_id=10 + b, theta_d=td, theta_c=tc,
+                                   options=opts) as h1:
             assert np.array_equal(h1.export_matrix_mf(), mats[b])
             assert h1.logDet() == lds[b]
             assert np.array_equal(h1.RGFdiag(), diags[b])
@@ -944,13 +979,14 @@ def test_large_batch_count_with_arrow(gpu):
     single-matrix handles bitwise and the last matches the oracle."""
     ns, nt, nd, count = 48, 2, 3, 200
     say(f"large batch ns={ns} nt={nt} nd={nd} count={count}")
+    tiles = rgf_amd.default_options(cholesky=rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES)   # the batch's schedule
     with rgf_amd.RGF.generated(ns, nt, nd, seed=6, count=count, theta_mode=1) as hb:
         lds = hb.logdet_batch()
         diags = hb.selinv_diag_batch()
     for b in (0, 129, count - 1):
         td = 0.5 + 1.5 * oracle.gen_uniform(6, b, 0)
         tc = 0.1 + 0.9 * oracle.gen_uniform(6, b, 1)
-        with rgf_amd.RGF.generated(ns, nt, nd, seed=6, matrix_id=b, theta_d=td, theta_c=tc) as h1:
+        with rgf_amd.RGF.generated(ns, nt, nd, seed=6, matrix_id=b, theta_d=td, theta_c=tc, options=tiles) as h1:
             assert h1.logDet() == lds[b]
             assert np.array_equal(h1.RGFdiag(), diags[b])
             L = oracle.factor(h1.export_matrix_mf(), ns, nt, nd)
