@@ -282,7 +282,8 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     // solve buffers (Y, X, Bv, R, Dc: n x 16 each) and the skinny products' partials
     size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
     size_t bytes = (per * count + once) * sizeof(double) + potrf_dag_flag_elems(ns) * count * sizeof(int);
-    if (potrf_dag_schedule(cholesky, count)) bytes += potrf_dag_xbuf_elems(ns) * count * sizeof(double);
+    if (precision == RGF_FP64 && potrf_dag_schedule(cholesky, count))
+        bytes += potrf_dag_xbuf_elems(ns) * count * sizeof(double);
     if (precision == RGF_MIXED_BF16X3)
         bytes += (split_scratch_need(ns, nd, count) + split_x_need(ns, nd, count) +
                   lsplit_need(ns, nt, nd, count, keep_splits) + xcopy_need(ns, nt, count, keep_splits)) * 2;
@@ -325,7 +326,11 @@ int alloc_core(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->cflags, potrf_dag_flag_elems(ns) * B))) return rc;
     if ((rc = (int)hipMemsetAsync(h->cflags, 0, potrf_dag_flag_elems(ns) * B * sizeof(int), h->stream)))
         return dev_status(h, rc, "create");
-    if (potrf_dag_schedule(h->opt.cholesky, B) && (rc = dalloc(h, &h->xbuf, potrf_dag_xbuf_elems(ns) * B))) return rc;
+    // L_ii^-1 in the same launch on the FP64 path (the mixed path's inverse merges are split-BF16
+    // products, cheaper than the launch's FP64 tiles)
+    if (h->opt.precision == RGF_FP64 && potrf_dag_schedule(h->opt.cholesky, B) &&
+        (rc = dalloc(h, &h->xbuf, potrf_dag_xbuf_elems(ns) * B)))
+        return rc;
     if ((rc = dalloc(h, &h->scal, 64))) return rc;
     if (h->opt.precision == RGF_MIXED_BF16X3) {
         h->split_elems = split_scratch_need(ns, nd, B);
@@ -574,7 +579,7 @@ int factor_sweep(rgf_handle_s* h)
         bool x_done = false;
         if ((rc = potrf_f64(P, ldp, ms, ns, B, h->parts + (long long)i * h->T, h->partstride, h->dinfo, i * ns,
                             h->xdiag, h->xstride, false, s, h->cflags, h->xbuf, (long long)potrf_dag_xbuf_elems(ns),
-                            &x_done, h->opt.cholesky)))
+                            h->xbuf ? &x_done : nullptr, h->opt.cholesky)))
             return rc;
         // mixed path: the inverse keeps split copies of X_i's rows (the product below) and
         // columns (the selected inversion's)
@@ -1915,6 +1920,11 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
     if (key == 18 && value >= 2) {
         const int old = g_chol_dag_min;
         g_chol_dag_min = value;
+        return old;
+    }
+    if (key == 19 && (value == 0 || value == 1)) {
+        const int old = g_chol_dag_x;
+        g_chol_dag_x = value;
         return old;
     }
     if (key == 9 && (value == 0 || value == 1)) {

@@ -102,9 +102,30 @@ __device__ __forceinline__ void tile_sub_panel(double (*Ls)[LP], const double (*
     __syncthreads();
 }
 
+// The same over eight waves (16-row strips x 32-column halves; the two strips wholly above
+// the diagonal skip).
+__device__ __forceinline__ void tile_sub_panel8(double (*Ls)[LP], const double (*Pk)[LP], int wave, int l16, int lq)
+{
+    const int wm = wave & 3, wn = wave >> 2;
+    if (16 * wm + 15 >= 32 * wn) {
+        v4d acc[1][2];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[0][b][t] = Ls[16 * wm + l16][32 * wn + 16 * b + 4 * t + lq];
+        mfma_nt<1>(acc, Pk, Pk, -1.0, wm, wn, l16, lq);
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) Ls[16 * wm + l16][32 * wn + 16 * b + 4 * t + lq] = acc[0][b][t];
+    }
+    __syncthreads();
+}
+
 // Cholesky of the tile in Ls (identity-padded beyond kb) in four 16-column panels; each
 // row's 1/sqrt and sqrt of its pivot go to rinv and dgl; bad receives the first
 // non-positive pivot (wave 0).
+template <int NW = 4>
 __device__ __forceinline__ void tile_factor(double (*Ls)[LP], double* colb, double* rinv, double* dgl, int kb,
                                             int& bad, int tid)
 {
@@ -195,7 +216,7 @@ __device__ __forceinline__ void tile_factor(double (*Ls)[LP], double* colb, doub
         // trailing blocks (I, J), K < J <= I: T_IJ -= L_IK L_JK^T, one MFMA block per wave
         // (accumulator item q of lane (l16, lq) is T[16 I + l16][16 J + 4 q + lq])
         const int nbr = 3 - K, nblk = nbr * (nbr + 1) / 2;
-        for (int q = wave; q < nblk; q += 4) {
+        for (int q = wave; q < nblk; q += NW) {
             int ii = 0;
             while ((ii + 1) * (ii + 2) / 2 <= q) ii++;
             const int I = K + 1 + ii, J = K + 1 + (q - ii * (ii + 1) / 2);
@@ -234,13 +255,14 @@ __device__ __forceinline__ void tile_logdet(const double* dgl, int kb, int bad, 
 }
 
 // X = L^-1 of the factored tile in Ls into Xs (Ts: merge scratch).
+template <int NW = 4>
 __device__ __forceinline__ void tile_inverse(const double (*Ls)[LP], double (*Xs)[LP],
                                              double (*Ts)[NB / 2][NB / 2 + 2], const double* rinv, int tid)
 {
     const int lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     // ---- inverse, level 0: wave w inverts diagonal block w (lane c < 16 owns column c)
-    if (lane < 16) {
+    if (lane < 16 && wave < 4) {
         const int o = 16 * wave, c = lane;
         double x[16];
 #pragma unroll
@@ -280,22 +302,27 @@ __device__ __forceinline__ void tile_inverse(const double (*Ls)[LP], double (*Xs
     }
     __syncthreads();
     {
-        const int bi = 16 * (wave >> 1), bj = 16 * (wave & 1);
+        // waves 0..3, one 16 x 16 block each (further waves only take part in the barriers)
+        const int bi = 16 * ((wave >> 1) & 1), bj = 16 * (wave & 1);
         v4d acc = {0.0, 0.0, 0.0, 0.0};
+        if (wave < 4) {
 #pragma unroll
-        for (int ks = 0; ks < 8; ks++)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[4 * ks + lq][bj + l16], Ls[32 + bi + l16][4 * ks + lq], acc, 0,
-                                                       0, 0);
+            for (int ks = 0; ks < 8; ks++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[4 * ks + lq][bj + l16], Ls[32 + bi + l16][4 * ks + lq],
+                                                           acc, 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < 4; t++) Ts[0][bi + l16][bj + 4 * t + lq] = acc[t];
+            for (int t = 0; t < 4; t++) Ts[0][bi + l16][bj + 4 * t + lq] = acc[t];
+        }
         __syncthreads();
-        acc = (v4d){0.0, 0.0, 0.0, 0.0};
+        if (wave < 4) {
+            acc = (v4d){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int ks = 0; ks < 8; ks++)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[0][4 * ks + lq][bj + l16], Xs[32 + bi + l16][32 + 4 * ks + lq],
-                                                       acc, 0, 0, 0);
+            for (int ks = 0; ks < 8; ks++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[0][4 * ks + lq][bj + l16],
+                                                           Xs[32 + bi + l16][32 + 4 * ks + lq], acc, 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < 4; t++) Xs[32 + bi + l16][bj + 4 * t + lq] = -acc[t];
+            for (int t = 0; t < 4; t++) Xs[32 + bi + l16][bj + 4 * t + lq] = -acc[t];
+        }
     }
     __syncthreads();
 }
@@ -580,21 +607,25 @@ __device__ __forceinline__ void dag_release2(int* f, int v, int* f2, int v2, int
     }
 }
 
+// workgroup of the task-graph launch: eight waves (two per SIMD); a 64 x 64 tile product
+// gives each wave a 16 x 32 part (mfma_nt<1>)
+constexpr int DNT = 512, DNW = DNT / 64, DRA = 8 / DNW;
+
 // rows x cols of the tile at (r0, c0) into S, zero padded (lanes along rows: coalesced)
 __device__ __forceinline__ void dag_load(double (*S)[LP], const double* A, int lda, int r0, int c0, int rows,
                                          int cols, int tid)
 {
-    double v[NB * NB / 256];
+    double v[NB * NB / DNT];
 #pragma unroll
-    for (int q = 0; q < NB * NB / 256; q++) {
-        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+    for (int q = 0; q < NB * NB / DNT; q++) {
+        const int e = q * DNT + tid, r = e & (NB - 1), c = e >> 6;
         const bool in = r < rows && c < cols;
         v[q] = ldc(A + (in ? (long long)(c0 + c) * lda + (r0 + r) : (long long)c0 * lda + r0));
         v[q] = in ? v[q] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < NB * NB / 256; q++) {
-        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+    for (int q = 0; q < NB * NB / DNT; q++) {
+        const int e = q * DNT + tid, r = e & (NB - 1), c = e >> 6;
         S[r][c] = v[q];
     }
 }
@@ -603,84 +634,90 @@ __device__ __forceinline__ void dag_load(double (*S)[LP], const double* A, int l
 __device__ __forceinline__ void dag_load_t(double (*S)[LP], const double* A, int lda, int r0, int c0, int rows,
                                            int cols, int tid)
 {
-    double v[NB * NB / 256];
+    double v[NB * NB / DNT];
 #pragma unroll
-    for (int q = 0; q < NB * NB / 256; q++) {
-        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+    for (int q = 0; q < NB * NB / DNT; q++) {
+        const int e = q * DNT + tid, r = e & (NB - 1), c = e >> 6;
         const bool in = r < rows && c < cols;
         v[q] = ldc(A + (in ? (long long)(c0 + c) * lda + (r0 + r) : (long long)c0 * lda + r0));
         v[q] = in ? v[q] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < NB * NB / 256; q++) {
-        const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+    for (int q = 0; q < NB * NB / DNT; q++) {
+        const int e = q * DNT + tid, r = e & (NB - 1), c = e >> 6;
         S[c][r] = v[q];
     }
 }
 
-// the wave's 32 x 32 quarter of a 64 x 64 accumulator (mfma_nt<2> layout) from / to LDS
-__device__ __forceinline__ void acc_from(v4d (&acc)[2][2], const double (*S)[LP], int wm, int wn, int l16, int lq)
+// the wave's (16 RA) x 32 part of a 64 x 64 accumulator (mfma_nt<RA> layout) from / to LDS
+template <int RA>
+__device__ __forceinline__ void acc_from(v4d (&acc)[RA][2], const double (*S)[LP], int wm, int wn, int l16, int lq)
 {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[a][b][t] = S[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
+            for (int t = 0; t < 4; t++) acc[a][b][t] = S[16 * RA * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq];
 }
-__device__ __forceinline__ void acc_to(const v4d (&acc)[2][2], double (*S)[LP], int wm, int wn, int l16, int lq)
+template <int RA>
+__device__ __forceinline__ void acc_to(const v4d (&acc)[RA][2], double (*S)[LP], int wm, int wn, int l16, int lq)
 {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) S[32 * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
+            for (int t = 0; t < 4; t++) S[16 * RA * wm + 16 * a + l16][32 * wn + 16 * b + 4 * t + lq] = acc[a][b][t];
 }
-__device__ __forceinline__ void acc_to_t(const v4d (&acc)[2][2], double (*S)[LP], int wm, int wn, int l16, int lq)
+template <int RA>
+__device__ __forceinline__ void acc_to_t(const v4d (&acc)[RA][2], double (*S)[LP], int wm, int wn, int l16, int lq)
 {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) S[32 * wn + 16 * b + 4 * t + lq][32 * wm + 16 * a + l16] = acc[a][b][t];
+            for (int t = 0; t < 4; t++) S[32 * wn + 16 * b + 4 * t + lq][16 * RA * wm + 16 * a + l16] = acc[a][b][t];
 }
 // the accumulator from the tile at (r0, c0) (64 x 64 readable)
-__device__ __forceinline__ void acc_load(v4d (&acc)[2][2], const double* A, int lda, int r0, int c0, int wm, int wn,
+template <int RA>
+__device__ __forceinline__ void acc_load(v4d (&acc)[RA][2], const double* A, int lda, int r0, int c0, int wm, int wn,
                                          int l16, int lq)
 {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                acc[a][b][t] = ldc(A + (long long)(c0 + 32 * wn + 16 * b + 4 * t + lq) * lda + (r0 + 32 * wm + 16 * a + l16));
+                acc[a][b][t] = ldc(A + (long long)(c0 + 32 * wn + 16 * b + 4 * t + lq) * lda + (r0 + 16 * RA * wm + 16 * a + l16));
 }
-__device__ __forceinline__ void acc_zero(v4d (&acc)[2][2])
+template <int RA>
+__device__ __forceinline__ void acc_zero(v4d (&acc)[RA][2])
 {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
 }
 // rows x cols of the accumulator into the tile at (r0, c0)
-__device__ __forceinline__ void acc_store(const v4d (&acc)[2][2], double* A, int lda, int r0, int c0, int rows,
+template <int RA>
+__device__ __forceinline__ void acc_store(const v4d (&acc)[RA][2], double* A, int lda, int r0, int c0, int rows,
                                           int cols, int wm, int wn, int l16, int lq)
 {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int r = 32 * wm + 16 * a + l16, c = 32 * wn + 16 * b + 4 * t + lq;
+                const int r = 16 * RA * wm + 16 * a + l16, c = 32 * wn + 16 * b + 4 * t + lq;
                 if (r < rows && c < cols) stc(A + (long long)(c0 + c) * lda + (r0 + r), acc[a][b][t]);
             }
 }
 
-__global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
+__global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
 {
     const long long mb = blockIdx.y;
     double* const A = a.A + mb * a.sA;
@@ -712,10 +749,10 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
         reinterpret_cast<double (*)[NB / 2][NB / 2 + 2]>(pool + 3 * NB + 4 * NB * LP);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    const int wm = wave & 1, wn = wave >> 1;
-    for (int e = tid; e < 4 * NB * LP; e += 256) pool[3 * NB + e] = 0.0;   // X's upper blocks stay zero
+    const int wm = wave % (4 / DRA), wn = wave / (4 / DRA);
+    for (int e = tid; e < 4 * NB * LP; e += DNT) pool[3 * NB + e] = 0.0;   // X's upper blocks stay zero
     __syncthreads();
-    v4d acc[2][2];
+    v4d acc[DRA][2];
 
     if (blockIdx.x == 0) {
         // ---- the chain
@@ -731,16 +768,16 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
                 break;
             const long long c1 = wall_clock64();
             {   // tile (k, k): lower triangle, identity beyond kb
-                double v[NB * NB / 256];
+                double v[NB * NB / DNT];
 #pragma unroll
-                for (int q = 0; q < NB * NB / 256; q++) {
-                    const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+                for (int q = 0; q < NB * NB / DNT; q++) {
+                    const int e = q * DNT + tid, r = e & (NB - 1), c = e >> 6;
                     const bool in = r < kb && c < kb;
                     v[q] = ldc(A + (in ? (long long)(k0 + c) * lda + (k0 + r) : (long long)k0 * lda + k0));
                 }
 #pragma unroll
-                for (int q = 0; q < NB * NB / 256; q++) {
-                    const int e = q * 256 + tid, r = e & (NB - 1), c = e >> 6;
+                for (int q = 0; q < NB * NB / DNT; q++) {
+                    const int e = q * DNT + tid, r = e & (NB - 1), c = e >> 6;
                     Bl[r][c] = (r < kb && c < kb) ? (c <= r ? v[q] : 0.0) : (r == c ? 1.0 : 0.0);
                 }
             }
@@ -748,21 +785,21 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
             const long long cl = wall_clock64();
             if (k >= 1) {
                 acc_zero(acc);
-                mfma_nt<2>(acc, Ba, Bx, 1.0, wm, wn, l16, lq);   // L_{k,k-1} = A_{k,k-1} X_{k-1}^T
+                mfma_nt<DRA>(acc, Ba, Bx, 1.0, wm, wn, l16, lq);   // L_{k,k-1} = A_{k,k-1} X_{k-1}^T
                 acc_store(acc, A, lda, k0, k0 - NB, kb, NB, wm, wn, l16, lq);
                 __syncthreads();
                 acc_to(acc, Ba, wm, wn, l16, lq);
                 __syncthreads();
-                tile_sub_panel(Bl, Ba, wave, l16, lq);
+                tile_sub_panel8(Bl, Ba, wave, l16, lq);
             }
             const long long cp = wall_clock64();
             int bad = INT_MAX;
-            tile_factor(Bl, colb, rinv, dgl, kb, bad, tid);
+            tile_factor<DNW>(Bl, colb, rinv, dgl, kb, bad, tid);
             const long long cf = wall_clock64();
             tile_logdet(dgl, kb, bad, true, parts + k, info, a.info_base + k0, tid);
-            tile_inverse(Bl, Bx, Ts, rinv, tid);
+            tile_inverse<DNW>(Bl, Bx, Ts, rinv, tid);
             const long long ci = wall_clock64();
-            for (int e = tid; e < NB * NB; e += 256) {
+            for (int e = tid; e < NB * NB; e += DNT) {
                 const int r = e & (NB - 1), c = e >> 6;
                 stc(xd + (long long)k * NB * NB + c * NB + r, Bx[r][c]);
             }
@@ -788,7 +825,7 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
                 __syncthreads();
                 if (k >= 1) {
                     acc_from(acc, Bn, wm, wn, l16, lq);
-                    mfma_nt<2>(acc, Bl, Ba, -1.0, wm, wn, l16, lq);
+                    mfma_nt<DRA>(acc, Bl, Ba, -1.0, wm, wn, l16, lq);
                     __syncthreads();
                     acc_to(acc, Bn, wm, wn, l16, lq);
                 }
@@ -835,7 +872,7 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
                 else
                     acc_zero(acc);
                 __syncthreads();
-                mfma_nt<2>(acc, B0, B1, 1.0, wm, wn, l16, lq);   // Y_ij += L_ik X_kj
+                mfma_nt<DRA>(acc, B0, B1, 1.0, wm, wn, l16, lq);   // Y_ij += L_ik X_kj
                 if (k < i - 1) {
                     acc_store(acc, xb, ldx, NB * i, NB * j, NB, NB, wm, wn, l16, lq);
                     dag_release(acnt + i * T + j, k - j + 1, tid);
@@ -843,7 +880,7 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
                     acc_to_t(acc, B3, wm, wn, l16, lq);
                     __syncthreads();
                     acc_zero(acc);
-                    mfma_nt<2>(acc, B2, B3, -1.0, wm, wn, l16, lq);   // X_ij = -X_ii Y_ij
+                    mfma_nt<DRA>(acc, B2, B3, -1.0, wm, wn, l16, lq);   // X_ij = -X_ii Y_ij
                     acc_store(acc, xb, ldx, NB * i, NB * j, NB, NB, wm, wn, l16, lq);
                     dag_release(xrdy + i * T + j, 1, tid);
                 }
@@ -864,13 +901,13 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
                 __syncthreads();
                 if (g >= 1) {
                     acc_from(acc, B3, wm, wn, l16, lq);
-                    mfma_nt<2>(acc, B0, B2, -1.0, wm, wn, l16, lq);
+                    mfma_nt<DRA>(acc, B0, B2, -1.0, wm, wn, l16, lq);
                     __syncthreads();
                     acc_to(acc, B3, wm, wn, l16, lq);
                     __syncthreads();
                 }
                 acc_zero(acc);
-                mfma_nt<2>(acc, B3, B1, 1.0, wm, wn, l16, lq);
+                mfma_nt<DRA>(acc, B3, B1, 1.0, wm, wn, l16, lq);
                 acc_store(acc, A, lda, NB * i, NB * g, rows, NB, wm, wn, l16, lq);
                 dag_release(ldone + i * T + g, 1, tid);
                 if (tid == 0) dag_trace(a, 1, i, g, g, w0, w1, wall_clock64());
@@ -893,7 +930,7 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
                 if (j != i) dag_load(B2, A, lda, NB * j, NB * p, rj, NB, tid);
                 __syncthreads();
                 acc_from(acc, B3, wm, wn, l16, lq);
-                mfma_nt<2>(acc, B0, j != i ? B2 : B0, -1.0, wm, wn, l16, lq);
+                mfma_nt<DRA>(acc, B0, j != i ? B2 : B0, -1.0, wm, wn, l16, lq);
                 acc_store(acc, A, lda, NB * i, NB * j, ri, rj, wm, wn, l16, lq);
                 dag_release(cnt + i * T + j, p + 1, tid);
                 if (tid == 0) dag_trace(a, 2, i, j, g, w0, w1, wall_clock64());
@@ -909,13 +946,14 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs a)
     __syncthreads();
     if (s_ok) {
         const int ne = dag_flag_elems(T);
-        for (int e = tid; e < ne; e += 256) __hip_atomic_store(fl + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int e = tid; e < ne; e += DNT) __hip_atomic_store(fl + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 int g_chol_dag = 1;         // 1: diagonal-block Cholesky as one task-graph launch (where flags are given)
 int g_chol_dag_wgs = 256;   // its workgroups per launch (all matrices together)
 int g_chol_dag_min = 64;    // ... used only when each matrix gets at least this many
+int g_chol_dag_x = 1;       // 1: the task graph also forms L^-1 (where the caller gives a buffer)
 
 size_t potrf_dag_flag_elems(int n) { return (size_t)dag_flag_elems((n + NB - 1) / NB); }
 size_t potrf_dag_xbuf_elems(int n)
@@ -1015,12 +1053,12 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
     if (x_done) *x_done = false;
     if (dag_flags && !store_diag_l && potrf_dag_schedule(schedule, count)) {
         const int T = (n + NB - 1) / NB;
-        const bool with_x = xbuf && x_done;
+        const bool with_x = xbuf && x_done && g_chol_dag_x;
         DagArgs a{A, sA, xdiag, sX, logdet_parts, sP, info, dag_flags, (long long)dag_flag_elems(T),
                   dag_timeout_ticks(), lda, n, T, dag_tasks(T, with_x), info_base, g_chol_dag_trace,
                   g_chol_dag_trace_cap, with_x ? xbuf : nullptr, sXb, T * NB};
         const int G = std::min(std::max(2, g_chol_dag_wgs / std::max(count, 1)), 1 + a.ntasks);
-        hipLaunchKernelGGL(potrf_dag_kernel, dim3(G, count), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(potrf_dag_kernel, dim3(G, count), dim3(DNT), 0, s, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess || !with_x) return (int)e;
         hipLaunchKernelGGL(dag_finish_kernel, dim3(T * T, count), dim3(256), 0, s, A, lda, sA, n, T, xdiag, sX, xbuf,

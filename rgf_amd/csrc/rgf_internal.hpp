@@ -264,7 +264,7 @@ bool potrf_dag_schedule(int schedule, int count);
 size_t potrf_dag_flag_elems(int n);
 size_t potrf_dag_xbuf_elems(int n);
 bool potrf_uses_dag(int count);   // a launch over `count` matrices takes the task-graph path
-extern int g_chol_dag, g_chol_dag_wgs, g_chol_dag_min;
+extern int g_chol_dag, g_chol_dag_wgs, g_chol_dag_min, g_chol_dag_x;
 extern long long* g_chol_dag_trace;   // device buffer: [0] records, then 8 longs per task (tuning)
 extern long long g_chol_dag_trace_cap;
 // A product launcher supplied by the caller (precision policy of a handle).
