@@ -54,6 +54,8 @@ def parse():
                    help="serving: HBM budget of live handles (the arena keeps up to --cache-gb more)")
     p.add_argument("--cache-gb", type=float, default=115.0,
                    help="serving: idle device memory the library's arena keeps for reuse (GiB)")
+    p.add_argument("--serve-cholesky", choices=["tiles", "graph"], default="tiles",
+                   help="serving: Cholesky schedule of the request handles (tile launches or task graph)")
     p.add_argument("--c3", action="store_true", help="c3: 64 matrices (1500, 60, 5), per-instance theta")
     p.add_argument("--only-batched", choices=["fp64", "mixed"], default=None,
                    help="run only the batched measurement of one precision (for profiling)")
@@ -195,7 +197,7 @@ def run_serve(args):
     out = {"metric": "serving req/s + p50/p99 latency (factorize+logdet+solve per request)",
            "workload": f"c4: {args.requests} requests, Poisson rate {args.rate}/s, ns~U{{300..3000}}, "
                        f"nt~U{{10..120}}, nd~U{{0..8}}, 10% follow-up solves, {args.slots} slots, "
-                       f"{os.environ.get('GPU_MAX_HW_QUEUES')} HW queues",
+                       f"{os.environ.get('GPU_MAX_HW_QUEUES')} HW queues, Cholesky {args.serve_cholesky}",
            "data": "synthetic (device generator, per-request seed)"}
     import ctypes as C
     lib = rgf_amd._lib.load()
@@ -207,7 +209,9 @@ def run_serve(args):
 
     for name, prec in (("fp64", rgf_amd.RGF_FP64), ("mixed_bf16x3", rgf_amd.RGF_MIXED_BF16X3)):
         reqs = serve.poisson_workload(args.requests, args.rate, seed=args.seed, followup_frac=0.1)
-        srv = serve.Server(serve.GPUExecutor(prec, device_cache_gb=args.cache_gb), slots=args.slots,
+        sched = (rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES if args.serve_cholesky == "tiles"
+                 else rgf_amd.RGF_CHOLESKY_TASK_GRAPH)
+        srv = serve.Server(serve.GPUExecutor(prec, device_cache_gb=args.cache_gb, cholesky=sched), slots=args.slots,
                            budget_bytes=int(args.budget_gb * (1 << 30)),
                            progress=lambda d, t, nm=name: print(f"serve {nm}: {d}/{t} done", file=sys.stderr,
                                                                 flush=True))

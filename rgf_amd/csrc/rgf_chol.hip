@@ -1057,7 +1057,12 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
         DagArgs a{A, sA, xdiag, sX, logdet_parts, sP, info, dag_flags, (long long)dag_flag_elems(T),
                   dag_timeout_ticks(), lda, n, T, dag_tasks(T, with_x), info_base, g_chol_dag_trace,
                   g_chol_dag_trace_cap, with_x ? xbuf : nullptr, sXb, T * NB};
-        const int G = std::min(std::max(2, g_chol_dag_wgs / std::max(count, 1)), 1 + a.ntasks);
+        // workers: enough to finish the tasks in the chain's time -- a task takes about a quarter of
+        // a chain step (a 64 x 64 x 64 MFMA product and three tile loads against a tile factorization),
+        // so about ntasks / (4 T) workgroups; the early generations hold most tasks and the chain
+        // waits on them unless there are three times that (measured at ns = 2865); more only spin
+        const int want = 1 + (int)std::min<long long>((long long)a.ntasks, (3LL * a.ntasks + 4 * T - 1) / (4LL * T));
+        const int G = std::min(std::max(2, g_chol_dag_wgs / std::max(count, 1)), std::max(2, want));
         hipLaunchKernelGGL(potrf_dag_kernel, dim3(G, count), dim3(DNT), 0, s, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess || !with_x) return (int)e;

@@ -88,10 +88,14 @@ class Executor:
 class GPUExecutor(Executor):
     """librgf_hip.so: one handle (one stream) per request."""
 
-    def __init__(self, precision: int, refine_tol: float = 1e-12, device_cache_gb: Optional[float] = None):
+    def __init__(self, precision: int, refine_tol: float = 1e-12, device_cache_gb: Optional[float] = None,
+                 cholesky: Optional[int] = None):
         """device_cache_gb: idle device memory the library's arena may keep for reuse
         (None: the library's default).  A serving process sets it high: handles come and
-        go continuously, and giving slabs back calls hipFree, which waits for the device."""
+        go continuously, and giving slabs back calls hipFree, which waits for the device.
+        cholesky: the handles' Cholesky schedule (None: tile launches -- with many requests in
+        flight the device is shared, and the task graph's single-matrix latency gain would come
+        at the price of workgroups that wait on one matrix's chain while others could run)."""
         import rgf_amd
         self.rgf = rgf_amd
         if device_cache_gb is not None:
@@ -99,7 +103,9 @@ class GPUExecutor(Executor):
         self.precision = precision
         self.keep = 1 if precision != rgf_amd.RGF_FP64 else 0   # refinement needs A
         self.refine_tol = refine_tol
-        self.opts = rgf_amd.default_options(precision=precision, keep_matrix=self.keep, refine_tol=refine_tol)
+        self.opts = rgf_amd.default_options(
+            precision=precision, keep_matrix=self.keep, refine_tol=refine_tol,
+            cholesky=rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES if cholesky is None else cholesky)
 
     def footprint(self, req: Request) -> int:
         return self.rgf.footprint_bytes(req.ns, req.nt, req.nd, 1, self.opts)
