@@ -103,6 +103,7 @@ struct rgf_handle_s {
     double* Dc = nullptr;       // n x 16 correction of the refinement
     double* work = nullptr;     // skinny split-k partials
     size_t work_elems = 0;
+    int* sctr = nullptr;        // skinny products' arrival counters (SKINNY_CTRS, zero between launches)
     int last_refine_iters = 0;  // refinement steps taken by the last mixed solve
     double last_refine_res = 0.0;
     double* scal = nullptr;     // small device scalars
@@ -280,7 +281,7 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
                  (size_t)std::max(nt * nd * nd, 1);
     per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
     // solve buffers (Y, X, Bv, R, Dc: n x 16 each) and the skinny products' partials
-    size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count);
+    size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count) + SKINNY_CTRS / 2;
     size_t bytes = (per * count + once) * sizeof(double) + potrf_dag_flag_elems(ns) * count * sizeof(int);
     if (precision == RGF_FP64 && potrf_dag_schedule(cholesky, count))
         bytes += potrf_dag_xbuf_elems(ns) * count * sizeof(double);
@@ -356,6 +357,8 @@ int ensure_solve_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->Dc, n16))) return rc;
     h->work_elems = solve_work_elems(h->ns, h->nt, h->nd, h->count);
     if ((rc = dalloc(h, &h->work, h->work_elems))) return rc;
+    if ((rc = dalloc(h, &h->sctr, (size_t)SKINNY_CTRS))) return rc;
+    if ((rc = (int)hipMemsetAsync(h->sctr, 0, SKINNY_CTRS * sizeof(int), h->stream))) return dev_status(h, rc, "solve buffers");
     return RGF_OK;
 }
 
@@ -369,9 +372,10 @@ struct SolveBufs {
     double* Bv;
     double* work;
     size_t work_elems;
+    int* ctr;   // the skinny products' arrival counters (SKINNY_CTRS)
 };
 
-SolveBufs handle_bufs(rgf_handle_s* h) { return {h->Y, h->R, h->Dc, h->Bv, h->work, h->work_elems}; }
+SolveBufs handle_bufs(rgf_handle_s* h) { return {h->Y, h->R, h->Dc, h->Bv, h->work, h->work_elems, h->sctr}; }
 
 int ensure_selinv_buffers(rgf_handle_s* h)
 {
@@ -564,7 +568,7 @@ int factor_sweep(rgf_handle_s* h)
                 const int nc = std::min(16, nd - c0);
                 rc = skinny_f64s(Op::N, 0, ns, ns, nc, -1.0, Pm + ns, ldp, Pm + 2 * ns + c0, ldp, 1, 1.0,
                                  A + 2 * ns + c0, ldp, 1, Fd + c0, ldF, 1, h->work, h->work_elems, s, B,
-                                 ms, ms, ms, sF);
+                                 ms, ms, ms, sF, h->sctr);
                 if (rc) return rc;
             }
             if (panel_presplit(h, i - 1)) {   // B_{i-1}: the first ns rows of its panel's copy
@@ -745,42 +749,50 @@ int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const d
     double* Y = sb.Y;
     double* w = sb.work;
     const size_t we = sb.work_elems;
+    int* const ct = sb.ctr;
+    const Op N = Op::N, T = Op::T;
     const double* Xdd = h->Xdd + (long long)mb * h->cstride;
     int rc;
     if (x != b && (rc = copy2d(b, n, x, n, n, nr, 1.0, s))) return rc;   // x holds r during the forward pass
     double* xd = x + (long long)nt * ns;
     double* yd = Y + (long long)nt * ns;
+    // forward: [r_i; x_d] -= [B_{i-1}; D_{i-1}] y_{i-1} (the coupling and arrow rows of panel
+    // i - 1 are contiguous: one product, its rows past ns accumulate into x_d), y_i = X_i r_i
     for (int i = 0; i < nt; i++) {
         double* P = l_panel(h, i, mb);
         double* ri = x + (long long)i * ns;
         double* yi = Y + (long long)i * ns;
         if (i > 0) {
             const double* Pm = l_panel(h, i - 1, mb);
-            if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, n, 1.0, ri, n,
-                                 ri, n, w, s, we)))
+            if ((rc = skinny_f64s(N, 0, ns + nd, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, 1, n, 1.0, ri,
+                                  1, n, ri, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nullptr, 0, nd ? xd : nullptr, ns)))
                 return rc;
         }
-        if ((rc = skinny_f64(Op::N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we))) return rc;
-        if (nd > 0 &&
-            (rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, yi, n, 1.0, xd, n, xd, n, w, s, we)))
-            return rc;
+        if ((rc = skinny_f64(N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we, ct))) return rc;
     }
     if (nd > 0) {
-        if ((rc = skinny_f64(Op::N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we))) return rc;
-        if ((rc = skinny_f64(Op::T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we))) return rc;
+        const double* P = l_panel(h, nt - 1, mb);   // x_d -= D_{nt-1} y_{nt-1}
+        if ((rc = skinny_f64(N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, Y + (long long)(nt - 1) * ns, n, 1.0, xd, n, xd,
+                             n, w, s, we, ct)))
+            return rc;
+        if ((rc = skinny_f64(N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we, ct))) return rc;
+        if ((rc = skinny_f64(T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we, ct))) return rc;
     }
+    // backward: y_i -= [B_i; D_i]^T [x_{i+1}; x_d] (one product over both row groups of the
+    // panel), x_i = X_i^T y_i
     for (int i = nt - 1; i >= 0; i--) {
         double* P = l_panel(h, i, mb);
         double* yi = Y + (long long)i * ns;
         double* xi = x + (long long)i * ns;
-        if (nd > 0 &&
-            (rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s, we)))
+        if (i < nt - 1) {
+            if ((rc = skinny_f64s(T, 0, ns, ns + nd, nr, -1.0, P + ns, ldp, x + (long long)(i + 1) * ns, 1, n, 1.0,
+                                  yi, 1, n, yi, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns)))
+                return rc;
+        } else if (nd > 0 &&
+                   (rc = skinny_f64(T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s, we, ct))) {
             return rc;
-        if (i < nt - 1 &&
-            (rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, P + ns, ldp, x + (long long)(i + 1) * ns, n, 1.0, yi, n,
-                             yi, n, w, s, we)))
-            return rc;
-        if ((rc = skinny_f64(Op::T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we))) return rc;
+        }
+        if ((rc = skinny_f64(T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we, ct))) return rc;
     }
     return 0;
 }
@@ -1117,6 +1129,8 @@ int block_matvec_residual(rgf_handle_s* h, const SolveBufs& sb, int mb, const do
     hipStream_t s = h->stream;
     double* w = sb.work;
     const size_t we = sb.work_elems;
+    int* const ct = sb.ctr;
+    const Op N = Op::N, T = Op::T;
     int rc;
     if ((rc = copy2d(b, n, r, n, n, nr, 1.0, s))) return rc;
     const double* xd = x + (long long)nt * ns;
@@ -1125,20 +1139,28 @@ int block_matvec_residual(rgf_handle_s* h, const SolveBufs& sb, int mb, const do
         const double* A = h->Ap + (long long)mb * h->mstride + (long long)t * h->pstride;
         const double* xt = x + (long long)t * ns;
         double* rt = r + (long long)t * ns;
-        if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s, we))) return rc;
         if (t < nt - 1) {
-            if ((rc = skinny_f64(Op::T, 0, ns, ns, nr, -1.0, A + ns, ldp, xt + ns, n, 1.0, rt, n, rt, n, w, s, we))) return rc;
-            if ((rc = skinny_f64(Op::N, 0, ns, ns, nr, -1.0, A + ns, ldp, xt, n, 1.0, rt + ns, n, rt + ns, n, w, s, we)))
+            // [r_t; r_{t+1}; r_d] -= [A_tt; A_{t+1,t}; A_{d,t}] x_t (the whole panel, rows past 2 ns into r_d)
+            if ((rc = skinny_f64s(N, 0, 2 * ns + nd, ns, nr, -1.0, A, ldp, xt, 1, n, 1.0, rt, 1, n, rt, 1, n, w, we, s, 1,
+                                  0, 0, 0, 0, ct, nullptr, 0, nd ? rd : nullptr, 2 * ns)))
                 return rc;
-        }
-        if (nd > 0) {
-            if ((rc = skinny_f64(Op::T, 0, ns, nd, nr, -1.0, A + 2 * ns, ldp, xd, n, 1.0, rt, n, rt, n, w, s, we))) return rc;
-            if ((rc = skinny_f64(Op::N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s, we))) return rc;
+            // r_t -= [A_{t+1,t}; A_{d,t}]^T [x_{t+1}; x_d]
+            if ((rc = skinny_f64s(T, 0, ns, ns + nd, nr, -1.0, A + ns, ldp, xt + ns, 1, n, 1.0, rt, 1, n, rt, 1, n, w, we,
+                                  s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns)))
+                return rc;
+        } else {
+            if ((rc = skinny_f64(N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s, we, ct))) return rc;
+            if (nd > 0) {
+                if ((rc = skinny_f64(T, 0, ns, nd, nr, -1.0, A + 2 * ns, ldp, xd, n, 1.0, rt, n, rt, n, w, s, we, ct)))
+                    return rc;
+                if ((rc = skinny_f64(N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s, we, ct)))
+                    return rc;
+            }
         }
     }
     if (nd > 0 &&
-        (rc = skinny_f64(Op::N, 0, nd, nd, nr, -1.0, h->Add + (long long)mb * h->cstride, nd, xd, n, 1.0, rd, n, rd,
-                         n, w, s, we)))
+        (rc = skinny_f64(N, 0, nd, nd, nr, -1.0, h->Add + (long long)mb * h->cstride, nd, xd, n, 1.0, rd, n, rd,
+                         n, w, s, we, ct)))
         return rc;
     return 0;
 }
@@ -1287,18 +1309,23 @@ size_t solve_call_work(const rgf_handle_s* h, int nr)
     size_t w = 0;
     const Op N = Op::N, T = Op::T;
     for (auto c : {std::make_tuple(N, ns, ns), std::make_tuple(N, nd, ns), std::make_tuple(N, nd, nd),
-                   std::make_tuple(T, nd, nd), std::make_tuple(T, ns, nd), std::make_tuple(T, ns, ns)})
+                   std::make_tuple(T, nd, nd), std::make_tuple(T, ns, nd), std::make_tuple(T, ns, ns),
+                   std::make_tuple(N, ns + nd, ns), std::make_tuple(T, ns, ns + nd), std::make_tuple(N, 2 * ns + nd, ns)})
         w = std::max(w, skinny_call_work(std::get<0>(c), std::get<1>(c), std::get<2>(c), nr));
     return w;
 }
 
 inline size_t r32(size_t v) { return (v + 31) / 32 * 32; }   // 256-byte steps
 
+// the skinny products' arrival counters at the start of the scratch (zero when the array
+// is loaded; every launch leaves them zero)
+constexpr size_t kScratchCtr = SKINNY_CTRS / 2;   // doubles
+
 // scratch doubles of a chunk of nr columns (refined: R, Dc and, in place, Bv besides Y)
 size_t noalloc_need(const rgf_handle_s* h, int nr, bool refine, bool in_place)
 {
     const size_t col = r32((size_t)h->n * nr);
-    return col * (1 + (refine ? 2 : 0) + (refine && in_place ? 1 : 0)) + r32(solve_call_work(h, nr));
+    return kScratchCtr + col * (1 + (refine ? 2 : 0) + (refine && in_place ? 1 : 0)) + r32(solve_call_work(h, nr));
 }
 
 int solve_device_noalloc(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
@@ -1325,8 +1352,9 @@ int solve_device_noalloc(rgf_handle_s* h, double* x, const double* b, size_t nrh
     if (rc) return dev_status(h, rc, "scratch address");
     const size_t col = r32((size_t)n * nr);
     SolveBufs sb{};
-    sb.Y = base;
-    size_t off = col;
+    sb.ctr = (int*)base;
+    sb.Y = base + kScratchCtr;
+    size_t off = kScratchCtr + col;
     if (refine) {
         sb.R = base + off; off += col;
         sb.Dc = base + off; off += col;

@@ -319,9 +319,12 @@ size_t skinny_work_elems(int m, int k, int nr);
 // into slabs depends on m and k only, so each matrix is summed as it would be alone).
 size_t skinny_batch_work_elems(Op ta, int m, int k, int count);
 // work_elems: capacity of `work` (0: skinny_work_elems(m, k, nr), the most any call needs)
+// ctr (may be null): SKINNY_CTRS zero ints, left zero; with them the split-k partials are
+// reduced inside the product launch (same sums, same order), else by a second launch.
+constexpr int SKINNY_CTRS = 16384;
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
-               double* work, hipStream_t s, size_t work_elems = 0);
+               double* work, hipStream_t s, size_t work_elems = 0, int* ctr = nullptr);
 // Partial doubles one call of this shape with nr right-hand sides actually writes (0 when
 // k is not split).
 size_t skinny_call_work(Op ta, int m, int k, int nr);
@@ -331,7 +334,10 @@ size_t skinny_call_work(Op ta, int m, int k, int nr);
 int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda, const double* R,
                 long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
                 double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s,
-                int count = 1, long long sA = 0, long long sR = 0, long long sY0 = 0, long long sY = 0);
+                int count = 1, long long sA = 0, long long sR = 0, long long sY0 = 0, long long sY = 0,
+                int* ctr = nullptr, const double* R2 = nullptr, int k1 = 0, double* Y2 = nullptr, int m1 = 0);
+// (R2, k1): rows kk >= k1 of R read from R2 (row kk - k1, R's strides); (Y2, m1): rows
+// i >= m1 of Y (and of Y0) are those of Y2 (row i - m1, Y's strides).  Unbatched only.
 
 // ---------------------------------------------------------------- generator / ingestion
 struct GenParams {

@@ -77,7 +77,84 @@ struct SkinnyIO {
     double* Y;
     long long ysi, ysj;
     long long sA, sR, sY0, sY;   // strides between the matrices of a batch (blockIdx.z)
+    // split-k partials reduced in the product launch: one arrival counter per row block and
+    // matrix (zero before the launch, left zero), alpha / beta of the epilogue (null: a
+    // separate reduction launch)
+    int* ctr;
+    double alpha, beta;
+    int m, nr, slabs;
+    // two-segment operands (unbatched): rows kk >= k1 of R come from R2 (row kk - k1, R's
+    // strides), rows i >= m1 of Y (and Y0) are those of Y2 (row i - m1, Y's strides)
+    const double* R2;
+    double* Y2;
+    int k1, m1;
 };
+
+__device__ __forceinline__ double r_at(const SkinnyIO& io, int kk, int c)
+{
+    return kk < io.k1 ? io.R[(long long)kk * io.rsk + (long long)c * io.rsj]
+                      : io.R2[(long long)(kk - io.k1) * io.rsk + (long long)c * io.rsj];
+}
+__device__ __forceinline__ double y0_at(const SkinnyIO& io, int i, int c)
+{
+    return i < io.m1 ? io.Y0[(long long)i * io.y0si + (long long)c * io.y0sj]
+                     : io.Y2[(long long)(i - io.m1) * io.ysi + (long long)c * io.ysj];
+}
+__device__ __forceinline__ double* y_at(const SkinnyIO& io, int i, int c)
+{
+    return i < io.m1 ? io.Y + (long long)i * io.ysi + (long long)c * io.ysj
+                     : io.Y2 + (long long)(i - io.m1) * io.ysi + (long long)c * io.ysj;
+}
+
+// The split-k partials of the rows [r0, r0 + rows) of matrix b are in `work` (written by
+// this workgroup for slab blockIdx.y): every workgroup stores its partials with device-
+// coherent stores, waits for them, and counts itself in; the last of the row block's slabs
+// to arrive sums the partials of all slabs in slab order, s = 0, 1, ... (the order of the
+// separate reduction kernel, so the result is the same) and writes Y.  It then clears the
+// counter for the next launch.
+__device__ __forceinline__ double ld_sc1(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_sc1(double* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int NR>
+__device__ __forceinline__ void fused_reduce(const SkinnyIO& io, const double* work_b, int r0, int rows)
+{
+    __shared__ int s_last;
+    __builtin_amdgcn_s_waitcnt(0);   // this workgroup's partial stores are complete
+    __syncthreads();
+    int* const ctr = io.ctr + (long long)blockIdx.z * gridDim.x + blockIdx.x;
+    if (threadIdx.x == 0)
+        s_last = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == io.slabs - 1;
+    __syncthreads();
+    if (!s_last) return;
+    const int m = io.m;
+    // device-coherent plain loads (the sc1 cache policy), eight slabs in flight at a time
+    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc((void*)work_b, (short)0, 0x7fffffff, 0x00020000);
+    auto ld = [&](int sl, int c, int i) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rw, (int)(((sl * NR + c) * (long long)m + i) * 8), 0, 16));
+    };
+    for (int e = threadIdx.x; e < rows * io.nr; e += blockDim.x) {
+        const int i = r0 + e % rows, c = e / rows;
+        double v = 0.0;
+        int sl = 0;
+        for (; sl + 8 <= io.slabs; sl += 8) {
+            double t[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) t[q] = ld(sl + q, c, i);
+#pragma unroll
+            for (int q = 0; q < 8; q++) v += t[q];
+        }
+        for (; sl < io.slabs; sl++) v += ld(sl, c, i);
+        v *= io.alpha;
+        if (io.beta != 0.0) v += io.beta * y0_at(io, i, c);
+        *y_at(io, i, c) = v;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // partial layout: work[((b * slabs + s) * NR + c) * m + i] for matrix b = blockIdx.z
 // N: one lane per row of op(A) = A (column reads coalesced), KU columns of A in flight
@@ -112,7 +189,7 @@ __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int
         __syncthreads();
         for (int e = tid; e < KCH * NR; e += 256) {
             const int kk = e % KCH, c = e / KCH;
-            Rs[kk][c] = (kk < kc && c < nr) ? io.R[(long long)(k0 + kk) * io.rsk + (long long)c * io.rsj] : 0.0;
+            Rs[kk][c] = (kk < kc && c < nr) ? r_at(io, k0 + kk, c) : 0.0;
         }
         __syncthreads();
         if (i < m) {
@@ -128,19 +205,24 @@ __global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int
             }
         }
     }
-    if (i >= m) return;
-    if (direct) {
+    if (i < m) {
+        if (direct) {
 #pragma unroll
-        for (int c = 0; c < NR; c++) {
-            if (c >= nr) break;
-            double v = alpha * acc[c];
-            if (beta != 0.0) v += beta * io.Y0[(long long)i * io.y0si + (long long)c * io.y0sj];
-            io.Y[(long long)i * io.ysi + (long long)c * io.ysj] = v;
+            for (int c = 0; c < NR; c++) {
+                if (c >= nr) break;
+                double v = alpha * acc[c];
+                if (beta != 0.0) v += beta * y0_at(io, i, c);
+                *y_at(io, i, c) = v;
+            }
+        } else if (!io.ctr) {
+#pragma unroll
+            for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + i] = acc[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NR; c++) st_sc1(work + ((long long)s * NR + c) * m + i, acc[c]);
         }
-    } else {
-#pragma unroll
-        for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + i] = acc[c];
     }
+    if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * 256, min(256, m - (int)blockIdx.x * 256));
 }
 
 // N4 (one or two right-hand sides): 64 rows of op(A) = A per workgroup, each row by four lanes (one per wave), lane q
@@ -180,7 +262,7 @@ __global__ __launch_bounds__(256) void skinny_n4_kernel(int m, int k, int nr, in
         __syncthreads();
         for (int e = tid; e < KCH * NR; e += 256) {
             const int kk = e % KCH, c = e / KCH;
-            Rs[kk][c] = (kk < kc && c < nr) ? io.R[(long long)(k0 + kk) * io.rsk + (long long)c * io.rsj] : 0.0;
+            Rs[kk][c] = (kk < kc && c < nr) ? r_at(io, k0 + kk, c) : 0.0;
         }
         __syncthreads();
         if (i < m) {
@@ -200,21 +282,26 @@ __global__ __launch_bounds__(256) void skinny_n4_kernel(int m, int k, int nr, in
 #pragma unroll
     for (int c = 0; c < NR; c++) red[q][c][row] = acc[c];
     __syncthreads();
-    if (q != 0 || i >= m) return;
+    if (q == 0 && i < m) {
 #pragma unroll
-    for (int c = 0; c < NR; c++) acc[c] = ((red[0][c][row] + red[1][c][row]) + red[2][c][row]) + red[3][c][row];
-    if (direct) {
+        for (int c = 0; c < NR; c++) acc[c] = ((red[0][c][row] + red[1][c][row]) + red[2][c][row]) + red[3][c][row];
+        if (direct) {
 #pragma unroll
-        for (int c = 0; c < NR; c++) {
-            if (c >= nr) break;
-            double v = alpha * acc[c];
-            if (beta != 0.0) v += beta * io.Y0[(long long)i * io.y0si + (long long)c * io.y0sj];
-            io.Y[(long long)i * io.ysi + (long long)c * io.ysj] = v;
+            for (int c = 0; c < NR; c++) {
+                if (c >= nr) break;
+                double v = alpha * acc[c];
+                if (beta != 0.0) v += beta * y0_at(io, i, c);
+                *y_at(io, i, c) = v;
+            }
+        } else if (!io.ctr) {
+#pragma unroll
+            for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + i] = acc[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NR; c++) st_sc1(work + ((long long)s * NR + c) * m + i, acc[c]);
         }
-    } else {
-#pragma unroll
-        for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + i] = acc[c];
     }
+    if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * NROWS, min(NROWS, m - (int)blockIdx.x * NROWS));
 }
 
 // T: one wave per output row j of op(A) = A^T (column j of A read contiguously),
@@ -250,7 +337,7 @@ __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int
         __syncthreads();
         for (int e = tid; e < KCT * NR; e += 256) {
             const int kk = e % KCT, c = e / KCT;
-            Rs[c][kk] = (kk < kc && c < nr) ? io.R[(long long)(k0 + kk) * io.rsk + (long long)c * io.rsj] : 0.0;
+            Rs[c][kk] = (kk < kc && c < nr) ? r_at(io, k0 + kk, c) : 0.0;
         }
         __syncthreads();
         if (j < m) {
@@ -269,19 +356,24 @@ __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
         acc[c] = v;
     }
-    if (j >= m || lane != 0) return;
-    if (direct) {
+    if (j < m && lane == 0) {
+        if (direct) {
 #pragma unroll
-        for (int c = 0; c < NR; c++) {
-            if (c >= nr) break;
-            double v = alpha * acc[c];
-            if (beta != 0.0) v += beta * io.Y0[(long long)j * io.y0si + (long long)c * io.y0sj];
-            io.Y[(long long)j * io.ysi + (long long)c * io.ysj] = v;
+            for (int c = 0; c < NR; c++) {
+                if (c >= nr) break;
+                double v = alpha * acc[c];
+                if (beta != 0.0) v += beta * y0_at(io, j, c);
+                *y_at(io, j, c) = v;
+            }
+        } else if (!io.ctr) {
+#pragma unroll
+            for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + j] = acc[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NR; c++) st_sc1(work + ((long long)s * NR + c) * m + j, acc[c]);
         }
-    } else {
-#pragma unroll
-        for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + j] = acc[c];
     }
+    if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * 4, min(4, m - (int)blockIdx.x * 4));
 }
 
 __global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, int count, double alpha,
@@ -296,6 +388,11 @@ __global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, int 
         double v = 0.0;
         for (int s = 0; s < slabs; s++) v += w[((long long)s * wstride + c) * m + i];
         v *= alpha;
+        if (i >= io.m1) {   // second output segment (unbatched)
+            if (beta != 0.0) v += beta * y0_at(io, i, c);
+            *y_at(io, i, c) = v;
+            continue;
+        }
         if (beta != 0.0) v += beta * io.Y0[b * io.sY0 + (long long)i * io.y0si + (long long)c * io.y0sj];
         io.Y[b * io.sY + (long long)i * io.ysi + (long long)c * io.ysj] = v;
     }
@@ -303,11 +400,24 @@ __global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, int 
 
 template <int NR>
 static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
-                         const SkinnyIO& io, double beta, double* work, size_t work_elems, int count, hipStream_t s)
+                         const SkinnyIO& io_in, double beta, double* work, size_t work_elems, int count, hipStream_t s)
 {
     const SkinnyPlan p = plan_for(ta, m, k, nr);
     const int direct = p.slabs == 1;
     if (!direct && (size_t)count * p.slabs * NR * m > work_elems) return (int)hipErrorInvalidValue;
+    SkinnyIO io = io_in;
+    {   // reduce the partials in the product launch when the counters cover its row blocks
+        // (one workgroup per row block sums its slabs: only where that is a few partials per
+        // thread -- one or two right-hand sides -- else the reduction launch spreads them out)
+        const int rpw = ta == Op::T ? 4 : (n4_kernel(nr) ? NROWS : 256);
+        if (direct || !io.ctr || (long long)cdiv(m, rpw) * count > SKINNY_CTRS || (long long)rpw * nr > 256)
+            io.ctr = nullptr;
+        io.alpha = alpha;
+        io.beta = beta;
+        io.m = m;
+        io.nr = nr;
+        io.slabs = p.slabs;
+    }
     if (ta == Op::N) {
         if (n4_kernel(nr)) {
             dim3 grid(cdiv(m, NROWS), p.slabs, count);
@@ -325,7 +435,7 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
-    if (!direct) {
+    if (!direct && !io.ctr) {
         long long total = (long long)m * nr * count;
         int grid = (int)std::min((total + 255) / 256, (long long)4096);
         hipLaunchKernelGGL(skinny_reduce_kernel, dim3(grid), dim3(256), 0, s, m, nr, NR, p.slabs, count, alpha, work,
@@ -338,11 +448,14 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
 int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda, const double* R,
                 long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
                 double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s, int count,
-                long long sA, long long sR, long long sY0, long long sY)
+                long long sA, long long sR, long long sY0, long long sY, int* ctr, const double* R2, int k1,
+                double* Y2, int m1)
 {
     if (m <= 0 || nr <= 0 || count <= 0) return 0;
     if (nr > MAXR) return (int)hipErrorInvalidValue;
-    const SkinnyIO io{R, rsk, rsj, Y0, y0si, y0sj, Y, ysi, ysj, sA, sR, sY0, sY};
+    const SkinnyIO io{R, rsk, rsj, Y0, y0si, y0sj, Y, ysi, ysj, sA, sR, sY0, sY, ctr, 0.0, 0.0, 0, 0, 0,
+                      R2 ? R2 : R, Y2 ? Y2 : Y, R2 ? k1 : k, Y2 ? m1 : m};
+    if ((R2 || Y2) && count != 1) return (int)hipErrorInvalidValue;
     if (k <= 0) {   // Y = beta*Y0
         if (count == 1 && ysi == 1 && y0si == 1) return copy2d(Y0, (int)y0sj, Y, (int)ysj, m, nr, beta, s);
         return (int)hipErrorInvalidValue;
@@ -356,10 +469,10 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
 
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
-               double* work, hipStream_t s, size_t work_elems)
+               double* work, hipStream_t s, size_t work_elems, int* ctr)
 {
     return skinny_f64s(ta, tri, m, k, nr, alpha, A, lda, R, 1, ldr, beta, Y0, 1, ldy0, Y, 1, ldy, work,
-                       work_elems ? work_elems : skinny_work_elems(m, k, nr), s);
+                       work_elems ? work_elems : skinny_work_elems(m, k, nr), s, 1, 0, 0, 0, 0, ctr);
 }
 
 size_t skinny_call_work(Op ta, int m, int k, int nr)
