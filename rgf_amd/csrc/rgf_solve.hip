@@ -4,24 +4,28 @@
 // residual (RGF.H:631-663).  The factor stores L_ii^-1, so every triangular
 // solve is a product here and the whole solve is a stream over the factor.
 //
-// N: one lane per row of op(A) (coalesced column reads, 8 columns in flight), the
-//    R chunk staged in LDS; with one or two right-hand sides 64 rows per workgroup, four
-//    lanes per row each over a quarter of every k-chunk (four times the workgroups: the
-//    launch is otherwise too small to keep HBM busy); k split into slabs across
-//    workgroups for memory-level parallelism, partials reduced in a fixed order
-//    (deterministic).
-// T: one wave per output row (a column of A read contiguously by 64 lanes, 4 loads
-//    in flight), wave-level reduction; k split into slabs the same way.
+// One or two right-hand sides (FP64 VALU):
+//   N: 64 rows of op(A) per workgroup, four lanes per row each over a quarter of every
+//      k-chunk (coalesced column reads, 8 columns in flight), the R chunk staged in LDS;
+//   T: one wave per output row (a column of A read contiguously by 64 lanes, 8 loads in
+//      flight), wave-level reduction.
+// Three to sixteen right-hand sides: FP64 MFMA, 64 rows of op(A) per workgroup.
+// k is split into slabs across workgroups for memory-level parallelism; the partials are
+// summed in a fixed order (deterministic), inside the product launch where that is a few
+// partials per thread, else by a reduction launch.
 #include "rgf_internal.hpp"
 #include <algorithm>
 
 namespace rgf {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int MAXR = 16;
 constexpr int KCH = 128;   // k-chunk of R staged in LDS (N kernel)
 constexpr int KU = 8;      // A loads in flight per lane
 constexpr int NROWS = 64;  // rows of op(A) per workgroup of the N kernel
 constexpr int KCT = 256;   // k-chunk of the T kernel (4 per lane x 64 lanes)
+constexpr int KT1 = 8;     // A loads in flight per lane in the T kernel for one or two right-hand sides
 constexpr int TARGET_WG = 2048;
 
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -43,14 +47,15 @@ static SkinnyPlan plan(int wgm, int k, int chunk)
     return {std::max(slabs, 1), kslab};
 }
 
-// the N kernel takes NROWS rows per workgroup for one or two right-hand sides (its R
-// staging would dominate at more), 256 rows otherwise
-static inline bool n4_kernel(int nr) { return nr <= 2; }
+
+// three or more right-hand sides: the MFMA kernel (64 rows of op(A) per workgroup)
+static inline bool mfma_kernel(int nr) { return nr > 2; }
 
 static SkinnyPlan plan_for(Op ta, int m, int k, int nr)
 {
+    if (mfma_kernel(nr)) return plan(cdiv(m, 64), k, KCT);
     if (ta == Op::T) return plan(cdiv(m, 4), k, KCT);
-    return plan(cdiv(m, n4_kernel(nr) ? NROWS : 256), k, KCH);
+    return plan(cdiv(m, NROWS), k, KCH);
 }
 
 size_t skinny_batch_work_elems(Op ta, int m, int k, int count)
@@ -157,74 +162,6 @@ __device__ __forceinline__ void fused_reduce(const SkinnyIO& io, const double* w
 }
 
 // partial layout: work[((b * slabs + s) * NR + c) * m + i] for matrix b = blockIdx.z
-// N: one lane per row of op(A) = A (column reads coalesced), KU columns of A in flight
-template <int NR>
-__global__ __launch_bounds__(256) void skinny_n_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
-                                                       const double* __restrict__ A, int lda, SkinnyIO io,
-                                                       double beta, double* work, int direct)
-{
-    __shared__ double Rs[KCH][NR];
-    const int tid = threadIdx.x;
-    const int i = blockIdx.x * 256 + tid;
-    const int s = blockIdx.y;
-    {
-        const long long b = blockIdx.z;
-        A += b * io.sA;
-        io.R += b * io.sR;
-        io.Y0 += b * io.sY0;
-        io.Y += b * io.sY;
-        work += b * (long long)gridDim.y * NR * m;
-    }
-    const int kb = s * kslab, ke = min(k, kb + kslab);
-    double acc[NR];
-#pragma unroll
-    for (int c = 0; c < NR; c++) acc[c] = 0.0;
-    const int rowmax = min(m, (int)(blockIdx.x * 256 + 256)) - 1;
-    int kend = ke;
-    if (tri == 1) kend = min(kend, rowmax + 1);   // op(A) lower: A[i][kk] == 0 for kk > i
-    int kbeg = kb;
-    if (tri == 2) kbeg = max(kbeg, (int)(blockIdx.x * 256));   // op(A) upper
-    for (int k0 = kbeg; k0 < kend; k0 += KCH) {
-        const int kc = min(KCH, kend - k0);
-        __syncthreads();
-        for (int e = tid; e < KCH * NR; e += 256) {
-            const int kk = e % KCH, c = e / KCH;
-            Rs[kk][c] = (kk < kc && c < nr) ? r_at(io, k0 + kk, c) : 0.0;
-        }
-        __syncthreads();
-        if (i < m) {
-            const double* Ap = A + (long long)k0 * lda + i;
-            for (int kk = 0; kk < kc; kk += KU) {
-                double a[KU];
-#pragma unroll
-                for (int u = 0; u < KU; u++) a[u] = kk + u < kc ? Ap[(long long)(kk + u) * lda] : 0.0;
-#pragma unroll
-                for (int u = 0; u < KU; u++)
-#pragma unroll
-                    for (int c = 0; c < NR; c++) acc[c] = fma(a[u], Rs[min(kk + u, KCH - 1)][c], acc[c]);
-            }
-        }
-    }
-    if (i < m) {
-        if (direct) {
-#pragma unroll
-            for (int c = 0; c < NR; c++) {
-                if (c >= nr) break;
-                double v = alpha * acc[c];
-                if (beta != 0.0) v += beta * y0_at(io, i, c);
-                *y_at(io, i, c) = v;
-            }
-        } else if (!io.ctr) {
-#pragma unroll
-            for (int c = 0; c < NR; c++) work[((long long)s * NR + c) * m + i] = acc[c];
-        } else {
-#pragma unroll
-            for (int c = 0; c < NR; c++) st_sc1(work + ((long long)s * NR + c) * m + i, acc[c]);
-        }
-    }
-    if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * 256, min(256, m - (int)blockIdx.x * 256));
-}
-
 // N4 (one or two right-hand sides): 64 rows of op(A) = A per workgroup, each row by four lanes (one per wave), lane q
 // taking the q-th quarter of every KCH-wide k-chunk (column reads coalesced across the
 // wave, KU columns in flight per lane); the four partial sums of a row are added in a
@@ -304,14 +241,14 @@ __global__ __launch_bounds__(256) void skinny_n4_kernel(int m, int k, int nr, in
     if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * NROWS, min(NROWS, m - (int)blockIdx.x * NROWS));
 }
 
-// T: one wave per output row j of op(A) = A^T (column j of A read contiguously),
-// lanes over k with 4 loads in flight, wave-level reduction
+// T for one or two right-hand sides: one wave per output row j of op(A) = A^T (column j of A
+// read contiguously), lanes over k with KT1 loads of A in flight, each lane reading its own R
+// entries, wave-level reduction
 template <int NR>
-__global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
+__global__ __launch_bounds__(256) void skinny_t1_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
                                                        const double* __restrict__ A, int lda, SkinnyIO io,
                                                        double beta, double* work, int direct)
 {
-    __shared__ double Rs[NR][KCT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * 4 + wave;   // output row (column of A)
     const int s = blockIdx.y;
@@ -332,22 +269,21 @@ __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int
     if (tri == 2) kbeg = max(kbeg, (jmin / KCT) * KCT);   // op(A) upper: A^T[j][kk] == 0 for kk < j
     if (tri == 1) kend = min(kend, jmax + 1);             // op(A) lower
     const double* Aj = A + (long long)min(j, m - 1) * lda;
-    for (int k0 = kbeg; k0 < kend; k0 += KCT) {
-        const int kc = min(KCT, kend - k0);
-        __syncthreads();
-        for (int e = tid; e < KCT * NR; e += 256) {
-            const int kk = e % KCT, c = e / KCT;
-            Rs[c][kk] = (kk < kc && c < nr) ? r_at(io, k0 + kk, c) : 0.0;
-        }
-        __syncthreads();
-        if (j < m) {
-            double a[KCT / 64];
+    if (j < m) {
+        for (int k0 = kbeg; k0 < kend; k0 += 64 * KT1) {
+            double a[KT1], r[KT1][NR];
 #pragma unroll
-            for (int u = 0; u < KCT / 64; u++) a[u] = lane + 64 * u < kc ? Aj[k0 + lane + 64 * u] : 0.0;
+            for (int u = 0; u < KT1; u++) {
+                const int kk = k0 + lane + 64 * u;
+                const bool in = kk < kend;
+                a[u] = in ? Aj[kk] : 0.0;
 #pragma unroll
-            for (int u = 0; u < KCT / 64; u++)
+                for (int c = 0; c < NR; c++) r[u][c] = in && c < nr ? r_at(io, kk, c) : 0.0;
+            }
 #pragma unroll
-                for (int c = 0; c < NR; c++) acc[c] = fma(a[u], Rs[c][lane + 64 * u], acc[c]);
+            for (int u = 0; u < KT1; u++)
+#pragma unroll
+                for (int c = 0; c < NR; c++) acc[c] = fma(a[u], r[u][c], acc[c]);
         }
     }
 #pragma unroll
@@ -374,6 +310,73 @@ __global__ __launch_bounds__(256) void skinny_t_kernel(int m, int k, int nr, int
         }
     }
     if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * 4, min(4, m - (int)blockIdx.x * 4));
+}
+
+
+// Three to sixteen right-hand sides on FP64 MFMA (v_mfma_f64_16x16x4: 16 rows of op(A) x 4 k
+// times 4 k x 16 right-hand sides per instruction).  A workgroup takes 64 rows of op(A) (16
+// per wave) over a k-slab.  N: the A fragment (16 consecutive rows at 4 consecutive k) is
+// read straight from the column-major A; T: op(A) row j is column j of A, the fragment 16
+// columns at 4 consecutive k, each column's 128-byte line used by four successive steps.
+// R is read per lane (rows of R stay in cache: every workgroup reads them).  Partials and
+// the epilogue as in the other skinny kernels.
+constexpr int MKU = 8;   // k-steps of 4 with their loads in flight
+template <bool TRANS>
+__global__ __launch_bounds__(256) void skinny_mfma_kernel(int m, int k, int nr, int kslab, int tri, double alpha,
+                                                          const double* __restrict__ A, int lda, SkinnyIO io,
+                                                          double beta, double* work, int direct)
+{
+    constexpr int NR = MAXR;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int s = blockIdx.y;
+    {
+        const long long b = blockIdx.z;
+        A += b * io.sA;
+        io.R += b * io.sR;
+        io.Y0 += b * io.sY0;
+        io.Y += b * io.sY;
+        work += b * (long long)gridDim.y * NR * m;
+    }
+    const int r0 = blockIdx.x * 64 + 16 * wave;   // this wave's rows of op(A)
+    const int kb = s * kslab, ke = min(k, kb + kslab);
+    int kbeg = kb, kend = ke;
+    const int bmin = blockIdx.x * 64, bmax = min(m, bmin + 64) - 1;
+    if (tri == 1) kend = min(kend, bmax + 1);             // op(A) lower
+    if (tri == 2) kbeg = max(kbeg, bmin & ~3);            // op(A) upper
+    const int ra = min(r0 + l16, m - 1);                  // A-fragment row (clamped; rows >= m are not stored)
+    const double* Ap = TRANS ? A + (long long)ra * lda : A + ra;
+    const long long kst = TRANS ? 1 : lda;                // step of A along k
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kbeg; k0 < kend; k0 += 4 * MKU) {
+        double a[MKU], r[MKU];
+#pragma unroll
+        for (int u = 0; u < MKU; u++) {
+            const int kk = k0 + 4 * u + lq;
+            const bool in = kk < kend;
+            a[u] = in ? Ap[(long long)kk * kst] : 0.0;
+            r[u] = in && l16 < nr ? r_at(io, kk, l16) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < MKU; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], r[u], acc, 0, 0, 0);
+    }
+    // lane (l16, lq) item t: row r0 + 4 t + lq of op(A), right-hand side l16
+    const int c = l16;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int i = r0 + 4 * t + lq;
+        if (i >= m || c >= nr) continue;
+        if (direct) {
+            double v = alpha * acc[t];
+            if (beta != 0.0) v += beta * y0_at(io, i, c);
+            *y_at(io, i, c) = v;
+        } else if (!io.ctr) {
+            work[((long long)s * NR + c) * m + i] = acc[t];
+        } else {
+            st_sc1(work + ((long long)s * NR + c) * m + i, acc[t]);
+        }
+    }
+    if (!direct && io.ctr) fused_reduce<NR>(io, work, blockIdx.x * 64, min(64, m - (int)blockIdx.x * 64));
 }
 
 __global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, int count, double alpha,
@@ -409,7 +412,7 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
     {   // reduce the partials in the product launch when the counters cover its row blocks
         // (one workgroup per row block sums its slabs: only where that is a few partials per
         // thread -- one or two right-hand sides -- else the reduction launch spreads them out)
-        const int rpw = ta == Op::T ? 4 : (n4_kernel(nr) ? NROWS : 256);
+        const int rpw = mfma_kernel(nr) ? 64 : ta == Op::T ? 4 : NROWS;
         if (direct || !io.ctr || (long long)cdiv(m, rpw) * count > SKINNY_CTRS || (long long)rpw * nr > 256)
             io.ctr = nullptr;
         io.alpha = alpha;
@@ -418,19 +421,23 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
         io.nr = nr;
         io.slabs = p.slabs;
     }
-    if (ta == Op::N) {
-        if (n4_kernel(nr)) {
-            dim3 grid(cdiv(m, NROWS), p.slabs, count);
-            hipLaunchKernelGGL((skinny_n4_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda,
-                               io, beta, work, direct);
-        } else {
-            dim3 grid(cdiv(m, 256), p.slabs, count);
-            hipLaunchKernelGGL((skinny_n_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
-                               beta, work, direct);
-        }
+    if (mfma_kernel(nr)) {
+        // (NR == MAXR here: every nr > 2 takes the 16-column instantiation; the MFMA kernel
+        // always keeps 16 columns of partials)
+        dim3 grid(cdiv(m, 64), p.slabs, count);
+        if (ta == Op::N)
+            hipLaunchKernelGGL((skinny_mfma_kernel<false>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A,
+                               lda, io, beta, work, direct);
+        else
+            hipLaunchKernelGGL((skinny_mfma_kernel<true>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A,
+                               lda, io, beta, work, direct);
+    } else if (ta == Op::N) {
+        dim3 grid(cdiv(m, NROWS), p.slabs, count);
+        hipLaunchKernelGGL((skinny_n4_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
+                           beta, work, direct);
     } else {
         dim3 grid(cdiv(m, 4), p.slabs, count);
-        hipLaunchKernelGGL((skinny_t_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
+        hipLaunchKernelGGL((skinny_t1_kernel<NR>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A, lda, io,
                            beta, work, direct);
     }
     hipError_t e = hipGetLastError();
@@ -462,9 +469,7 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
     }
     if (nr == 1) return launch_skinny<1>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
     if (nr <= 2) return launch_skinny<2>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
-    if (nr <= 4) return launch_skinny<4>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
-    if (nr <= 8) return launch_skinny<8>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
-    return launch_skinny<16>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
+    return launch_skinny<MAXR>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);   // MFMA kernel
 }
 
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
@@ -480,7 +485,7 @@ size_t skinny_call_work(Op ta, int m, int k, int nr)
     if (m <= 0 || k <= 0 || nr <= 0) return 0;
     const SkinnyPlan p = plan_for(ta, m, k, nr);
     if (p.slabs == 1) return 0;
-    const int NR = nr == 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : nr <= 8 ? 8 : 16;   // launch_skinny's buckets
+    const int NR = nr == 1 ? 1 : nr <= 2 ? 2 : MAXR;   // launch_skinny's buckets
     return (size_t)p.slabs * NR * (size_t)m;
 }
 
