@@ -74,6 +74,38 @@ __device__ __forceinline__ void mfma_nt(v4d (&acc)[RA][2], const double (*X)[LP]
     }
 }
 
+// acc += X Y^T (the layout and sign of mfma_nt with +1) for lower-triangular Y: output column
+// block c0 + 16 b needs the k-steps below c0 + 16 b + 16 only (Y[c][k] = 0 for k > c); each
+// output entry gets mfma_nt's MFMA sequence with the all-zero steps left out.  K0: k-steps of
+// both column blocks of the wave, K1: of its second block.
+template <int RA, int K0, int K1>
+__device__ __forceinline__ void mfma_nt_lower_w(v4d (&acc)[RA][2], const double (*X)[LP], const double (*Y)[LP],
+                                                int wm, int wn, int l16, int lq)
+{
+#pragma unroll
+    for (int ks = 0; ks < K1; ks++) {
+        double af[RA], bf[2];
+#pragma unroll
+        for (int a = 0; a < RA; a++) af[a] = X[16 * RA * wm + 16 * a + l16][4 * ks + lq];
+#pragma unroll
+        for (int b = 0; b < 2; b++) bf[b] = Y[32 * wn + 16 * b + l16][4 * ks + lq];
+#pragma unroll
+        for (int a = 0; a < RA; a++) {
+            if (ks < K0) acc[a][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[0], af[a], acc[a][0], 0, 0, 0);
+            acc[a][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[1], af[a], acc[a][1], 0, 0, 0);
+        }
+    }
+}
+template <int RA>
+__device__ __forceinline__ void mfma_nt_lower(v4d (&acc)[RA][2], const double (*X)[LP], const double (*Y)[LP],
+                                              int wm, int wn, int l16, int lq)
+{
+    if (wn == 0)
+        mfma_nt_lower_w<RA, 4, 8>(acc, X, Y, wm, wn, l16, lq);
+    else
+        mfma_nt_lower_w<RA, 12, 16>(acc, X, Y, wm, wn, l16, lq);
+}
+
 // rows of the panel below the tile handled per workgroup pass (half a tile: the MFMA work
 // of a pass, on the launch's critical path, is half that of a 64-row block)
 constexpr int RB = 32;
@@ -785,7 +817,7 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
             const long long cl = wall_clock64();
             if (k >= 1) {
                 acc_zero(acc);
-                mfma_nt<DRA>(acc, Ba, Bx, 1.0, wm, wn, l16, lq);   // L_{k,k-1} = A_{k,k-1} X_{k-1}^T
+                mfma_nt_lower<DRA>(acc, Ba, Bx, wm, wn, l16, lq);   // L_{k,k-1} = A_{k,k-1} X_{k-1}^T (X lower)
                 acc_store(acc, A, lda, k0, k0 - NB, kb, NB, wm, wn, l16, lq);
                 __syncthreads();
                 acc_to(acc, Ba, wm, wn, l16, lq);
@@ -907,7 +939,7 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
                     __syncthreads();
                 }
                 acc_zero(acc);
-                mfma_nt<DRA>(acc, B3, B1, 1.0, wm, wn, l16, lq);
+                mfma_nt_lower<DRA>(acc, B3, B1, wm, wn, l16, lq);   // L_ig = A_ig X_g^T (X lower)
                 acc_store(acc, A, lda, NB * i, NB * g, rows, NB, wm, wn, l16, lq);
                 dag_release(ldone + i * T + g, 1, tid);
                 if (tid == 0) dag_trace(a, 1, i, g, g, w0, w1, wall_clock64());
