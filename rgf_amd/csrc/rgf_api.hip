@@ -1955,6 +1955,21 @@ extern "C" int rgf_debug_set_tuning(int key, int value)
         g_chol_dag_x = value;
         return old;
     }
+    if (key == 20 && value >= 0 && value <= 16) {
+        const int old = g_chol_dag_xlag;
+        g_chol_dag_xlag = value;
+        return old;
+    }
+    if (key == 21 && (value == 0 || value == 1)) {
+        const int old = g_chol_dag_promo;
+        g_chol_dag_promo = value;
+        return old;
+    }
+    if (key == 22 && value >= 64 && value <= 2048) {   // the work buffers are sized for targets up to 2048
+        const int old = g_skinny_target_wg;
+        g_skinny_target_wg = value;
+        return old;
+    }
     if (key == 9 && (value == 0 || value == 1)) {
         const int old = g_rr_bands;
         g_rr_bands = value;

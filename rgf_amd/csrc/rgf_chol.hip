@@ -515,21 +515,35 @@ constexpr int FL_TICKET = 0, FL_EXIT = 1, FL_ABORT = 2, FL_HDR = 4;
 // flags: header, xdone[T], then T x T arrays ldone, cnt, acnt (terms summed into Y_ij), xrdy (X_ij stored)
 __host__ __device__ inline int dag_flag_elems(int T) { return FL_HDR + T + 4 * T * T; }
 
-__host__ __device__ inline int dag_gen_tasks(int T, int g, bool with_x, int& ntr, int& nup, int& nxa)
+// Tasks of generation g in claim order: ntr TRSM(i, g); nup UPDATE(i, j, g - 1) (column j
+// ascending, rows i >= j ascending), without the first npo of them when they were promoted
+// into generation g - 1; npi promoted UPDATE(g + 2, g + 2, g) and UPDATE(g + 3, g + 2, g) --
+// the chain's needs at step g + 2 (its diagonal tile and the P3 operand A_{g+3,g+2}), one
+// generation before their own, right after their inputs (TRSM(., g) and the UPDATE(., ., g - 1)
+// of the same tiles) in the order; nxa XACC(i, j, k = g - 2 - xlag): the inverse's terms, off
+// the chain's path, lagging xlag generations so the chain's inputs are claimed first.
+struct DagGen {
+    int ntr, nup, npo, npi, nxa;
+    __host__ __device__ int total() const { return ntr + nup - npo + npi + nxa; }
+};
+__host__ __device__ inline DagGen dag_gen(int T, int g, bool with_x, int xlag, bool promo)
 {
-    ntr = T - g - 2 > 0 ? T - g - 2 : 0;
-    nup = g >= 1 && T - g - 1 > 0 ? (T - g - 1) * (T - g) / 2 : 0;
-    const int k = g - 2;
-    nxa = with_x && k >= 0 && k < T - 1 ? (k + 1) * (T - 1 - k) : 0;
-    return ntr + nup + nxa;
+    DagGen d;
+    d.ntr = T - g - 2 > 0 ? T - g - 2 : 0;
+    d.nup = g >= 1 && T - g - 1 > 0 ? (T - g - 1) * (T - g) / 2 : 0;
+    d.npo = promo && g >= 1 ? min(2, max(0, T - g - 1)) : 0;
+    d.npi = promo ? min(2, max(0, T - g - 2)) : 0;
+    const int k = g - 2 - xlag;
+    d.nxa = with_x && k >= 0 && k < T - 1 ? (k + 1) * (T - 1 - k) : 0;
+    return d;
 }
 
-__host__ __device__ inline int dag_gens(int T, bool with_x) { return with_x ? T + 1 : T - 1; }
+__host__ __device__ inline int dag_gens(int T, bool with_x, int xlag) { return with_x ? T + 1 + xlag : T - 1; }
 
-__host__ __device__ inline int dag_tasks(int T, bool with_x)
+__host__ __device__ inline int dag_tasks(int T, bool with_x, int xlag, bool promo)
 {
-    int total = 0, a, b, c;
-    for (int g = 0; g < dag_gens(T, with_x); g++) total += dag_gen_tasks(T, g, with_x, a, b, c);
+    int total = 0;
+    for (int g = 0; g < dag_gens(T, with_x, xlag); g++) total += dag_gen(T, g, with_x, xlag, promo).total();
     return total;
 }
 
@@ -550,6 +564,7 @@ struct DagArgs {
     double* xbuf;        // null, or the off-diagonal tiles of X = L^-1 (ld ldx, per matrix sXb)
     long long sXb;
     int ldx;
+    int xlag, promo;   // claim order (dag_gen)
 };
 
 // (thread 0, tracing) one record: workgroup, kind (0 chain step, 1 TRSM, 2 UPDATE, 3 the
@@ -879,15 +894,17 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
             int t = s_task;
             if (t < 0 || t >= a.ntasks) break;
             const long long w0 = wall_clock64();
-            int g = 0, ntr, nup, nxa;
-            while (t >= dag_gen_tasks(T, g, with_x, ntr, nup, nxa)) {
-                t -= ntr + nup + nxa;
-                g++;
+            int g = 0;
+            DagGen dg = dag_gen(T, g, with_x, a.xlag, a.promo);
+            while (t >= dg.total()) {
+                t -= dg.total();
+                dg = dag_gen(T, ++g, with_x, a.xlag, a.promo);
             }
-            if (t >= ntr + nup) {
-                // XACC(i, j, k = g - 2)
-                t -= ntr + nup;
-                const int k = g - 2, i = k + 1 + t / (k + 1), j = t % (k + 1);
+            const int ntr = dg.ntr, nupg = dg.nup - dg.npo;
+            if (t >= ntr + nupg + dg.npi) {
+                // XACC(i, j, k = g - 2 - xlag)
+                t -= ntr + nupg + dg.npi;
+                const int k = g - 2 - a.xlag, i = k + 1 + t / (k + 1), j = t % (k + 1);
                 const int ri = min(NB, n - NB * i);
                 if (!dag_acquire(a, fl, info, ldone + i * T + k, 1, k == j ? xdone + j : xrdy + k * T + j, 1,
                                  k > j ? acnt + i * T + j : nullptr, k - j, &s_ok, tid))
@@ -944,14 +961,24 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
                 dag_release(ldone + i * T + g, 1, tid);
                 if (tid == 0) dag_trace(a, 1, i, g, g, w0, w1, wall_clock64());
             } else {
-                // UPDATE(i, j, p = g - 1)
+                // UPDATE(i, j, p): of generation g (p = g - 1), or promoted into it (p = g)
                 t -= ntr;
-                int j = g + 1;
-                while (t >= T - j) {
-                    t -= T - j;
-                    j++;
+                int i, j, p;
+                if (t >= nupg) {
+                    t -= nupg;
+                    i = g + 2 + t;
+                    j = g + 2;
+                    p = g;
+                } else {
+                    t += dg.npo;
+                    j = g + 1;
+                    while (t >= T - j) {
+                        t -= T - j;
+                        j++;
+                    }
+                    i = j + t;
+                    p = g - 1;
                 }
-                const int i = j + t, p = g - 1;
                 const int ri = min(NB, n - NB * i), rj = min(NB, n - NB * j);
                 if (!dag_acquire(a, fl, info, ldone + i * T + p, 1, j != i ? ldone + j * T + p : nullptr, 1,
                                  p >= 1 ? cnt + i * T + j : nullptr, p, &s_ok, tid))
@@ -986,6 +1013,10 @@ int g_chol_dag = 1;         // 1: diagonal-block Cholesky as one task-graph laun
 int g_chol_dag_wgs = 256;   // its workgroups per launch (all matrices together)
 int g_chol_dag_min = 64;    // ... used only when each matrix gets at least this many
 int g_chol_dag_x = 1;       // 1: the task graph also forms L^-1 (where the caller gives a buffer)
+// claim order: the inverse's tasks two generations late and the chain's own updates one
+// early (n = 2865 Cholesky + L^-1 1498 -> 1453 us, c2 FP64 step -1 %; same bits)
+int g_chol_dag_xlag = 2;    // generations the inverse's tasks lag in the claim order
+int g_chol_dag_promo = 1;   // 1: the chain's two updates per step are claimed a generation early
 
 size_t potrf_dag_flag_elems(int n) { return (size_t)dag_flag_elems((n + NB - 1) / NB); }
 size_t potrf_dag_xbuf_elems(int n)
@@ -1087,8 +1118,9 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
         const int T = (n + NB - 1) / NB;
         const bool with_x = xbuf && x_done && g_chol_dag_x;
         DagArgs a{A, sA, xdiag, sX, logdet_parts, sP, info, dag_flags, (long long)dag_flag_elems(T),
-                  dag_timeout_ticks(), lda, n, T, dag_tasks(T, with_x), info_base, g_chol_dag_trace,
-                  g_chol_dag_trace_cap, with_x ? xbuf : nullptr, sXb, T * NB};
+                  dag_timeout_ticks(), lda, n, T, dag_tasks(T, with_x, g_chol_dag_xlag, g_chol_dag_promo != 0),
+                  info_base, g_chol_dag_trace, g_chol_dag_trace_cap, with_x ? xbuf : nullptr, sXb, T * NB,
+                  g_chol_dag_xlag, g_chol_dag_promo};
         // workers: enough to finish the tasks in the chain's time -- a task takes about a quarter of
         // a chain step (a 64 x 64 x 64 MFMA product and three tile loads against a tile factorization),
         // so about ntasks / (4 T) workgroups; the early generations hold most tasks and the chain

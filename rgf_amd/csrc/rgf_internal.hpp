@@ -264,7 +264,7 @@ bool potrf_dag_schedule(int schedule, int count);
 size_t potrf_dag_flag_elems(int n);
 size_t potrf_dag_xbuf_elems(int n);
 bool potrf_uses_dag(int count);   // a launch over `count` matrices takes the task-graph path
-extern int g_chol_dag, g_chol_dag_wgs, g_chol_dag_min, g_chol_dag_x;
+extern int g_chol_dag, g_chol_dag_wgs, g_chol_dag_min, g_chol_dag_x, g_chol_dag_xlag, g_chol_dag_promo;
 extern long long* g_chol_dag_trace;   // device buffer: [0] records, then 8 longs per task (tuning)
 extern long long g_chol_dag_trace_cap;
 // A product launcher supplied by the caller (precision policy of a handle).
@@ -328,6 +328,7 @@ int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double*
 // Partial doubles one call of this shape with nr right-hand sides actually writes (0 when
 // k is not split).
 size_t skinny_call_work(Op ta, int m, int k, int nr);
+extern int g_skinny_target_wg;
 // Same with element strides: R(kk, c) = R[kk * rsk + c * rsj], Y(i, c) = Y[i * ysi + c * ysj]
 // (Y0 likewise), e.g. row-major operands as transposed views, over `count` matrices at
 // strides sA, sR, sY0, sY in one launch; work holds work_elems doubles (>= count * 16 * m).

@@ -26,7 +26,11 @@ constexpr int KU = 8;      // A loads in flight per lane
 constexpr int NROWS = 64;  // rows of op(A) per workgroup of the N kernel
 constexpr int KCT = 256;   // k-chunk of the T kernel (4 per lane x 64 lanes)
 constexpr int KT1 = 8;     // A loads in flight per lane in the T kernel for one or two right-hand sides
-constexpr int TARGET_WG = 2048;
+// workgroups one product launch aims at (k split into slabs until there are about this many):
+// 512 measured against 256 ... 2048 at ns = 2865 -- one-RHS solve 6.71 -> 6.10 ms (fewer
+// partials to reduce, longer streams per workgroup), 16 RHS unchanged; 256 slows 16 RHS
+int g_skinny_target_wg = 512;
+constexpr int kMaxTargetWG = 2048;   // work buffers are sized for targets up to this
 
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
@@ -34,14 +38,14 @@ struct SkinnyPlan {
     int slabs, kslab;
 };
 
-// k is split into slabs (partials reduced afterwards) until one matrix's launch has
-// ~2048 workgroups: the products stream A once from HBM and need many loads in flight.
+// k is split into slabs (partials reduced afterwards) until one matrix's launch has about
+// `target` workgroups: the products stream A once from HBM and need many loads in flight.
 // The split depends on (m, k) alone -- not on the batch count or the caller's buffer --
 // so a matrix of a batch is summed in the same order as the same matrix alone (bitwise
 // equal results); at most 64 slabs, which skinny_work_elems provides for one matrix.
-static SkinnyPlan plan(int wgm, int k, int chunk)
+static SkinnyPlan plan(int wgm, int k, int chunk, int target)
 {
-    int slabs = std::max(1, std::min({cdiv(k, chunk), cdiv(TARGET_WG, (long long)wgm), 64}));
+    int slabs = std::max(1, std::min({cdiv(k, chunk), cdiv(target, (long long)wgm), 64}));
     int kslab = cdiv(cdiv(k, slabs), chunk) * chunk;
     slabs = cdiv(k, kslab);
     return {std::max(slabs, 1), kslab};
@@ -51,17 +55,18 @@ static SkinnyPlan plan(int wgm, int k, int chunk)
 // three or more right-hand sides: the MFMA kernel (64 rows of op(A) per workgroup)
 static inline bool mfma_kernel(int nr) { return nr > 2; }
 
-static SkinnyPlan plan_for(Op ta, int m, int k, int nr)
+static SkinnyPlan plan_for(Op ta, int m, int k, int nr, int target = g_skinny_target_wg)
 {
-    if (mfma_kernel(nr)) return plan(cdiv(m, 64), k, KCT);
-    if (ta == Op::T) return plan(cdiv(m, 4), k, KCT);
-    return plan(cdiv(m, NROWS), k, KCH);
+    if (mfma_kernel(nr)) return plan(cdiv(m, 64), k, KCT, target);
+    if (ta == Op::T) return plan(cdiv(m, 4), k, KCT, target);
+    return plan(cdiv(m, NROWS), k, KCH, target);
 }
 
 size_t skinny_batch_work_elems(Op ta, int m, int k, int count)
 {
-    // the most slabs any column count gives (fewer rows per workgroup never take more)
-    const SkinnyPlan p = plan_for(ta, m, k, MAXR);
+    // the most slabs any column count gives (the VALU kernels' partials -- at most twice the
+    // slabs of two columns -- take less than the MFMA kernel's sixteen columns)
+    const SkinnyPlan p = plan_for(ta, m, k, MAXR, kMaxTargetWG);
     return (size_t)count * (size_t)(p.slabs > 1 ? p.slabs : 1) * MAXR * (size_t)m;
 }
 

@@ -107,7 +107,8 @@ def test_tuning_switches_validate_without_device():
             assert lib.rgf_debug_set_tuning(key, v) == -1
         assert lib.rgf_debug_set_tuning(key, good[1]) == good[0]
         assert lib.rgf_debug_set_tuning(key, cur) == good[1]
-    for key, good, bad in ((16, (0, 1), (2, -1)),):
+    for key, good, bad in ((16, (0, 1), (2, -1)), (21, (1, 0), (2, -1)), (20, (2, 4), (17, -1)),
+                           (22, (512, 2048), (63, 2049))):
         cur = lib.rgf_debug_set_tuning(key, good[1])
         assert cur in good and lib.rgf_debug_set_tuning(key, bad[0]) == -1
         assert lib.rgf_debug_set_tuning(key, cur) == good[1]

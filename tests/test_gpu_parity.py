@@ -925,13 +925,13 @@ def test_graph_replays_match_eager(gpu, prec):
 
 # ------------------------------------------------------------------ variant switches keep the bits
 @pytest.mark.parametrize("prec", [rgf_amd.RGF_FP64, rgf_amd.RGF_MIXED_BF16X3])
-@pytest.mark.parametrize("key,alt", [(4, 0), (8, 0), (11, 0), (12, 0)])
+@pytest.mark.parametrize("key,alt", [(4, 0), (8, 0), (11, 0), (12, 0), (20, 0), (21, 0)])
 def test_sweep_variant_switches_keep_bits(gpu, prec, key, alt):
     """The adopted kernel variants change how the same products are loaded, dealt over
     the XCDs or launched together, not what is summed: logdet and diag(A^-1) of a batch and
     of a single matrix are bitwise those of the alternative (key 4 masked loads, 8 tile
     ranges for triangular products, 11 separate launches of X^T X and W^T Z, 12 W of the next
-    block in a launch of its own)."""
+    block in a launch of its own, 20 / 21 the task-graph Cholesky's claim order)."""
     say(f"switch {key}={alt} prec={prec}")
     lib = _lib.load()
     opts = rgf_amd.default_options(precision=prec)
