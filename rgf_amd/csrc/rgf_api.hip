@@ -1132,31 +1132,34 @@ int block_matvec_residual(rgf_handle_s* h, const SolveBufs& sb, int mb, const do
     int* const ct = sb.ctr;
     const Op N = Op::N, T = Op::T;
     int rc;
-    if ((rc = copy2d(b, n, r, n, n, nr, 1.0, s))) return rc;
+    // The products of r = b - A x are independent: each block-wide product runs over all
+    // blocks in one launch (batch index = block, panels at pstride), without k-slabs -- the
+    // blocks give the launch its workgroups -- and they accumulate into r in a fixed order:
+    // r_t = b_t - A_tt x_t; r_{t+1} -= A_{t+1,t} x_t; r_t -= A_{t+1,t}^T x_{t+1};
+    // r_t -= A_{d,t}^T x_d; r_d = b_d - A_d x (the arrow rows of all panels are one nd x nt ns
+    // matrix: panels are contiguous) - A_dd x_d.
+    const double* A0 = h->Ap + (long long)mb * h->mstride;
+    const long long ps = h->pstride;
     const double* xd = x + (long long)nt * ns;
     double* rd = r + (long long)nt * ns;
-    for (int t = 0; t < nt; t++) {
-        const double* A = h->Ap + (long long)mb * h->mstride + (long long)t * h->pstride;
-        const double* xt = x + (long long)t * ns;
-        double* rt = r + (long long)t * ns;
-        if (t < nt - 1) {
-            // [r_t; r_{t+1}; r_d] -= [A_tt; A_{t+1,t}; A_{d,t}] x_t (the whole panel, rows past 2 ns into r_d)
-            if ((rc = skinny_f64s(N, 0, 2 * ns + nd, ns, nr, -1.0, A, ldp, xt, 1, n, 1.0, rt, 1, n, rt, 1, n, w, we, s, 1,
-                                  0, 0, 0, 0, ct, nullptr, 0, nd ? rd : nullptr, 2 * ns)))
-                return rc;
-            // r_t -= [A_{t+1,t}; A_{d,t}]^T [x_{t+1}; x_d]
-            if ((rc = skinny_f64s(T, 0, ns, ns + nd, nr, -1.0, A + ns, ldp, xt + ns, 1, n, 1.0, rt, 1, n, rt, 1, n, w, we,
-                                  s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns)))
-                return rc;
-        } else {
-            if ((rc = skinny_f64(N, 0, ns, ns, nr, -1.0, A, ldp, xt, n, 1.0, rt, n, rt, n, w, s, we, ct))) return rc;
-            if (nd > 0) {
-                if ((rc = skinny_f64(T, 0, ns, nd, nr, -1.0, A + 2 * ns, ldp, xd, n, 1.0, rt, n, rt, n, w, s, we, ct)))
-                    return rc;
-                if ((rc = skinny_f64(N, 0, nd, ns, nr, -1.0, A + 2 * ns, ldp, xt, n, 1.0, rd, n, rd, n, w, s, we, ct)))
-                    return rc;
-            }
-        }
+    if ((rc = skinny_f64s(N, 0, ns, ns, nr, -1.0, A0, ldp, x, 1, n, 1.0, b, 1, n, r, 1, n, w, we, s, nt, ps, ns, ns, ns,
+                          ct, nullptr, 0, nullptr, 0, 1)))
+        return rc;
+    if (nt > 1) {
+        if ((rc = skinny_f64s(N, 0, ns, ns, nr, -1.0, A0 + ns, ldp, x, 1, n, 1.0, r + ns, 1, n, r + ns, 1, n, w, we, s,
+                              nt - 1, ps, ns, ns, ns, ct, nullptr, 0, nullptr, 0, 1)))
+            return rc;
+        if ((rc = skinny_f64s(T, 0, ns, ns, nr, -1.0, A0 + ns, ldp, x + ns, 1, n, 1.0, r, 1, n, r, 1, n, w, we, s,
+                              nt - 1, ps, ns, ns, ns, ct, nullptr, 0, nullptr, 0, 1)))
+            return rc;
+    }
+    if (nd > 0) {
+        if ((rc = skinny_f64s(T, 0, ns, nd, nr, -1.0, A0 + 2 * ns, ldp, xd, 1, n, 1.0, r, 1, n, r, 1, n, w, we, s, nt,
+                              ps, 0, ns, ns, ct, nullptr, 0, nullptr, 0, 1)))
+            return rc;
+        if ((rc = skinny_f64(N, 0, nd, nt * ns, nr, -1.0, A0 + 2 * ns, ldp, x, n, 1.0, b + (long long)nt * ns, n, rd,
+                             n, w, s, we, ct)))
+            return rc;
     }
     if (nd > 0 &&
         (rc = skinny_f64(N, 0, nd, nd, nr, -1.0, h->Add + (long long)mb * h->cstride, nd, xd, n, 1.0, rd, n, rd,

@@ -336,9 +336,11 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
                 long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
                 double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s,
                 int count = 1, long long sA = 0, long long sR = 0, long long sY0 = 0, long long sY = 0,
-                int* ctr = nullptr, const double* R2 = nullptr, int k1 = 0, double* Y2 = nullptr, int m1 = 0);
+                int* ctr = nullptr, const double* R2 = nullptr, int k1 = 0, double* Y2 = nullptr, int m1 = 0,
+                int slab_cap = 64);
 // (R2, k1): rows kk >= k1 of R read from R2 (row kk - k1, R's strides); (Y2, m1): rows
 // i >= m1 of Y (and of Y0) are those of Y2 (row i - m1, Y's strides).  Unbatched only.
+// slab_cap: at most this many k-slabs (1: no partials -- for launches over many blocks).
 
 // ---------------------------------------------------------------- generator / ingestion
 struct GenParams {

@@ -408,9 +408,15 @@ __global__ void skinny_reduce_kernel(int m, int nr, int wstride, int slabs, int 
 
 template <int NR>
 static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
-                         const SkinnyIO& io_in, double beta, double* work, size_t work_elems, int count, hipStream_t s)
+                         const SkinnyIO& io_in, double beta, double* work, size_t work_elems, int count, hipStream_t s,
+                         int slab_cap)
 {
-    const SkinnyPlan p = plan_for(ta, m, k, nr);
+    SkinnyPlan p = plan_for(ta, m, k, nr);
+    if (p.slabs > slab_cap) {   // fewer, longer slabs (a launch over many blocks has enough workgroups)
+        const int chunk = mfma_kernel(nr) || ta == Op::T ? KCT : KCH;
+        p.kslab = cdiv(cdiv(k, slab_cap), chunk) * chunk;
+        p.slabs = cdiv(k, p.kslab);
+    }
     const int direct = p.slabs == 1;
     if (!direct && (size_t)count * p.slabs * NR * m > work_elems) return (int)hipErrorInvalidValue;
     SkinnyIO io = io_in;
@@ -461,7 +467,7 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
                 long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
                 double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s, int count,
                 long long sA, long long sR, long long sY0, long long sY, int* ctr, const double* R2, int k1,
-                double* Y2, int m1)
+                double* Y2, int m1, int slab_cap)
 {
     if (m <= 0 || nr <= 0 || count <= 0) return 0;
     if (nr > MAXR) return (int)hipErrorInvalidValue;
@@ -472,9 +478,11 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
         if (count == 1 && ysi == 1 && y0si == 1) return copy2d(Y0, (int)y0sj, Y, (int)ysj, m, nr, beta, s);
         return (int)hipErrorInvalidValue;
     }
-    if (nr == 1) return launch_skinny<1>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
-    if (nr <= 2) return launch_skinny<2>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);
-    return launch_skinny<MAXR>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s);   // MFMA kernel
+    slab_cap = std::max(1, slab_cap);
+    if (nr == 1) return launch_skinny<1>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s, slab_cap);
+    if (nr <= 2) return launch_skinny<2>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s, slab_cap);
+    return launch_skinny<MAXR>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s,
+                               slab_cap);   // MFMA kernel
 }
 
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
