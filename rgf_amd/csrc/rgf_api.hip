@@ -373,6 +373,7 @@ struct SolveBufs {
     double* work;
     size_t work_elems;
     int* ctr;   // the skinny products' arrival counters (SKINNY_CTRS)
+    bool wide = false;   // the MFMA skinny kernel for every pass (a solve of three or more columns)
 };
 
 SolveBufs handle_bufs(rgf_handle_s* h) { return {h->Y, h->R, h->Dc, h->Bv, h->work, h->work_elems, h->sctr}; }
@@ -765,18 +766,18 @@ int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const d
         if (i > 0) {
             const double* Pm = l_panel(h, i - 1, mb);
             if ((rc = skinny_f64s(N, 0, ns + nd, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, 1, n, 1.0, ri,
-                                  1, n, ri, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nullptr, 0, nd ? xd : nullptr, ns)))
+                                  1, n, ri, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nullptr, 0, nd ? xd : nullptr, ns, 64, sb.wide)))
                 return rc;
         }
-        if ((rc = skinny_f64(N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we, ct))) return rc;
+        if ((rc = skinny_f64(N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we, ct, sb.wide))) return rc;
     }
     if (nd > 0) {
         const double* P = l_panel(h, nt - 1, mb);   // x_d -= D_{nt-1} y_{nt-1}
         if ((rc = skinny_f64(N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, Y + (long long)(nt - 1) * ns, n, 1.0, xd, n, xd,
-                             n, w, s, we, ct)))
+                             n, w, s, we, ct, sb.wide)))
             return rc;
-        if ((rc = skinny_f64(N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we, ct))) return rc;
-        if ((rc = skinny_f64(T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we, ct))) return rc;
+        if ((rc = skinny_f64(N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we, ct, sb.wide))) return rc;
+        if ((rc = skinny_f64(T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we, ct, sb.wide))) return rc;
     }
     // backward: y_i -= [B_i; D_i]^T [x_{i+1}; x_d] (one product over both row groups of the
     // panel), x_i = X_i^T y_i
@@ -786,13 +787,13 @@ int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const d
         double* xi = x + (long long)i * ns;
         if (i < nt - 1) {
             if ((rc = skinny_f64s(T, 0, ns, ns + nd, nr, -1.0, P + ns, ldp, x + (long long)(i + 1) * ns, 1, n, 1.0,
-                                  yi, 1, n, yi, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns)))
+                                  yi, 1, n, yi, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns, nullptr, 0, 64, sb.wide)))
                 return rc;
         } else if (nd > 0 &&
-                   (rc = skinny_f64(T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s, we, ct))) {
+                   (rc = skinny_f64(T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s, we, ct, sb.wide))) {
             return rc;
         }
-        if ((rc = skinny_f64(T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we, ct))) return rc;
+        if ((rc = skinny_f64(T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we, ct, sb.wide))) return rc;
     }
     return 0;
 }
@@ -876,7 +877,8 @@ int do_solve_device(rgf_handle_s* h, double* x, const double* b, size_t nrhs)
     if ((rc = ensure_solve_buffers(h))) return rc;
     const long long n = h->n;
     const bool refine = h->opt.precision != RGF_FP64;
-    const SolveBufs sb = handle_bufs(h);
+    SolveBufs sb = handle_bufs(h);
+    sb.wide = nrhs > 2;   // every pass on the MFMA kernel: a column's sums do not depend on its pass
     for (size_t c0 = 0; c0 < nrhs; c0 += 16) {
         const int nr = (int)std::min<size_t>(16, nrhs - c0);
         if (refine) {
@@ -1143,27 +1145,27 @@ int block_matvec_residual(rgf_handle_s* h, const SolveBufs& sb, int mb, const do
     const double* xd = x + (long long)nt * ns;
     double* rd = r + (long long)nt * ns;
     if ((rc = skinny_f64s(N, 0, ns, ns, nr, -1.0, A0, ldp, x, 1, n, 1.0, b, 1, n, r, 1, n, w, we, s, nt, ps, ns, ns, ns,
-                          ct, nullptr, 0, nullptr, 0, 1)))
+                          ct, nullptr, 0, nullptr, 0, 1, sb.wide)))
         return rc;
     if (nt > 1) {
         if ((rc = skinny_f64s(N, 0, ns, ns, nr, -1.0, A0 + ns, ldp, x, 1, n, 1.0, r + ns, 1, n, r + ns, 1, n, w, we, s,
-                              nt - 1, ps, ns, ns, ns, ct, nullptr, 0, nullptr, 0, 1)))
+                              nt - 1, ps, ns, ns, ns, ct, nullptr, 0, nullptr, 0, 1, sb.wide)))
             return rc;
         if ((rc = skinny_f64s(T, 0, ns, ns, nr, -1.0, A0 + ns, ldp, x + ns, 1, n, 1.0, r, 1, n, r, 1, n, w, we, s,
-                              nt - 1, ps, ns, ns, ns, ct, nullptr, 0, nullptr, 0, 1)))
+                              nt - 1, ps, ns, ns, ns, ct, nullptr, 0, nullptr, 0, 1, sb.wide)))
             return rc;
     }
     if (nd > 0) {
         if ((rc = skinny_f64s(T, 0, ns, nd, nr, -1.0, A0 + 2 * ns, ldp, xd, 1, n, 1.0, r, 1, n, r, 1, n, w, we, s, nt,
-                              ps, 0, ns, ns, ct, nullptr, 0, nullptr, 0, 1)))
+                              ps, 0, ns, ns, ct, nullptr, 0, nullptr, 0, 1, sb.wide)))
             return rc;
         if ((rc = skinny_f64(N, 0, nd, nt * ns, nr, -1.0, A0 + 2 * ns, ldp, x, n, 1.0, b + (long long)nt * ns, n, rd,
-                             n, w, s, we, ct)))
+                             n, w, s, we, ct, sb.wide)))
             return rc;
     }
     if (nd > 0 &&
         (rc = skinny_f64(N, 0, nd, nd, nr, -1.0, h->Add + (long long)mb * h->cstride, nd, xd, n, 1.0, rd, n, rd,
-                         n, w, s, we, ct)))
+                         n, w, s, we, ct, sb.wide)))
         return rc;
     return 0;
 }
@@ -1336,7 +1338,8 @@ int solve_device_noalloc(rgf_handle_s* h, double* x, const double* b, size_t nrh
     const long long n = h->n;
     const bool refine = h->opt.precision != RGF_FP64 && h->Ap && h->opt.refine_iters > 0;
     const bool in_place = x == b;
-    // FP64 columns are independent of their chunk (same products, same order), so the
+    // FP64 columns are independent of their chunk (same products, same order: a call of three
+    // or more columns runs every chunk on the MFMA kernel, as rgf_solve_device does), so the
     // chunk shrinks to what the scratch holds; a refined chunk keeps the 16 columns of
     // rgf_solve_device, whose refinement stops on the chunk's worst column
     int nr = (int)std::min<size_t>(16, nrhs);
@@ -1355,6 +1358,7 @@ int solve_device_noalloc(rgf_handle_s* h, double* x, const double* b, size_t nrh
     if (rc) return dev_status(h, rc, "scratch address");
     const size_t col = r32((size_t)n * nr);
     SolveBufs sb{};
+    sb.wide = nrhs > 2;   // the kernels of rgf_solve_device for this call, whatever the chunk
     sb.ctr = (int*)base;
     sb.Y = base + kScratchCtr;
     size_t off = kScratchCtr + col;
@@ -1618,7 +1622,8 @@ int rgf_solve(rgf_handle_t h, double* x, const double* b, size_t nrhs)
         const size_t bytes = (size_t)n * nr * sizeof(double);
         hipError_t e = hipMemcpyAsync(h->Bv, b + c0 * n, bytes, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) return dev_status(h, (int)e, "solve upload");
-        const SolveBufs sb = handle_bufs(h);
+        SolveBufs sb = handle_bufs(h);
+        sb.wide = nrhs > 2;
         rc = h->opt.precision != RGF_FP64
                  ? solve_refined(h, sb, h->cur, h->X, h->Bv, nr, false)
                  : run_piece(h, 0, h->cur, nr, h->X, h->Bv, [&] { return solve_block(h, sb, h->cur, h->X, h->Bv, nr); });

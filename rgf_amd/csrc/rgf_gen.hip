@@ -74,9 +74,26 @@ __global__ void diag_add_kernel(double* A, int lda, int n, double v)
     if (i < n) A[(long long)i * lda + i] += v;
 }
 
+// Abs-sums of the columns of the coupling blocks A_{t+1,t} (t < nt - 1) into rowsum[t ns + a]:
+// column a of A_{t+1,t} is row a of block (t, t+1).  One wave per column, lanes along its
+// rows (coalesced), a fixed-order wave reduction.
+__global__ __launch_bounds__(256) void coupling_colsum_kernel(const double* P, long long pstride, int ldp, int ns,
+                                                              double* rowsum)
+{
+    const int t = blockIdx.y, lane = threadIdx.x & 63;
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= ns) return;
+    const double* col = P + (long long)t * pstride + (long long)a * ldp + ns;
+    double s = 0.0;
+    for (int j = lane; j < ns; j += 64) s += fabs(col[j]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) rowsum[(long long)t * ns + a] = s;
+}
+
 // row abs-sums of the full symmetric matrix, excluding the diagonal.
-// One lane per row a of block t: diag block row (coalesced), coupling column a of
-// A_{t+1,t} (block (t,t+1) is its transpose), coupling row a of A_{t,t-1}, arrow column.
+// One lane per row a of block t (coalesced across the lanes): the coupling part of rowsum
+// (coupling_colsum_kernel) + diag block row + coupling row a of A_{t,t-1} + arrow column.
 __global__ void rowsum_block_kernel(const double* P, long long pstride, int ldp, int ns, int nt, int nd,
                                     double* rowsum)
 {
@@ -84,15 +101,13 @@ __global__ void rowsum_block_kernel(const double* P, long long pstride, int ldp,
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= ns) return;
     const double* Pt = P + (long long)t * pstride;
-    double s = 0.0;
+    double s = t < nt - 1 ? rowsum[(long long)t * ns + a] : 0.0;
+#pragma unroll 8
     for (int j = 0; j < ns; j++)
         if (j != a) s += fabs(Pt[(long long)j * ldp + a]);
-    if (t < nt - 1) {
-        const double* col = Pt + (long long)a * ldp + ns;
-        for (int j = 0; j < ns; j++) s += fabs(col[j]);
-    }
     if (t > 0) {
         const double* Pm = P + (long long)(t - 1) * pstride + ns;
+#pragma unroll 8
         for (int j = 0; j < ns; j++) s += fabs(Pm[(long long)j * ldp + a]);
     }
     const double* ar = Pt + (long long)a * ldp + 2 * ns;
@@ -181,6 +196,11 @@ int generate_matrix(const GenParams& g, double* P, long long pstride, int ldp, d
         if ((rc = gemm_f64(Op::N, Op::T, &d, 1, s))) return rc;
         if ((rc = mirror_lower(Cdd, nd, nd, s))) return rc;
         hipLaunchKernelGGL(diag_add_kernel, dim3(1), dim3(256), 0, s, Cdd, nd, nd, (double)nd);
+        if ((rc = (int)hipGetLastError())) return rc;
+    }
+    if (nt > 1) {
+        hipLaunchKernelGGL(coupling_colsum_kernel, dim3((ns + 3) / 4, nt - 1), dim3(256), 0, s, P, pstride, ldp, ns,
+                           rowsum);
         if ((rc = (int)hipGetLastError())) return rc;
     }
     hipLaunchKernelGGL(rowsum_block_kernel, dim3((ns + 255) / 256, nt), dim3(256), 0, s, P, pstride, ldp, ns, nt,

@@ -324,7 +324,7 @@ size_t skinny_batch_work_elems(Op ta, int m, int k, int count);
 constexpr int SKINNY_CTRS = 16384;
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
-               double* work, hipStream_t s, size_t work_elems = 0, int* ctr = nullptr);
+               double* work, hipStream_t s, size_t work_elems = 0, int* ctr = nullptr, bool wide = false);
 // Partial doubles one call of this shape with nr right-hand sides actually writes (0 when
 // k is not split).
 size_t skinny_call_work(Op ta, int m, int k, int nr);
@@ -337,10 +337,12 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
                 double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s,
                 int count = 1, long long sA = 0, long long sR = 0, long long sY0 = 0, long long sY = 0,
                 int* ctr = nullptr, const double* R2 = nullptr, int k1 = 0, double* Y2 = nullptr, int m1 = 0,
-                int slab_cap = 64);
+                int slab_cap = 64, bool wide = false);
 // (R2, k1): rows kk >= k1 of R read from R2 (row kk - k1, R's strides); (Y2, m1): rows
 // i >= m1 of Y (and of Y0) are those of Y2 (row i - m1, Y's strides).  Unbatched only.
 // slab_cap: at most this many k-slabs (1: no partials -- for launches over many blocks).
+// wide: the MFMA kernel (that of three or more columns) for any column count, so a column
+// is summed the same way whatever the size of the pass that carries it.
 
 // ---------------------------------------------------------------- generator / ingestion
 struct GenParams {

@@ -411,9 +411,10 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
                          const SkinnyIO& io_in, double beta, double* work, size_t work_elems, int count, hipStream_t s,
                          int slab_cap)
 {
-    SkinnyPlan p = plan_for(ta, m, k, nr);
+    constexpr bool mf = NR == MAXR;   // the MFMA kernel (for any nr: 16 columns of partials)
+    SkinnyPlan p = plan_for(ta, m, k, mf ? MAXR : nr);
     if (p.slabs > slab_cap) {   // fewer, longer slabs (a launch over many blocks has enough workgroups)
-        const int chunk = mfma_kernel(nr) || ta == Op::T ? KCT : KCH;
+        const int chunk = mf || ta == Op::T ? KCT : KCH;
         p.kslab = cdiv(cdiv(k, slab_cap), chunk) * chunk;
         p.slabs = cdiv(k, p.kslab);
     }
@@ -423,7 +424,7 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
     {   // reduce the partials in the product launch when the counters cover its row blocks
         // (one workgroup per row block sums its slabs: only where that is a few partials per
         // thread -- one or two right-hand sides -- else the reduction launch spreads them out)
-        const int rpw = mfma_kernel(nr) ? 64 : ta == Op::T ? 4 : NROWS;
+        const int rpw = mf ? 64 : ta == Op::T ? 4 : NROWS;
         if (direct || !io.ctr || (long long)cdiv(m, rpw) * count > SKINNY_CTRS || (long long)rpw * nr > 256)
             io.ctr = nullptr;
         io.alpha = alpha;
@@ -432,9 +433,7 @@ static int launch_skinny(Op ta, int tri, int m, int k, int nr, double alpha, con
         io.nr = nr;
         io.slabs = p.slabs;
     }
-    if (mfma_kernel(nr)) {
-        // (NR == MAXR here: every nr > 2 takes the 16-column instantiation; the MFMA kernel
-        // always keeps 16 columns of partials)
+    if constexpr (mf) {
         dim3 grid(cdiv(m, 64), p.slabs, count);
         if (ta == Op::N)
             hipLaunchKernelGGL((skinny_mfma_kernel<false>), grid, dim3(256), 0, s, m, k, nr, p.kslab, tri, alpha, A,
@@ -467,7 +466,7 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
                 long long rsk, long long rsj, double beta, const double* Y0, long long y0si, long long y0sj,
                 double* Y, long long ysi, long long ysj, double* work, size_t work_elems, hipStream_t s, int count,
                 long long sA, long long sR, long long sY0, long long sY, int* ctr, const double* R2, int k1,
-                double* Y2, int m1, int slab_cap)
+                double* Y2, int m1, int slab_cap, bool wide)
 {
     if (m <= 0 || nr <= 0 || count <= 0) return 0;
     if (nr > MAXR) return (int)hipErrorInvalidValue;
@@ -479,6 +478,8 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
         return (int)hipErrorInvalidValue;
     }
     slab_cap = std::max(1, slab_cap);
+    if (wide) return launch_skinny<MAXR>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s,
+                                         slab_cap);
     if (nr == 1) return launch_skinny<1>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s, slab_cap);
     if (nr <= 2) return launch_skinny<2>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s, slab_cap);
     return launch_skinny<MAXR>(ta, tri, m, k, nr, alpha, A, lda, io, beta, work, work_elems, count, s,
@@ -487,10 +488,11 @@ int skinny_f64s(Op ta, int tri, int m, int k, int nr, double alpha, const double
 
 int skinny_f64(Op ta, int tri, int m, int k, int nr, double alpha, const double* A, int lda,
                const double* R, int ldr, double beta, const double* Y0, int ldy0, double* Y, int ldy,
-               double* work, hipStream_t s, size_t work_elems, int* ctr)
+               double* work, hipStream_t s, size_t work_elems, int* ctr, bool wide)
 {
     return skinny_f64s(ta, tri, m, k, nr, alpha, A, lda, R, 1, ldr, beta, Y0, 1, ldy0, Y, 1, ldy, work,
-                       work_elems ? work_elems : skinny_work_elems(m, k, nr), s, 1, 0, 0, 0, 0, ctr);
+                       work_elems ? work_elems : skinny_work_elems(m, k, nr), s, 1, 0, 0, 0, 0, ctr, nullptr, 0,
+                       nullptr, 0, 64, wide);
 }
 
 size_t skinny_call_work(Op ta, int m, int k, int nr)
