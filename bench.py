@@ -303,7 +303,9 @@ def main():
         s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
                                   options=rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3))
     else:
-        s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=rgf_amd.default_options(graphs=main_graphs))
+        # keep_w: the selected inversion's W_i stay for the solves after it (no extra work in the step)
+        s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
+                                  options=rgf_amd.default_options(graphs=main_graphs, keep_w=1))
     s.synchronize()
     t_gen = time.perf_counter() - t_gen
     diag = rgf_amd.DeviceBuffer(s.n)
@@ -358,10 +360,18 @@ def main():
     b.upload(np.random.default_rng(1).standard_normal(s.n * 16))
     s.solve_device(x, b, 1)
     s.synchronize()
+    # solves after the selected inversion: through the kept W_i (one dependent product per block
+    # and direction); then after a new factorization, the block solve (two per block)
     t = time.perf_counter(); s.solve_device(x, b, 1); s.synchronize(); ph["solve_1rhs_ms"] = (time.perf_counter() - t) * 1e3
     t = time.perf_counter(); s.solve_device(x, b, 16); s.synchronize(); ph["solve_16rhs_ms"] = (time.perf_counter() - t) * 1e3
     info = s.info()
     ph["solve_1rhs_GBps"] = info.bytes_solve_per_rhs / (ph["solve_1rhs_ms"] * 1e-3) / 1e9
+    s.factorize()
+    s.solve_device(x, b, 1)
+    s.synchronize()
+    t = time.perf_counter(); s.solve_device(x, b, 1); s.synchronize(); ph["solve_1rhs_block_ms"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter(); s.solve_device(x, b, 16); s.synchronize(); ph["solve_16rhs_block_ms"] = (time.perf_counter() - t) * 1e3
+    ph["solve_1rhs_block_GBps"] = info.bytes_solve_per_rhs / (ph["solve_1rhs_block_ms"] * 1e-3) / 1e9
     ph["factorize_TFLOPs"] = F / (ph["factorize_ms"] * 1e-3) / 1e12
     ph["selinv_TFLOPs"] = S / (ph["selinv_ms"] * 1e-3) / 1e12
     # canonical F and S put the triangular inverses (trtri, nt ns^3/3 + nd^3/3) in the
@@ -371,6 +381,8 @@ def main():
     ph["factorize_TFLOPs_as_run"] = (F + trtri) / (ph["factorize_ms"] * 1e-3) / 1e12
     ph["selinv_TFLOPs_as_run"] = (S - trtri) / (ph["selinv_ms"] * 1e-3) / 1e12
     ph = {k: round(v, 4) for k, v in ph.items()}
+    ph["solve_path"] = ("solve_*: after the selected inversion, through its kept W_i (keep_w=1); "
+                        "solve_*_block_*: after a new factorization, the block solve")
     x64 = x.download()
     fp64_diag = diag.download()
     s.close()
@@ -379,7 +391,7 @@ def main():
     # the same FP64 steps replayed from HIP graphs
     graph_replay = None
     if graphs and not main_graphs:
-        g = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=rgf_amd.default_options(graphs=1))
+        g = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed, options=rgf_amd.default_options(graphs=1, keep_w=1))
         gd = rgf_amd.DeviceBuffer(g.n)
         for _ in range(args.warmup):
             g.factorize(); g.logDet(); g.selinv_diag_device(gd)

@@ -66,6 +66,14 @@ typedef struct {
     int cholesky;       /* rgf_cholesky_schedule.  The two schedules sum the same terms in different
                            orders: results agree to rounding, and bitwise between handles that run
                            the same schedule (a matrix of a batch and the same matrix alone) */
+    int keep_w;         /* 1: keep the selected inversion's W_i = [B_i; D_i] L_ii^-1 of every block
+                           (nt (ns + nd) ns doubles per matrix, 6.6 GB at ns = 2865, nt = 100, nd = 6);
+                           solves after a selected inversion of the current factor then chain one
+                           product per block and direction, r_{i+1} = b_{i+1} - W_i r_i and
+                           x_i = X_i^T y_i - W_i^T [x_{i+1}; x_d], with the X_i products of all blocks
+                           in one launch each way.  The same solution to rounding.  FP64 handles
+                           only (ignored with mixed precision: through its split-BF16 W_i a refined
+                           solve needs a second refinement step); 0 by default */
 } rgf_options;
 
 typedef struct {

@@ -44,7 +44,7 @@ TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rg
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("selinv_mode", C.c_int), ("keep_matrix", C.c_int),
                 ("device", C.c_int), ("refine_iters", C.c_int), ("refine_tol", C.c_double),
-                ("graphs", C.c_int), ("keep_splits", C.c_int), ("cholesky", C.c_int)]
+                ("graphs", C.c_int), ("keep_splits", C.c_int), ("cholesky", C.c_int), ("keep_w", C.c_int)]
 
 
 class GenParams(C.Structure):

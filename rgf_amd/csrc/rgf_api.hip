@@ -116,6 +116,8 @@ struct rgf_handle_s {
     __bf16* xr = nullptr;       // split copy of X_i's rows (the current block's)
     __bf16* xc = nullptr;       // split copies of X_i's columns: one per block with keep_splits
     bool factorized = false;
+    double* Wall = nullptr;     // keep_w: W_i of every block (slot i of matrix b at (b nt + i) fstride)
+    bool w_valid = false;       // Wall holds the W_i of the current factor (a selected inversion ran)
     bool broken = false;        // in-place factorization failed: A is gone
     bool generated = false;
     rgf_gen_params gen{};
@@ -267,7 +269,8 @@ size_t solve_work_elems(int ns, int nt, int nd, int count)
     return std::max(skinny_work_elems(mx, mx, 16), skinny_batch_work_elems(Op::N, ns, ns, count));
 }
 
-size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision, int keep_splits, int cholesky)
+size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precision, int keep_splits, int cholesky,
+                       int keep_w)
 {
     const size_t ldp = (size_t)r8(2LL * ns + nd), ldf = (size_t)r8((long long)ns + nd);
     const size_t T = (size_t)(ns + 63) / 64;
@@ -280,6 +283,7 @@ size_t footprint_bytes(int ns, int nt, int nd, int count, int keep, int precisio
     size_t per = ldp * ns * nt * (keep ? 2 : 1) + cnd * (keep ? 3 : 2) + ldf * ns + tt + T * 4096 + nt * T + 1 +
                  (size_t)std::max(nt * nd * nd, 1);
     per += 2 * ldf * (ns + nd) + 2 * ldf * ns + (size_t)ns * nt + nd;   // selected inversion
+    if (keep_w && precision == RGF_FP64) per += ldf * ns * nt;              // W_i of every block
     // solve buffers (Y, X, Bv, R, Dc: n x 16 each) and the skinny products' partials
     size_t once = 5 * ((size_t)ns * nt + nd) * 16 + solve_work_elems(ns, nt, nd, count) + SKINNY_CTRS / 2;
     size_t bytes = (per * count + once) * sizeof(double) + potrf_dag_flag_elems(ns) * count * sizeof(int);
@@ -388,6 +392,10 @@ int ensure_selinv_buffers(rgf_handle_s* h)
     if ((rc = dalloc(h, &h->W, (size_t)h->fstride * B))) return rc;
     if ((rc = dalloc(h, &h->Z, (size_t)h->fstride * B))) return rc;
     if ((rc = dalloc(h, &h->diag, (size_t)h->n * B))) return rc;
+    // (FP64 only: through the split-BF16 W_i of the mixed path a refined solve took two
+    // refinement steps instead of one, c2 1 RHS 21.0 -> 27.2 ms)
+    if (h->opt.keep_w && h->opt.precision == RGF_FP64 && (rc = dalloc(h, &h->Wall, (size_t)h->fstride * h->nt * B)))
+        return rc;
     if (h->opt.precision == RGF_MIXED_BF16X3) {
         if ((rc = dalloc(h, &h->split_x, split_x_need(h->ns, h->nd, B)))) return rc;
         h->split_s = h->split_x + (size_t)B * split_entry_elems(h->ns, h->ns);
@@ -412,6 +420,11 @@ rgf_handle_s* new_handle(size_t ns, size_t nt, size_t nd, size_t count, const rg
     h->device = h->opt.device;
     if (h->opt.precision != RGF_FP64 && h->opt.precision != RGF_MIXED_BF16X3) {
         *rc = set_err(nullptr, RGF_EINVAL, "precision mode %d is not available in this build", h->opt.precision);
+        delete h;
+        return nullptr;
+    }
+    if (h->opt.keep_w != 0 && h->opt.keep_w != 1) {
+        *rc = set_err(nullptr, RGF_EINVAL, "keep_w must be 0 or 1 (got %d)", h->opt.keep_w);
         delete h;
         return nullptr;
     }
@@ -673,6 +686,7 @@ template <class Fn>
 int run_piece(rgf_handle_s* h, int kind, int mb, int nr, const void* x, const void* b, Fn&& fn)
 {
     if (!h->opt.graphs) return fn();
+    if (h->w_valid && h->Wall) kind += 100;   // a solve through the kept W_i is another sequence
     auto& v = h->gx_pieces;
     size_t i = 0;
     while (i < v.size() && !(v[i].kind == kind && v[i].mb == mb && v[i].nr == nr && v[i].x == x && v[i].b == b)) i++;
@@ -698,6 +712,7 @@ int do_factorize(rgf_handle_s* h)
     hipSetDevice(h->device);
     int rc;
     if (h->nd > 0 && h->nt > 1 && (rc = ensure_solve_buffers(h))) return rc;   // arrow products use `work`
+    h->w_valid = false;   // the kept W_i belong to the factor being replaced
     rc = run_captured(h, h->gx_factor, h->gx_epoch_factor, [h] { return factor_sweep(h); });
     if (rc) return dev_status(h, rc, "factorize");
     return check_factor_info(h);
@@ -757,6 +772,51 @@ int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const d
     if (x != b && (rc = copy2d(b, n, x, n, n, nr, 1.0, s))) return rc;   // x holds r during the forward pass
     double* xd = x + (long long)nt * ns;
     double* yd = Y + (long long)nt * ns;
+    if (h->w_valid && h->Wall) {
+        // With the selected inversion's W_i = [B_i; D_i] X_i kept: r_i = b_i - B_{i-1} y_{i-1} =
+        // b_i - W_{i-1} r_{i-1} (and x_d -= W_i's arrow rows r_i) is a chain of one product per
+        // block; y_i = X_i r_i for all blocks in one launch; after the corner, x_i = X_i^T y_i for
+        // all blocks in one launch, then x_i -= W_i^T [x_{i+1}; x_d] block by block
+        const long long fs = h->fstride;
+        const double* const W0 = h->Wall + (long long)mb * nt * fs;
+        const int ldw = h->ldf;
+        const long long ps = h->pstride;
+        const double* const P0 = l_panel(h, 0, mb);
+        for (int i = 1; i < nt; i++) {
+            double* ri = x + (long long)i * ns;
+            if ((rc = skinny_f64s(N, 0, ns + nd, ns, nr, -1.0, W0 + (long long)(i - 1) * fs, ldw,
+                                  x + (long long)(i - 1) * ns, 1, n, 1.0, ri, 1, n, ri, 1, n, w, we, s, 1, 0, 0, 0, 0,
+                                  ct, nullptr, 0, nd ? xd : nullptr, ns, 64, sb.wide)))
+                return rc;
+        }
+        if (nd > 0 && (rc = skinny_f64(N, 0, nd, ns, nr, -1.0, W0 + (long long)(nt - 1) * fs, ldw,
+                                       x + (long long)(nt - 1) * ns, n, 1.0, xd, n, xd, n, w, s, we, ct, sb.wide)))
+            return rc;
+        if ((rc = skinny_f64s(N, 1, ns, ns, nr, 1.0, P0, ldp, x, 1, n, 0.0, Y, 1, n, Y, 1, n, w, we, s, nt, ps, ns,
+                              ns, ns, ct, nullptr, 0, nullptr, 0, 1, sb.wide)))
+            return rc;
+        if (nd > 0) {
+            if ((rc = skinny_f64(N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we, ct, sb.wide)))
+                return rc;
+            if ((rc = skinny_f64(T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we, ct, sb.wide)))
+                return rc;
+        }
+        if ((rc = skinny_f64s(T, 2, ns, ns, nr, 1.0, P0, ldp, Y, 1, n, 0.0, x, 1, n, x, 1, n, w, we, s, nt, ps, ns,
+                              ns, ns, ct, nullptr, 0, nullptr, 0, 1, sb.wide)))
+            return rc;
+        double* const xl = x + (long long)(nt - 1) * ns;
+        if (nd > 0 && (rc = skinny_f64(T, 0, ns, nd, nr, -1.0, W0 + (long long)(nt - 1) * fs, ldw, xd, n, 1.0, xl,
+                                       n, xl, n, w, s, we, ct, sb.wide)))
+            return rc;
+        for (int i = nt - 2; i >= 0; i--) {
+            double* xi = x + (long long)i * ns;
+            if ((rc = skinny_f64s(T, 0, ns, ns + nd, nr, -1.0, W0 + (long long)i * fs, ldw, x + (long long)(i + 1) * ns,
+                                  1, n, 1.0, xi, 1, n, xi, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns,
+                                  nullptr, 0, 64, sb.wide)))
+                return rc;
+        }
+        return 0;
+    }
     // forward: [r_i; x_d] -= [B_{i-1}; D_{i-1}] y_{i-1} (the coupling and arrow rows of panel
     // i - 1 are contiguous: one product, its rows past ns accumulate into x_d), y_i = X_i r_i
     for (int i = 0; i < nt; i++) {
@@ -966,12 +1026,16 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
     // between h->W and Ftmp (idle outside the factor sweep)
     const bool pipe = h->opt.precision == RGF_FP64 && g_selinv_fuse && g_selinv_pipe;
     double* const wb[2] = {h->W, h->Ftmp};
+    // keep_w: W_j in its own slot of Wall (kept for the solves), matrices nt slots apart
+    const bool keepw = h->Wall != nullptr;
+    const long long wst = keepw ? (long long)nt * fs : fs;
+    auto wptr = [&](int j) { return keepw ? h->Wall + (long long)j * fs : pipe ? wb[j & 1] : h->W; };
     auto wdesc = [&](int j) {
         const double* Pj = l_panel(h, j);
         const bool lj = j == nt - 1;
-        GemmDesc d = gdesc(lj ? Pj + 2 * ns : Pj + ns, ldp, Pj, ldp, wb[j & 1], lds, lj ? nd : m, ns, ns, 1.0, 0.0,
+        GemmDesc d = gdesc(lj ? Pj + 2 * ns : Pj + ns, ldp, Pj, ldp, wptr(j), lds, lj ? nd : m, ns, ns, 1.0, 0.0,
                            G_BLOWER);
-        batch2(d, B, ms, ms, fs);
+        batch2(d, B, ms, ms, wst);
         return d;
     };
     if (pipe && nd > 0) {
@@ -998,19 +1062,19 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             const PresplitCall c{dx, xv, xv, SplitView{}, SplitView{}};
             return gemm_bf16x3_presplit_n(Op::T, Op::N, &c, 1, s);
         };
+        double* const Wi = wptr(i);
         if (pipe) {
-            double* const Wi = wb[i & 1];
             GemmDesc zz[2];
             int nz = 0;
             if (mF > 0) {
                 zz[nz] = gdesc(Sc + (long long)r0 * lds + r0, lds, Wi, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
-                batch2(zz[nz++], B, ss, fs, fs);
+                batch2(zz[nz++], B, ss, wst, fs);
             }
             if (i > 0) zz[nz++] = wdesc(i - 1);   // into wb[(i - 1) & 1], free since W^T Z of block i + 1
             if (nz && (rc = prod(h, Op::N, Op::N, zz, nz))) return rc;
             if (mF > 0) {
                 GemmDesc d = gdesc(Wi, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, 0.0, G_CLOWER);
-                batch2(d, B, fs, fs, ss);
+                batch2(d, B, wst, fs, ss);
                 GemmDesc two[2] = {dx, d};
                 two[0].C = Sc;
                 two[0].Cin = Sc;
@@ -1020,8 +1084,8 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
                 return rc;
             }
         } else if (mF > 0) {
-            GemmDesc d = gdesc(F, ldp, P, ldp, h->W, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
-            batch2(d, B, ms, ms, fs);
+            GemmDesc d = gdesc(F, ldp, P, ldp, Wi, lds, mF, ns, ns, 1.0, 0.0, G_BLOWER);
+            batch2(d, B, ms, ms, wst);
             const bool pw = px && presplit_ok(h, d);
             if (pw) {   // op(A) = [B_i; D_i]: the factorization's copy (keep_splits) or split now;
                         // op(B) = X: the split of X^T X's op(A); the product writes the split of W
@@ -1035,8 +1099,8 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             } else if ((rc = prod(h, Op::N, Op::N, &d, 1))) {
                 return rc;
             }
-            d = gdesc(Sc + (long long)r0 * lds + r0, lds, h->W, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
-            batch2(d, B, ss, fs, fs);
+            d = gdesc(Sc + (long long)r0 * lds + r0, lds, Wi, lds, h->Z, lds, mF, ns, mF, 1.0, 0.0, 0);
+            batch2(d, B, ss, wst, fs);
             if (pw) {   // op(A) = S_c: its copy from the step before (or split now); op(B) = W from
                         // the W product; the product writes the split of Z
                 const bool kept = sc_copy && r0 == 0 && mF == m;
@@ -1055,8 +1119,8 @@ int selinv_sweep(rgf_handle_s* h, double* diag_out, const SigmaOut* so = nullptr
             } else if (!fuse && (rc = prod(h, Op::T, Op::N, &dx, 1))) {
                 return rc;
             }
-            d = gdesc(h->W, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, fuse ? 0.0 : 1.0, G_CLOWER);
-            batch2(d, B, fs, fs, ss);
+            d = gdesc(Wi, lds, h->Z, lds, Sn, lds, ns, ns, mF, 1.0, fuse ? 0.0 : 1.0, G_CLOWER);
+            batch2(d, B, wst, fs, ss);
             if (pw) {   // op(A) = W^T: the split of Z = Sb W's op(B); op(B) = Z from the Z product
                 if ((rc = gemm_bf16x3_presplit(Op::T, Op::N, &d, rW, rZ, s))) return rc;
             } else if (fuse) {
@@ -1105,6 +1169,7 @@ int do_selinv_device(rgf_handle_s* h, double* diag_dev)
                                      h->stream);
     }
     if (rc) return dev_status(h, rc, "selinv");
+    h->w_valid = h->Wall != nullptr;
     return RGF_OK;
 }
 
@@ -1250,6 +1315,7 @@ int generate_all(rgf_handle_s* h)
     cache_free(h->device, rows, gr);
     if (e != hipSuccess || krc) return dev_status(h, e != hipSuccess ? (int)e : krc, "generate");
     h->factorized = false;
+    h->w_valid = false;
     h->broken = false;
     return RGF_OK;
 }
@@ -1404,6 +1470,7 @@ void rgf_default_options(rgf_options* o)
     o->refine_tol = 1e-14;
     o->graphs = 0;
     o->keep_splits = 1;
+    o->keep_w = 0;
     o->cholesky = RGF_CHOLESKY_AUTO;
 }
 
@@ -1503,7 +1570,7 @@ size_t rgf_footprint_bytes(size_t ns, size_t nt, size_t nd, size_t count, const 
     rgf_default_options(&o);
     if (opts) o = *opts;
     return footprint_bytes((int)ns, (int)nt, (int)nd, (int)count, o.keep_matrix, o.precision, o.keep_splits,
-                           o.cholesky);
+                           o.cholesky, o.keep_w);
 }
 
 int rgf_destroy(rgf_handle_t h)
@@ -1655,6 +1722,7 @@ int rgf_selinv_blocks_device(rgf_handle_t h, double* diag_blocks, double* sub_bl
     const SigmaOut so{diag_blocks, sub_blocks, arrow_blocks, corner, h->cur};
     rc = selinv_sweep(h, h->diag, &so);   // eager: the output buffers vary between calls
     if (rc) return dev_status(h, rc, "selinv blocks");
+    h->w_valid = h->Wall != nullptr;
     return RGF_OK;
 }
 
@@ -2240,6 +2308,7 @@ extern "C" int rgf_group_factorize(rgf_group_t g)
                                          "factorized in place already: regenerate the matrix before a group factorize"));
         int rc;
         if (h->nd > 0 && h->nt > 1 && (rc = ensure_solve_buffers(h))) return group_fail(h, rc);   // arrow products use `work`
+        h->w_valid = false;
     }
     int rc = group_run(g, g->gx_factor, g->ep_factor, [](rgf_handle_s* h, size_t) { return factor_sweep(h); });
     if (rc) return group_fail(g->hs[0], dev_status(g->hs[0], rc, "group factorize"));
@@ -2271,6 +2340,7 @@ extern "C" int rgf_group_selinv_diag_device(rgf_group_t g, double* const* diag_d
     int rc = group_run(g, g->gx_selinv, g->ep_selinv,
                        [&out](rgf_handle_s* h, size_t q) { return selinv_sweep(h, out[q]); });
     if (rc) return group_fail(g->hs[0], dev_status(g->hs[0], rc, "group selinv"));
+    for (rgf_handle_s* h : g->hs) h->w_valid = h->Wall != nullptr;
     return RGF_OK;
 }
 

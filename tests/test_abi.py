@@ -33,7 +33,7 @@ def test_defaults():
     o = rgf_amd.default_options()
     assert (o.precision, o.selinv_mode, o.keep_matrix) == (rgf_amd.RGF_FP64, rgf_amd.RGF_SELINV_EXACT, 1)
     assert (o.refine_iters, o.refine_tol, o.graphs) == (10, 1e-14, 0)
-    assert (o.keep_splits, o.cholesky) == (1, rgf_amd.RGF_CHOLESKY_AUTO)
+    assert (o.keep_splits, o.cholesky, o.keep_w) == (1, rgf_amd.RGF_CHOLESKY_AUTO, 0)
     g = rgf_amd.default_gen_params()
     assert (g.sigma_c, g.sigma_a, g.theta_d, g.theta_c) == (0.1, 0.05, 1.0, 1.0)
 
@@ -64,6 +64,10 @@ def test_bad_cholesky_schedule_rejected_without_device():
         with pytest.raises(rgf_amd.RGFError) as e:
             rgf_amd.RGF(ia, ja, np.ones(3), 3, 1, 0, options=rgf_amd.default_options(cholesky=bad))
         assert e.value.code == rgf_amd.RGF_EINVAL and "cholesky schedule" in str(e.value)
+    for bad in (-1, 2):
+        with pytest.raises(rgf_amd.RGFError) as e:
+            rgf_amd.RGF(ia, ja, np.ones(3), 3, 1, 0, options=rgf_amd.default_options(keep_w=bad))
+        assert e.value.code == rgf_amd.RGF_EINVAL and "keep_w" in str(e.value)
 
 
 def test_canonical_flops_closed_form():
@@ -90,6 +94,11 @@ def test_footprint_bytes_host_only():
     tiles = rgf_amd.default_options(keep_matrix=1, cholesky=rgf_amd.RGF_CHOLESKY_TILE_LAUNCHES)
     assert f - rgf_amd.footprint_bytes(500, 10, 4, 1, tiles) == 8 * 512 * 448
     assert rgf_amd.footprint_bytes(500, 10, 4, 8, keep) == rgf_amd.footprint_bytes(500, 10, 4, 8, tiles)
+    # the kept W_i of the selected inversion: nt blocks of r8(ns + nd) x ns doubles per matrix
+    kw = rgf_amd.default_options(keep_matrix=1, keep_w=1)
+    assert rgf_amd.footprint_bytes(500, 10, 4, 2, kw) - rgf_amd.footprint_bytes(500, 10, 4, 2, keep) == 8 * 2 * 10 * 504 * 500
+    mixed_kw = rgf_amd.default_options(keep_matrix=1, precision=rgf_amd.RGF_MIXED_BF16X3, keep_w=1)
+    assert rgf_amd.footprint_bytes(500, 10, 4, 1, mixed_kw) == rgf_amd.footprint_bytes(500, 10, 4, 1, mixed)
     assert rgf_amd.footprint_bytes(0, 10, 4, 1, keep) == 0
     assert rgf_amd.footprint_bytes(5, 10, 6, 1, keep) == 0     # nd > ns
 

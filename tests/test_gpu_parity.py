@@ -1009,3 +1009,65 @@ def test_generated_residual_at_perturbed_x(gpu):
             r = np.linalg.norm(b - A @ xp)
             assert s.residualNorm(xp, b) == pytest.approx(r, rel=1e-12)
             assert s.residualNormNormalized(xp, b) == pytest.approx(r / np.linalg.norm(b), rel=1e-12)
+
+
+# ------------------------------------------------------------------ solves through the kept W_i
+@pytest.mark.parametrize("ns,nt,nd,count", [(42, 3, 0, 1), (65, 4, 5, 1), (130, 3, 8, 1), (200, 6, 6, 1),
+                                            (1000, 5, 4, 1), (130, 4, 3, 3)])
+def test_solve_with_kept_w(gpu, ns, nt, nd, count):
+    """keep_w = 1: after a selected inversion, solves chain x through the kept W_i = [B_i; D_i] X_i
+    (one product per block and direction) -- the same solution as the block solve to rounding,
+    for 1, 3, 16 and 17 right-hand sides, through the oracle on small shapes; after a new
+    factorization (the kept W_i stale) the block solve again, bitwise that of keep_w = 0; HIP
+    graph replays keyed apart for the two sequences."""
+    say(f"kept W solve ns={ns} nt={nt} nd={nd} count={count}")
+    gen = dict(seed=77 + ns, count=count, theta_mode=1 if count > 1 else 0)
+    with rgf_amd.RGF.generated(ns, nt, nd, options=rgf_amd.default_options(), **gen) as h0, \
+         rgf_amd.RGF.generated(ns, nt, nd, options=rgf_amd.default_options(keep_w=1, graphs=1), **gen) as h1:
+        n = h0.n
+        rng = np.random.default_rng(ns)
+        b = rng.standard_normal((n, 17))
+        for mb in range(count):
+            if count > 1:
+                h0.select(mb)
+                h1.select(mb)
+            if mb == 0:   # before any selected inversion: the block solve
+                x_before = h1.solve(None, b[:, :3].copy(), 3)
+                assert np.array_equal(x_before, h0.solve(None, b[:, :3].copy(), 3))
+            h0.RGFdiag()
+            d1 = h1.RGFdiag()
+            assert np.array_equal(d1, h0.RGFdiag())           # keeping W changes no selected-inverse bit
+            for nr in (1, 3, 16, 17):
+                bb = b[:, :nr].copy()
+                xw = h1.solve(None, bb, nr)
+                x0 = h0.solve(None, bb, nr)
+                assert np.abs(xw - x0).max() <= 1e-12 * np.abs(x0).max(), (mb, nr)
+                assert np.array_equal(h1.solve(None, bb, nr), xw)   # graph replay
+                assert h1.residualNormNormalized(xw[:, :1].copy(), bb[:, :1].copy()) <= 1e-13
+            if ns * nt <= 2000 and count == 1:
+                A_mf = oracle.generate(77 + ns, ns, nt, nd)
+                L = oracle.factor(A_mf, ns, nt, nd)
+                xo = oracle.solve(L, ns, nt, nd, b[:, :1])
+                assert np.abs(h1.solve(None, b[:, :1].copy(), 1) - xo).max() <= 1e-10 * np.abs(xo).max()
+        # a new factorization makes the kept W_i stale: the block solve again, bitwise keep_w = 0's
+        h1.factorize()
+        h0.factorize()
+        assert np.array_equal(h1.solve(None, b[:, :1].copy(), 1), h0.solve(None, b[:, :1].copy(), 1))
+
+
+def test_mixed_ignores_keep_w(gpu):
+    """keep_w applies to FP64 handles: a mixed-precision handle with it solves (refined) and
+    inverts bitwise as without it."""
+    say("mixed handle with keep_w")
+    ns, nt, nd = 300, 5, 4
+    b = None
+    out = []
+    for kw in (0, 1):
+        opts = rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3, keep_w=kw)
+        with rgf_amd.RGF.generated(ns, nt, nd, seed=5, options=opts) as h:
+            d = h.RGFdiag()
+            if b is None:
+                b = np.random.default_rng(1).standard_normal((h.n, 16))
+            out.append((d, h.solve(None, b, 16), h.solve(None, b[:, :1].copy(), 1)))
+    for u, v in zip(out[0], out[1]):
+        assert np.array_equal(u, v)
