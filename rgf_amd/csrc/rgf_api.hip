@@ -826,18 +826,22 @@ int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const d
         if (i > 0) {
             const double* Pm = l_panel(h, i - 1, mb);
             if ((rc = skinny_f64s(N, 0, ns + nd, ns, nr, -1.0, Pm + ns, ldp, Y + (long long)(i - 1) * ns, 1, n, 1.0, ri,
-                                  1, n, ri, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nullptr, 0, nd ? xd : nullptr, ns, 64, sb.wide)))
+                                  1, n, ri, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nullptr, 0, nd ? xd : nullptr, ns, 64,
+                                  sb.wide)))
                 return rc;
         }
-        if ((rc = skinny_f64(N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we, ct, sb.wide))) return rc;
+        if ((rc = skinny_f64(N, 1, ns, ns, nr, 1.0, P, ldp, ri, n, 0.0, yi, n, yi, n, w, s, we, ct, sb.wide)))
+            return rc;
     }
     if (nd > 0) {
         const double* P = l_panel(h, nt - 1, mb);   // x_d -= D_{nt-1} y_{nt-1}
         if ((rc = skinny_f64(N, 0, nd, ns, nr, -1.0, P + 2 * ns, ldp, Y + (long long)(nt - 1) * ns, n, 1.0, xd, n, xd,
                              n, w, s, we, ct, sb.wide)))
             return rc;
-        if ((rc = skinny_f64(N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we, ct, sb.wide))) return rc;
-        if ((rc = skinny_f64(T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we, ct, sb.wide))) return rc;
+        if ((rc = skinny_f64(N, 1, nd, nd, nr, 1.0, Xdd, nd, xd, n, 0.0, yd, n, yd, n, w, s, we, ct, sb.wide)))
+            return rc;
+        if ((rc = skinny_f64(T, 2, nd, nd, nr, 1.0, Xdd, nd, yd, n, 0.0, xd, n, xd, n, w, s, we, ct, sb.wide)))
+            return rc;
     }
     // backward: y_i -= [B_i; D_i]^T [x_{i+1}; x_d] (one product over both row groups of the
     // panel), x_i = X_i^T y_i
@@ -847,13 +851,15 @@ int solve_block(rgf_handle_s* h, const SolveBufs& sb, int mb, double* x, const d
         double* xi = x + (long long)i * ns;
         if (i < nt - 1) {
             if ((rc = skinny_f64s(T, 0, ns, ns + nd, nr, -1.0, P + ns, ldp, x + (long long)(i + 1) * ns, 1, n, 1.0,
-                                  yi, 1, n, yi, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns, nullptr, 0, 64, sb.wide)))
+                                  yi, 1, n, yi, 1, n, w, we, s, 1, 0, 0, 0, 0, ct, nd ? xd : nullptr, ns, nullptr, 0,
+                                  64, sb.wide)))
                 return rc;
-        } else if (nd > 0 &&
-                   (rc = skinny_f64(T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s, we, ct, sb.wide))) {
+        } else if (nd > 0 && (rc = skinny_f64(T, 0, ns, nd, nr, -1.0, P + 2 * ns, ldp, xd, n, 1.0, yi, n, yi, n, w, s,
+                                              we, ct, sb.wide))) {
             return rc;
         }
-        if ((rc = skinny_f64(T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we, ct, sb.wide))) return rc;
+        if ((rc = skinny_f64(T, 2, ns, ns, nr, 1.0, P, ldp, yi, n, 0.0, xi, n, xi, n, w, s, we, ct, sb.wide)))
+            return rc;
     }
     return 0;
 }
