@@ -41,6 +41,12 @@ int rgf_debug_bench_potrf(int n, int reps, double *ms_potrf, double *ms_tile);
  * *nrec receives the number of tasks traced and *ticks_per_us the clock rate. */
 int rgf_debug_trace_potrf(int n, long long *out, long long cap, long long *nrec, double *ticks_per_us);
 
+/* The claim order of the task-graph Cholesky's workers for T tile rows (with_x: the L^-1 tasks
+ * too; xlag, promo: the order's two parameters, as tuning keys 20 and 21 set them): task t as
+ * out[5 t ..] = kind (1 TRSM, 2 UPDATE, 6 XACC), i, j, p, generation -- the same decoding the
+ * kernel's workers run.  Returns the task count (at most cap tasks written); host only. */
+int rgf_debug_dag_order(int T, int with_x, int xlag, int promo, int *out, int cap);
+
 /* Process-wide kernel variant switches, for measurements of whole sweeps: key 1 = k per
  * LDS stage of the pre-split split-BF16 product (32 or 64); key 3 = its LDS row layout
  * (0 padded rows, 1 unpadded rows with XOR-swizzled 16-byte chunks); key 4 = FP64 GEMM loads of

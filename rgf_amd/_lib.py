@@ -37,7 +37,7 @@ EXPORTS = [
 ]
 # include/rgf_testing.h
 TEST_EXPORTS = ["rgf_debug_gemm", "rgf_debug_potrf", "rgf_debug_bench_gemm", "rgf_debug_bench_potrf",
-                "rgf_debug_trace_potrf",
+                "rgf_debug_trace_potrf", "rgf_debug_dag_order",
                 "rgf_debug_cache_stats", "rgf_debug_set_tuning"]
 
 
@@ -148,6 +148,8 @@ def load():
         lp = C.POINTER(C.c_longlong)
         lib.rgf_debug_trace_potrf.argtypes = [C.c_int, lp, C.c_longlong, lp, dp]
         lib.rgf_debug_trace_potrf.restype = C.c_int
+        lib.rgf_debug_dag_order.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]
+        lib.rgf_debug_dag_order.restype = C.c_int
         lib.rgf_debug_cache_stats.argtypes = [lp, lp, lp, dp]
         lib.rgf_debug_cache_stats.restype = C.c_int
         lib.rgf_debug_set_tuning.argtypes = [C.c_int, C.c_int]

@@ -1970,6 +1970,12 @@ extern "C" int rgf_debug_bench_gemm(int ta, int tb, int m, int n, int k, int fla
     return RGF_OK;
 }
 
+extern "C" int rgf_debug_dag_order(int T, int with_x, int xlag, int promo, int* out, int cap)
+{
+    if (T < 1 || T > 4096 || xlag < 0 || xlag > 16 || cap < 0 || (cap > 0 && !out)) return -1;
+    return potrf_dag_order(T, with_x != 0, xlag, promo != 0, out, cap);
+}
+
 extern "C" int rgf_debug_set_tuning(int key, int value)
 {
     if (key == 1 && (value == 32 || value == 64)) {

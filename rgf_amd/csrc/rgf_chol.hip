@@ -547,6 +547,37 @@ __host__ __device__ inline int dag_tasks(int T, bool with_x, int xlag, bool prom
     return total;
 }
 
+// Task t of the claim order: kind 1 TRSM(i, g) (j = g), 2 UPDATE(i, j, p), 6 XACC(i, j, k = p);
+// g its generation
+struct DagTask {
+    int kind, i, j, p, g;
+};
+__host__ __device__ inline DagTask dag_decode(int T, bool with_x, int xlag, bool promo, int t)
+{
+    int g = 0;
+    DagGen dg = dag_gen(T, g, with_x, xlag, promo);
+    while (t >= dg.total()) {
+        t -= dg.total();
+        dg = dag_gen(T, ++g, with_x, xlag, promo);
+    }
+    const int ntr = dg.ntr, nupg = dg.nup - dg.npo;
+    if (t >= ntr + nupg + dg.npi) {   // XACC(i, j, k = g - 2 - xlag)
+        t -= ntr + nupg + dg.npi;
+        const int k = g - 2 - xlag;
+        return {6, k + 1 + t / (k + 1), t % (k + 1), k, g};
+    }
+    if (t < ntr) return {1, g + 2 + t, g, g, g};   // TRSM(i, g)
+    t -= ntr;
+    if (t >= nupg) return {2, g + 2 + (t - nupg), g + 2, g, g};   // UPDATE promoted into g (p = g)
+    t += dg.npo;   // UPDATE(i, j, g - 1), column j ascending, rows i >= j
+    int j = g + 1;
+    while (t >= T - j) {
+        t -= T - j;
+        j++;
+    }
+    return {2, j + t, j, g - 1, g};
+}
+
 struct DagArgs {
     double* A;
     long long sA;
@@ -894,17 +925,11 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
             int t = s_task;
             if (t < 0 || t >= a.ntasks) break;
             const long long w0 = wall_clock64();
-            int g = 0;
-            DagGen dg = dag_gen(T, g, with_x, a.xlag, a.promo);
-            while (t >= dg.total()) {
-                t -= dg.total();
-                dg = dag_gen(T, ++g, with_x, a.xlag, a.promo);
-            }
-            const int ntr = dg.ntr, nupg = dg.nup - dg.npo;
-            if (t >= ntr + nupg + dg.npi) {
-                // XACC(i, j, k = g - 2 - xlag)
-                t -= ntr + nupg + dg.npi;
-                const int k = g - 2 - a.xlag, i = k + 1 + t / (k + 1), j = t % (k + 1);
+            const DagTask tk = dag_decode(T, with_x, a.xlag, a.promo, t);
+            const int g = tk.g;
+            if (tk.kind == 6) {
+                // XACC(i, j, k)
+                const int k = tk.p, i = tk.i, j = tk.j;
                 const int ri = min(NB, n - NB * i);
                 if (!dag_acquire(a, fl, info, ldone + i * T + k, 1, k == j ? xdone + j : xrdy + k * T + j, 1,
                                  k > j ? acnt + i * T + j : nullptr, k - j, &s_ok, tid))
@@ -934,9 +959,9 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
                     dag_release(xrdy + i * T + j, 1, tid);
                 }
                 if (tid == 0) dag_trace(a, 6, i, j, g, w0, w1, wall_clock64());
-            } else if (t < ntr) {
+            } else if (tk.kind == 1) {
                 // TRSM(i, g)
-                const int i = g + 2 + t, rows = min(NB, n - NB * i);
+                const int i = tk.i, rows = min(NB, n - NB * i);
                 if (!dag_acquire(a, fl, info, xdone + g, 1, g >= 1 ? ldone + i * T + (g - 1) : nullptr, 1,
                                  g >= 2 ? cnt + i * T + g : nullptr, g - 1, &s_ok, tid))
                     break;
@@ -962,23 +987,7 @@ __global__ __launch_bounds__(DNT) void potrf_dag_kernel(DagArgs a)
                 if (tid == 0) dag_trace(a, 1, i, g, g, w0, w1, wall_clock64());
             } else {
                 // UPDATE(i, j, p): of generation g (p = g - 1), or promoted into it (p = g)
-                t -= ntr;
-                int i, j, p;
-                if (t >= nupg) {
-                    t -= nupg;
-                    i = g + 2 + t;
-                    j = g + 2;
-                    p = g;
-                } else {
-                    t += dg.npo;
-                    j = g + 1;
-                    while (t >= T - j) {
-                        t -= T - j;
-                        j++;
-                    }
-                    i = j + t;
-                    p = g - 1;
-                }
+                const int i = tk.i, j = tk.j, p = tk.p;
                 const int ri = min(NB, n - NB * i), rj = min(NB, n - NB * j);
                 if (!dag_acquire(a, fl, info, ldone + i * T + p, 1, j != i ? ldone + j * T + p : nullptr, 1,
                                  p >= 1 ? cnt + i * T + j : nullptr, p, &s_ok, tid))
@@ -1019,6 +1028,17 @@ int g_chol_dag_xlag = 2;    // generations the inverse's tasks lag in the claim 
 int g_chol_dag_promo = 1;   // 1: the chain's two updates per step are claimed a generation early
 
 size_t potrf_dag_flag_elems(int n) { return (size_t)dag_flag_elems((n + NB - 1) / NB); }
+
+int potrf_dag_order(int T, bool with_x, int xlag, bool promo, int* out, int cap)
+{
+    const int n = dag_tasks(T, with_x, xlag, promo);
+    for (int t = 0; t < n && t < cap; t++) {
+        const DagTask k = dag_decode(T, with_x, xlag, promo, t);
+        const int v[5] = {k.kind, k.i, k.j, k.p, k.g};
+        for (int q = 0; q < 5; q++) out[5 * t + q] = v[q];
+    }
+    return n;
+}
 size_t potrf_dag_xbuf_elems(int n)
 {
     const size_t T = (size_t)(n + NB - 1) / NB;

@@ -262,6 +262,8 @@ int potrf_f64(double* A, int lda, long long sA, int n, int count, double* logdet
               double* xbuf = nullptr, long long sXb = 0, bool* x_done = nullptr, int schedule = 0);
 bool potrf_dag_schedule(int schedule, int count);
 size_t potrf_dag_flag_elems(int n);
+// the workers' claim order for T tile rows (5 ints per task); returns the task count
+int potrf_dag_order(int T, bool with_x, int xlag, bool promo, int* out, int cap);
 size_t potrf_dag_xbuf_elems(int n);
 bool potrf_uses_dag(int count);   // a launch over `count` matrices takes the task-graph path
 extern int g_chol_dag, g_chol_dag_wgs, g_chol_dag_min, g_chol_dag_x, g_chol_dag_xlag, g_chol_dag_promo;
