@@ -61,8 +61,11 @@ def parse():
                    help="run only the batched measurement of one precision (for profiling)")
     p.add_argument("--profile-step", action="store_true",
                    help="only generate the matrix and run one step (for profiler counter runs)")
+    p.add_argument("--profile-solve", action="store_true",
+                   help="only generate the matrix, factorize it and run three 16-RHS solves (the block solve; "
+                        "refined with --profile-precision mixed), for profiler runs")
     p.add_argument("--profile-precision", choices=["fp64", "mixed"], default="fp64",
-                   help="precision of the --profile-step run")
+                   help="precision of the --profile-step / --profile-solve run")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB three times (scale for the HBM "
                         "byte counters FETCH_SIZE / WRITE_SIZE of a profiler run)")
@@ -297,11 +300,12 @@ def main():
     # same steps replayed from HIP graphs are timed separately (graph_replay), and the
     # mixed and batched measurements replay graphs (captured in their untimed warm-up).
     graphs = 1 if args.warmup > 0 else 0
-    main_graphs = graphs if (args.no_kernel_timing and not args.profile_step) else 0
+    main_graphs = graphs if (args.no_kernel_timing and not (args.profile_step or args.profile_solve)) else 0
     t_gen = time.perf_counter()
-    if args.profile_step and args.profile_precision == "mixed":
+    if (args.profile_step or args.profile_solve) and args.profile_precision == "mixed":
         s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
-                                  options=rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3))
+                                  options=rgf_amd.default_options(precision=rgf_amd.RGF_MIXED_BF16X3,
+                                                                  refine_tol=1e-13))
     else:
         # keep_w: the selected inversion's W_i stay for the solves after it (no extra work in the step)
         s = rgf_amd.RGF.generated(ns, nt, nd, seed=args.seed,
@@ -315,6 +319,25 @@ def main():
         ld = s.logDet()
         s.selinv_diag_device(diag)
         return ld
+
+    if args.profile_solve:   # factorize, then 16-RHS solves (the solve after a new factorization)
+        s.factorize()
+        bsolve = rgf_amd.DeviceBuffer(s.n * 16)
+        bsolve.upload(np.random.default_rng(args.seed).standard_normal(s.n * 16))
+        xsolve = rgf_amd.DeviceBuffer(s.n * 16)
+        times = []
+        for _ in range(3):
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.solve_device(xsolve, bsolve, 16)
+            s.synchronize()
+            times.append(round((time.perf_counter() - t0) * 1e3, 3))
+        out = {"precision": args.profile_precision, "solve_16rhs_ms": times}
+        if args.profile_precision == "mixed":
+            out["refine_steps"], out["refine_rel_residual"] = s.refine_stats()
+        print(json.dumps(out), flush=True)
+        s.close()
+        return
 
     if args.profile_step:   # one step, NN class timed with HIP events (compare with a profiler)
         rgf_amd.kernel_timing_enable(0 if args.profile_precision == "fp64" else 4)
